@@ -1,0 +1,377 @@
+// hn_compact.hip -- single-pass stream compaction and exclusive scan (gfx950).
+//
+// Both are one kernel over tiles with decoupled look-back: a tile takes a
+// ticket (so every tile it waits on has already started), publishes its own
+// aggregate, then walks back over its predecessors 64 at a time with one
+// wave until it meets an inclusive prefix, and publishes its own inclusive
+// prefix.  The input is read once and the output written once, against the
+// reference's count pass + Blelloch scan + scatter pass with a host sync
+// between kernels (split_tools.h:920-989, 363-418).
+//
+// Compaction is stable: tile-local ranks come from wave ballots over a
+// striped layout (element j*256 + t of the tile is item j of thread t), so
+// ranks follow input order.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "hn_device.hpp"
+#include "hn_host.hpp"
+
+namespace hn {
+
+constexpr int kBS = 256;         // threads per tile
+constexpr int kWaves = kBS / kWave;
+constexpr int kCompactIPT = 8;   // elements per thread in a compaction tile
+constexpr int kCompactTile = kBS * kCompactIPT;
+constexpr uint32_t kFlagAgg = 1, kFlagInc = 2;
+
+struct Lookback {
+   uint32_t* ticket;
+   uint32_t* flags;    // [tiles]
+   uint64_t* agg;      // [tiles]
+   uint64_t* inc;      // [tiles]
+};
+
+inline size_t lookback_bytes(uint64_t tiles) { return 256 + tiles * (4 + 8 + 8) + 64; }
+
+inline Lookback lookback_layout(void* scratch, uint64_t tiles) {
+   char* p = static_cast<char*>(scratch);
+   Lookback L;
+   L.ticket = reinterpret_cast<uint32_t*>(p);
+   L.agg = reinterpret_cast<uint64_t*>(p + 256);
+   L.inc = L.agg + tiles;
+   L.flags = reinterpret_cast<uint32_t*>(L.inc + tiles);
+   return L;
+}
+
+__global__ __launch_bounds__(256) void k_lookback_init(Lookback L, uint64_t tiles) {
+   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < tiles; i += stride)
+      L.flags[i] = 0;
+   if (blockIdx.x == 0 && threadIdx.x == 0) *L.ticket = 0;
+}
+
+__global__ void k_store_u64(uint64_t* p, uint64_t v) { *p = v; }
+
+// Exclusive prefix of tile `tile` whose own aggregate is `mine`; called by
+// one full wave.  Publishes the tile's aggregate and inclusive prefix.
+__device__ uint64_t lookback_prefix(const Lookback& L, uint32_t tile, uint64_t mine) {
+   const int lane = __lane_id();
+   if (tile == 0) {
+      if (lane == 0) {
+         L.inc[0] = mine;
+         __hip_atomic_store(&L.flags[0], kFlagInc, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      return 0;
+   }
+   if (lane == 0) {
+      L.agg[tile] = mine;
+      __hip_atomic_store(&L.flags[tile], kFlagAgg, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+   }
+   uint64_t prefix = 0;
+   int64_t hi = static_cast<int64_t>(tile) - 1; // newest predecessor not yet folded in
+   for (;;) {
+      const int64_t p = hi - lane;
+      uint32_t f = kFlagInc; // lanes past tile 0 behave as an inclusive zero
+      uint64_t v = 0;
+      if (p >= 0) {
+         f = __hip_atomic_load(&L.flags[p], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+         if (f == kFlagInc) v = L.inc[p];
+         else if (f == kFlagAgg) v = L.agg[p];
+      }
+      const uint64_t not_ready = __ballot(f == 0);
+      const uint64_t inc_mask = __ballot(f == kFlagInc);
+      // Lanes up to and including the first inclusive one, if none before it is still pending.
+      const int first_inc = inc_mask ? first_bit(inc_mask) : 64;
+      const uint64_t upto = first_inc >= 63 ? ~0ull : ((2ull << first_inc) - 1ull);
+      if (not_ready & upto) {
+         __builtin_amdgcn_s_sleep(1);
+         continue;
+      }
+      const uint64_t take = (lane <= first_inc) ? v : 0;
+      prefix += wave_sum(take);
+      if (first_inc < 64) break;
+      hi -= 64;
+   }
+   if (lane == 0) {
+      L.inc[tile] = prefix + mine;
+      __hip_atomic_store(&L.flags[tile], kFlagInc, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+   }
+   return prefix;
+}
+
+// ------------------------------------------------------------ compaction
+
+template <typename K>
+struct PairSel {
+   hn_pred p;
+   __device__ bool operator()(uint64_t, const Slot<K>& s) const {
+      const K f = p.field == 0 ? s.key : s.val;
+      switch (p.kind) {
+      case HN_PRED_ALL: return true;
+      case HN_PRED_VALID: return s.key != empty_key<K>() && s.key != tomb_key<K>();
+      case HN_PRED_MASKEQ: return (f & static_cast<K>(p.mask)) == static_cast<K>(p.value);
+      case HN_PRED_LT: return f < static_cast<K>(p.value);
+      case HN_PRED_GE: return f >= static_cast<K>(p.value);
+      default: return false;
+      }
+   }
+};
+
+template <typename T>
+struct FlagSel {
+   const uint8_t* flags;
+   __device__ bool operator()(uint64_t i, const T&) const { return flags[i] != 0; }
+};
+
+template <typename T>
+struct Whole {
+   __device__ T operator()(const T& x) const { return x; }
+};
+template <typename K>
+struct KeyOf {
+   __device__ K operator()(const Slot<K>& s) const { return s.key; }
+};
+
+template <typename In, typename Out, class Sel, class Proj>
+__global__ __launch_bounds__(kBS) void k_compact(const In* __restrict__ in, uint64_t n, Out* __restrict__ out,
+                                                 uint64_t* d_count, Lookback L, uint64_t tiles, Sel sel, Proj proj) {
+   __shared__ uint32_t s_tile;
+   __shared__ uint32_t s_cnt[kCompactIPT][kWaves];
+   __shared__ uint64_t s_prefix;
+   const int lane = __lane_id();
+   const int wave = threadIdx.x / kWave;
+   if (threadIdx.x == 0) s_tile = __hip_atomic_fetch_add(L.ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+   __syncthreads();
+   const uint32_t tile = s_tile;
+   const uint64_t base = static_cast<uint64_t>(tile) * kCompactTile;
+
+   In v[kCompactIPT];
+   uint64_t bal[kCompactIPT];
+#pragma unroll
+   for (int j = 0; j < kCompactIPT; ++j) {
+      const uint64_t i = base + static_cast<uint64_t>(j) * kBS + threadIdx.x;
+      bool keep = false;
+      if (i < n) {
+         v[j] = in[i];
+         keep = sel(i, v[j]);
+      }
+      bal[j] = __ballot(keep);
+      if (lane == 0) s_cnt[j][wave] = static_cast<uint32_t>(__popcll(bal[j]));
+   }
+   __syncthreads();
+   if (threadIdx.x == 0) { // j-major, wave-minor exclusive prefix = input order
+      uint32_t run = 0;
+      for (int j = 0; j < kCompactIPT; ++j)
+         for (int w = 0; w < kWaves; ++w) {
+            const uint32_t c = s_cnt[j][w];
+            s_cnt[j][w] = run;
+            run += c;
+         }
+      s_prefix = run;
+   }
+   __syncthreads();
+   const uint64_t total = s_prefix;
+   __syncthreads();
+   if (wave == 0) {
+      const uint64_t pre = lookback_prefix(L, tile, total);
+      if (lane == 0) {
+         s_prefix = pre;
+         if (tile == tiles - 1 && d_count) *d_count = pre + total;
+      }
+   }
+   __syncthreads();
+   const uint64_t pre = s_prefix;
+   const uint64_t below = (1ull << lane) - 1ull;
+#pragma unroll
+   for (int j = 0; j < kCompactIPT; ++j) {
+      if ((bal[j] >> lane) & 1ull) {
+         const uint64_t r = pre + s_cnt[j][wave] + static_cast<uint64_t>(__popcll(bal[j] & below));
+         out[r] = proj(v[j]);
+      }
+   }
+}
+
+// ------------------------------------------------------------ scan
+
+template <typename T>
+__device__ __forceinline__ T wave_incl_scan(T x) {
+   for (int o = 1; o < kWave; o <<= 1) {
+      T y = __shfl_up(x, o, kWave);
+      if (__lane_id() >= o) x += y;
+   }
+   return x;
+}
+// u32: DPP row shifts inside 16-lane rows, then row broadcasts across rows.
+template <>
+__device__ __forceinline__ uint32_t wave_incl_scan<uint32_t>(uint32_t x) {
+   int v = static_cast<int>(x);
+   v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false); // row_shr:1
+   v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false); // row_shr:2
+   v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false); // row_shr:4
+   v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false); // row_shr:8
+   v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false); // row_bcast:15 -> rows 1, 3
+   v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false); // row_bcast:31 -> rows 2, 3
+   return static_cast<uint32_t>(v);
+}
+
+// Each thread owns IPT consecutive elements (64 bytes); a tile is kBS*IPT.
+template <typename T, bool VEC>
+__global__ __launch_bounds__(kBS) void k_scan(const T* in, T* out, uint64_t n, Lookback L, uint64_t tiles) {
+   constexpr int IPT = 64 / sizeof(T);
+   constexpr uint64_t TILE = static_cast<uint64_t>(kBS) * IPT;
+   __shared__ uint32_t s_tile;
+   __shared__ T s_wave[kWaves];
+   __shared__ uint64_t s_prefix;
+   const int lane = __lane_id();
+   const int wave = threadIdx.x / kWave;
+   if (threadIdx.x == 0) s_tile = __hip_atomic_fetch_add(L.ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+   __syncthreads();
+   const uint32_t tile = s_tile;
+   const uint64_t i0 = static_cast<uint64_t>(tile) * TILE + static_cast<uint64_t>(threadIdx.x) * IPT;
+
+   T x[IPT];
+   if (VEC && i0 + IPT <= n) {
+      const uint4* src = reinterpret_cast<const uint4*>(in + i0);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) reinterpret_cast<uint4*>(x)[c] = src[c];
+   } else {
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) x[j] = (i0 + j < n) ? in[i0 + j] : T(0);
+   }
+   T sum = 0;
+#pragma unroll
+   for (int j = 0; j < IPT; ++j) sum += x[j];
+   const T incl = wave_incl_scan<T>(sum);
+   if (lane == kWave - 1) s_wave[wave] = incl;
+   __syncthreads();
+   T wave_pre = 0, tile_total = 0;
+#pragma unroll
+   for (int w = 0; w < kWaves; ++w) {
+      const T c = s_wave[w];
+      if (w < wave) wave_pre += c;
+      tile_total += c;
+   }
+   if (wave == 0) {
+      const uint64_t pre = lookback_prefix(L, tile, static_cast<uint64_t>(tile_total));
+      if (lane == 0) s_prefix = pre;
+   }
+   __syncthreads();
+   T run = static_cast<T>(s_prefix) + wave_pre + (incl - sum);
+#pragma unroll
+   for (int j = 0; j < IPT; ++j) {
+      const T e = x[j];
+      x[j] = run;
+      run += e;
+   }
+   if (VEC && i0 + IPT <= n) {
+      uint4* dst = reinterpret_cast<uint4*>(out + i0);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) dst[c] = reinterpret_cast<uint4*>(x)[c];
+   } else {
+#pragma unroll
+      for (int j = 0; j < IPT; ++j)
+         if (i0 + j < n) out[i0 + j] = x[j];
+   }
+}
+
+// ------------------------------------------------------------ host side
+
+inline uint64_t compact_tiles(uint64_t n) { return n ? (n + kCompactTile - 1) / kCompactTile : 0; }
+
+template <typename In, typename Out, class Sel, class Proj>
+int run_compact(const In* in, uint64_t n, Out* out, uint64_t* d_count, void* scratch, size_t bytes, hipStream_t s,
+                Sel sel, Proj proj) {
+   if ((n && (!in || !out)) || !d_count) return HN_EINVAL;
+   if (n == 0) {
+      hipLaunchKernelGGL(k_store_u64, dim3(1), dim3(1), 0, s, d_count, 0ull); // nothing selected
+      return static_cast<int>(hipGetLastError());
+   }
+   const uint64_t tiles = compact_tiles(n);
+   if (!scratch || bytes < lookback_bytes(tiles)) return HN_ESCRATCH;
+   Lookback L = lookback_layout(scratch, tiles);
+   hipLaunchKernelGGL(k_lookback_init, dim3(stream_blocks(tiles, 256)), dim3(256), 0, s, L, tiles);
+   hipLaunchKernelGGL((k_compact<In, Out, Sel, Proj>), dim3(static_cast<uint32_t>(tiles)), dim3(kBS), 0, s, in, n,
+                      out, d_count, L, tiles, sel, proj);
+   return static_cast<int>(hipGetLastError());
+}
+
+template <typename T>
+int run_scan(const T* in, T* out, uint64_t n, void* scratch, size_t bytes, hipStream_t s) {
+   if (n && (!in || !out)) return HN_EINVAL;
+   if (n == 0) return 0;
+   constexpr uint64_t TILE = static_cast<uint64_t>(kBS) * (64 / sizeof(T));
+   const uint64_t tiles = (n + TILE - 1) / TILE;
+   if (!scratch || bytes < lookback_bytes(tiles)) return HN_ESCRATCH;
+   Lookback L = lookback_layout(scratch, tiles);
+   hipLaunchKernelGGL(k_lookback_init, dim3(stream_blocks(tiles, 256)), dim3(256), 0, s, L, tiles);
+   const bool vec = (reinterpret_cast<uintptr_t>(in) % 16 == 0) && (reinterpret_cast<uintptr_t>(out) % 16 == 0);
+   if (vec)
+      hipLaunchKernelGGL((k_scan<T, true>), dim3(static_cast<uint32_t>(tiles)), dim3(kBS), 0, s, in, out, n, L, tiles);
+   else
+      hipLaunchKernelGGL((k_scan<T, false>), dim3(static_cast<uint32_t>(tiles)), dim3(kBS), 0, s, in, out, n, L, tiles);
+   return static_cast<int>(hipGetLastError());
+}
+
+} // namespace hn
+
+using namespace hn;
+
+extern "C" {
+
+size_t hn_compact_scratch_bytes(size_t n) {
+   // The compaction tile is the smaller one, so it bounds both kernels.
+   return lookback_bytes(compact_tiles(n) + 1);
+}
+
+int hn_copy_if_pairs_u32(const void* in, size_t n, const hn_pred* p, void* out, uint64_t* c, void* sc, size_t sb,
+                         hn_stream_t s) {
+   if (!p) return HN_EINVAL;
+   return run_compact(static_cast<const Slot<uint32_t>*>(in), n, static_cast<Slot<uint32_t>*>(out), c, sc, sb, s,
+                      PairSel<uint32_t>{*p}, Whole<Slot<uint32_t>>{});
+}
+int hn_copy_if_pairs_u64(const void* in, size_t n, const hn_pred* p, void* out, uint64_t* c, void* sc, size_t sb,
+                         hn_stream_t s) {
+   if (!p) return HN_EINVAL;
+   return run_compact(static_cast<const Slot<uint64_t>*>(in), n, static_cast<Slot<uint64_t>*>(out), c, sc, sb, s,
+                      PairSel<uint64_t>{*p}, Whole<Slot<uint64_t>>{});
+}
+int hn_copy_keys_if_pairs_u32(const void* in, size_t n, const hn_pred* p, uint32_t* out, uint64_t* c, void* sc,
+                              size_t sb, hn_stream_t s) {
+   if (!p) return HN_EINVAL;
+   return run_compact(static_cast<const Slot<uint32_t>*>(in), n, out, c, sc, sb, s, PairSel<uint32_t>{*p},
+                      KeyOf<uint32_t>{});
+}
+int hn_copy_keys_if_pairs_u64(const void* in, size_t n, const hn_pred* p, uint64_t* out, uint64_t* c, void* sc,
+                              size_t sb, hn_stream_t s) {
+   if (!p) return HN_EINVAL;
+   return run_compact(static_cast<const Slot<uint64_t>*>(in), n, out, c, sc, sb, s, PairSel<uint64_t>{*p},
+                      KeyOf<uint64_t>{});
+}
+
+int hn_copy_if_flags(const void* in, const uint8_t* flags, size_t n, size_t eb, void* out, uint64_t* c, void* sc,
+                     size_t sb, hn_stream_t s) {
+   if (n && !flags) return HN_EINVAL;
+   switch (eb) {
+   case 4:
+      return run_compact(static_cast<const uint32_t*>(in), n, static_cast<uint32_t*>(out), c, sc, sb, s,
+                         FlagSel<uint32_t>{flags}, Whole<uint32_t>{});
+   case 8:
+      return run_compact(static_cast<const uint64_t*>(in), n, static_cast<uint64_t*>(out), c, sc, sb, s,
+                         FlagSel<uint64_t>{flags}, Whole<uint64_t>{});
+   case 16:
+      return run_compact(static_cast<const Slot<uint64_t>*>(in), n, static_cast<Slot<uint64_t>*>(out), c, sc, sb, s,
+                         FlagSel<Slot<uint64_t>>{flags}, Whole<Slot<uint64_t>>{});
+   default: return HN_EINVAL;
+   }
+}
+
+int hn_exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t n, void* sc, size_t sb, hn_stream_t s) {
+   return run_scan<uint32_t>(in, out, n, sc, sb, s);
+}
+int hn_exclusive_scan_u64(const uint64_t* in, uint64_t* out, size_t n, void* sc, size_t sb, hn_stream_t s) {
+   return run_scan<uint64_t>(in, out, n, sc, sb, s);
+}
+
+} // extern "C"

@@ -1,0 +1,58 @@
+// hn_host.hpp -- host-side launch geometry shared by the C-ABI entry points.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "hashinator_amd.h"
+
+namespace hn {
+
+enum OpKind { OP_INSERT = 0, OP_RETRIEVE = 1, OP_ERASE = 2, OP_REINSERT = 3 };
+
+struct Launch {
+   int probe_width;
+   int block_size;
+   uint32_t blocks;
+};
+
+// Compute units of the current device (cached per device ordinal).
+int cu_count();
+
+// Default probe width: one 64-byte window per step (4 slots of 16 B for
+// u64 maps, 8 slots of 8 B for u32 maps).
+template <typename K>
+constexpr int default_probe_width(OpKind) {
+   return sizeof(K) == 8 ? 4 : 8;
+}
+
+constexpr int kDefaultBlock = 256;
+constexpr int kResidentBlocksPerCU = 8; // 8 x 256 threads = 32 waves = one full CU
+
+template <typename K>
+int resolve_launch(const hn_tune* t, OpKind op, uint64_t len, Launch* L) {
+   int G = (t && t->probe_width) ? t->probe_width : default_probe_width<K>(op);
+   int bs = (t && t->block_size) ? t->block_size : kDefaultBlock;
+   if (G < 1 || G > 64 || (G & (G - 1))) return HN_EINVAL;
+   if (bs < 64 || bs > 1024 || (bs % 64)) return HN_EINVAL;
+   const uint64_t per_block = static_cast<uint64_t>(bs / G);
+   uint64_t need = (len + per_block - 1) / per_block;
+   if (need == 0) need = 1;
+   uint64_t cap = (t && t->grid_blocks > 0) ? static_cast<uint64_t>(t->grid_blocks)
+                                            : static_cast<uint64_t>(cu_count()) * kResidentBlocksPerCU * (1024 / bs) / 4;
+   if (cap == 0) cap = 1;
+   if (cap > (1u << 20)) cap = 1u << 20;
+   L->probe_width = G;
+   L->block_size = bs;
+   L->blocks = static_cast<uint32_t>(need < cap ? need : cap);
+   return 0;
+}
+
+// Grid for a grid-stride streaming kernel over n items.
+inline uint32_t stream_blocks(uint64_t n, int bs) {
+   uint64_t need = (n + bs - 1) / bs;
+   const uint64_t cap = static_cast<uint64_t>(cu_count()) * 32;
+   if (need == 0) need = 1;
+   return static_cast<uint32_t>(need < cap ? need : cap);
+}
+
+} // namespace hn

@@ -1,0 +1,318 @@
+"""Host-side mirror of Hashinator's accelerated map API over the C-ABI.
+
+``Hashmap`` follows ``Hashinator::Hashmap<K,V>`` (kstppd/hashinator
+include/hashinator/hashinator.h:59-1802): same method names, argument meaning
+and synchronous behaviour (every accelerated call ends with a stream
+synchronize, hashers.h:55-161).  Device memory is torch tensors; torch is the
+allocator and stream source only, every operation runs the gfx950 kernels of
+libhashinator_amd.so.
+
+Keys and values are unsigned integers of 32 or 64 bits; torch tensors of
+int32/int64 (or uint32/uint64) carry their bit patterns.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, hn_info, hn_pred, hn_tune
+
+EMPTY = {32: 0xFFFFFFFF, 64: 0xFFFFFFFFFFFFFFFF}
+TOMBSTONE = {32: 0xFFFFFFFE, 64: 0xFFFFFFFFFFFFFFFE}
+_TDT = {32: torch.int32, 64: torch.int64}
+_NDT = {32: np.uint32, 64: np.uint64}
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _stream(stream=None):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+def as_device(a, key_bits, device="cuda"):
+    """numpy / torch / list of unsigned integers -> contiguous device tensor of bit patterns."""
+    if isinstance(a, torch.Tensor):
+        t = a
+        if t.dtype in (torch.uint32, torch.uint64):
+            t = t.view(_TDT[key_bits])
+        if t.dtype != _TDT[key_bits]:
+            raise TypeError(f"expected {key_bits}-bit integer tensor, got {t.dtype}")
+        return t.to(device).contiguous()
+    arr = np.ascontiguousarray(np.asarray(a, dtype=_NDT[key_bits]))
+    return torch.from_numpy(arr.view(np.int32 if key_bits == 32 else np.int64)).to(device)
+
+
+def to_numpy(t, key_bits):
+    """device tensor of bit patterns -> numpy unsigned array."""
+    return t.detach().cpu().contiguous().numpy().view(_NDT[key_bits])
+
+
+class targets:
+    """Hashinator::targets (common.h:90)."""
+
+    host, device, automatic = 0, 1, 2
+
+
+class status:
+    """Hashinator::status (common.h:85)."""
+
+    success, fail, invalid = 0, 1, 2
+
+
+class Hashmap:
+    """Hashinator::Hashmap<K,V> with K = V = uint32_t or uint64_t, accelerated API."""
+
+    def __init__(self, sizePower: int = 5, key_bits: int = 32, device="cuda", tune: hn_tune | None = None):
+        if key_bits not in (32, 64):
+            raise ValueError("key_bits must be 32 or 64")
+        self.kb = key_bits
+        self.device = torch.device(device)
+        self.tune = tune
+        self._L = _lib.load()
+        self._sfx = "u32" if key_bits == 32 else "u64"
+        self._sp = int(sizePower)
+        self.buckets = self._new_table(self._sp)
+        self._info = torch.zeros(5, dtype=torch.int64, device=self.device)  # 40 bytes == Info
+        check(self._L.hn_info_init(_ptr(self._info), self._sp, _stream()), "hn_info_init")
+        self._sync()
+
+    # ----------------------------------------------------------- plumbing
+    def _fn(self, name):
+        return getattr(self._L, f"hn_{name}_{self._sfx}")
+
+    def _tune(self):
+        return C.byref(self.tune) if self.tune is not None else None
+
+    def _new_table(self, sp):
+        t = torch.empty((1 << sp, 2), dtype=_TDT[self.kb], device=self.device)
+        check(self._fn("table_init")(_ptr(t), 1 << sp, _stream()), "hn_table_init")
+        return t
+
+    def _sync(self, stream=None):
+        (stream or torch.cuda.current_stream()).synchronize()
+
+    def _dev(self, a):
+        return as_device(a, self.kb, self.device)
+
+    # ----------------------------------------------------------- metadata (hashinator.h:434-620)
+    def info(self) -> hn_info:
+        """Host copy of the map's Info (a device-to-host read, like touching managed Info)."""
+        raw = self._info.cpu().numpy().tobytes()
+        return hn_info.from_buffer_copy(raw)
+
+    def info_ptr(self):
+        return self._info.data_ptr()
+
+    def bucket_ptr(self):
+        return self.buckets.data_ptr()
+
+    def getSizePower(self) -> int:
+        return self._sp
+
+    def size(self) -> int:
+        return int(self.info().fill)
+
+    def bucket_count(self) -> int:
+        return 1 << self._sp
+
+    def load_factor(self) -> float:
+        return float(np.float32(self.size()) / np.float32(self.bucket_count()))
+
+    def tombstone_count(self) -> int:
+        return int(self.info().tombstone_counter)
+
+    def tombstone_ratio(self) -> float:
+        t = self.tombstone_count()
+        return 0.0 if t == 0 else float(np.float32(t) / np.float32(self.bucket_count()))
+
+    def peek_status(self) -> int:
+        """Read err and reset it to invalid (hashinator.h:434-438)."""
+        i = self.info()
+        self._info.view(torch.int32)[8] = status.invalid
+        return int(i.err)
+
+    def get_emptybucket(self) -> int:
+        return EMPTY[self.kb]
+
+    def get_tombstone(self) -> int:
+        return TOMBSTONE[self.kb]
+
+    def stats(self) -> str:
+        i = self.info()
+        return (
+            f"Hashinator Stats \nBucket size= {self.bucket_count()}\nFill= {i.fill}, "
+            f"LoadFactor={self.load_factor():f} \nTombstones= {i.tombstone_counter}\n"
+            f"Overflow= {i.current_max_bucket_overflow}\n"
+        )
+
+    def host_buckets(self) -> np.ndarray:
+        """(N, 2) unsigned copy of the bucket array."""
+        return to_numpy(self.buckets, self.kb).reshape(-1, 2)
+
+    # ----------------------------------------------------------- accelerated API
+    def _grow_for(self, n, targetLF, stream):
+        """Auto-grow before an insert (hashinator.h:1329-1332)."""
+        fill = self.size()
+        needed = math.ceil(math.log2((fill + n) * (1.0 / targetLF)))
+        if needed > self._sp:
+            self.resize(needed, targets.device, stream)
+
+    def insert(self, keys, vals=None, targetLF: float = 0.5, stream=None):
+        """insert(keys, vals, len, targetLF, s) (hashinator.h:1318-1335), or
+        insert(pairs, len, targetLF, s) (1358-1374) when ``vals`` is None and
+        ``keys`` is an (n, 2) array of pairs."""
+        k = self._dev(keys)
+        if vals is None:
+            k = k.reshape(-1, 2)
+            n = k.shape[0]
+        else:
+            v = self._dev(vals)
+            n = k.numel()
+            if v.numel() != n:
+                raise ValueError("keys and vals differ in length")
+        if n == 0:
+            self._info.view(torch.int32)[8] = status.success
+            return
+        self._grow_for(n, targetLF, stream)
+        if vals is None:
+            rc = self._fn("insert_pairs")(_ptr(k), _ptr(self.buckets), _ptr(self._info), n, _stream(stream), self._tune())
+        else:
+            rc = self._fn("insert")(_ptr(k), _ptr(v), _ptr(self.buckets), _ptr(self._info), n, _stream(stream),
+                                    self._tune())
+        check(rc, "insert")
+        self._sync(stream)
+
+    def insertIndex(self, keys, targetLF: float = 0.5, stream=None):
+        """insertIndex (hashinator.h:1338-1355): value = position in ``keys``."""
+        k = self._dev(keys)
+        n = k.numel()
+        if n == 0:
+            self._info.view(torch.int32)[8] = status.success
+            return
+        self._grow_for(n, targetLF, stream)
+        check(self._fn("insert_index")(_ptr(k), _ptr(self.buckets), _ptr(self._info), n, _stream(stream),
+                                       self._tune()), "insertIndex")
+        self._sync(stream)
+
+    def retrieve(self, keys, vals=None, hit=None, stream=None):
+        """retrieve(keys, vals, len, s) (hashinator.h:1377-1384).  ``vals`` is
+        updated in place on hits and left untouched on misses; with ``vals``
+        None, (keys, vals, hit) device tensors are returned, vals zero-filled.
+        With ``keys`` an (n, 2) pairs tensor and ``vals`` None, the pairs
+        variant (1387-1394) writes .second in place."""
+        k = self._dev(keys)
+        if k.dim() == 2 and vals is None:
+            n = k.shape[0]
+            h = hit if hit is not None else torch.empty(n, dtype=torch.uint8, device=self.device)
+            check(self._fn("retrieve_pairs")(_ptr(k), _ptr(h), _ptr(self.buckets), _ptr(self._info), n,
+                                             _stream(stream), self._tune()), "retrieve(pairs)")
+            self._sync(stream)
+            return k, h
+        n = k.numel()
+        v = vals if vals is not None else torch.zeros(n, dtype=_TDT[self.kb], device=self.device)
+        h = hit if hit is not None else torch.empty(n, dtype=torch.uint8, device=self.device)
+        check(self._fn("retrieve")(_ptr(k), _ptr(v), _ptr(h), _ptr(self.buckets), _ptr(self._info), n,
+                                   _stream(stream), self._tune()), "retrieve")
+        self._sync(stream)
+        return k, v, h
+
+    def erase(self, keys, stream=None):
+        """erase(keys, len, s) (hashinator.h:1397-1409)."""
+        k = self._dev(keys)
+        check(self._fn("erase")(_ptr(k), _ptr(self.buckets), _ptr(self._info), k.numel(), _stream(stream),
+                                self._tune()), "erase")
+        self._sync(stream)
+
+    def clean_tombstones(self, stream=None, scratch_budget: int = 0):
+        """clean_tombstones(s) (hashinator.h:1261-1315).  ``scratch_budget``
+        bounds the scratch in bytes (0: enough for the whole table)."""
+        sb = getattr(self._L, f"hn_clean_tombstones_scratch_bytes_{self._sfx}")(self._sp, scratch_budget)
+        i = self.info()
+        if i.tombstone_counter == 0:
+            return
+        if i.fill + i.tombstone_counter >= self.bucket_count():
+            sb = getattr(self._L, f"hn_clean_tombstones_scratch_bytes_{self._sfx}")(self._sp, 0)
+        scratch = torch.empty(sb, dtype=torch.uint8, device=self.device)
+        check(self._fn("clean_tombstones")(_ptr(self.buckets), _ptr(self._info), self._sp, _ptr(scratch), sb,
+                                           _stream(stream)), "clean_tombstones")
+        self._sync(stream)
+
+    def device_rehash(self, newSizePower: int, stream=None):
+        """device_rehash(p, s) (hashinator.h:284-336)."""
+        if newSizePower > 32:
+            raise OverflowError("Hashmap ran into rehashing catastrophe and exceeded 32bit buckets.")
+        fill = self.size()
+        # insert(valid, n, targetLF=1) grows further when the elements do not fit (hashinator.h:332, 1329-1332)
+        if fill > 0:
+            newSizePower = max(newSizePower, math.ceil(math.log2(fill)))
+        nb = self._new_table(newSizePower)
+        scratch = torch.empty(self._L.hn_rehash_scratch_bytes(), dtype=torch.uint8, device=self.device)
+        check(self._fn("rehash")(_ptr(self.buckets), self._sp, _ptr(nb), newSizePower, _ptr(self._info),
+                                 _ptr(scratch), _stream(stream)), "device_rehash")
+        self._sync(stream)
+        self.buckets = nb
+        self._sp = newSizePower
+
+    def resize(self, newSizePower: int, t: int = targets.device, stream=None):
+        """resize(p, targets, s) (hashinator.h:552-566).  Every target runs
+        the device rehash here; the host rehash of the reference is its
+        CPU-only path, not part of the accelerated API."""
+        self.device_rehash(newSizePower, stream)
+
+    def resize_to_lf(self, targetLF: float = 0.5, t: int = targets.device, stream=None):
+        """resize_to_lf (hashinator.h:530-546)."""
+        while self.load_factor() > targetLF:
+            self.device_rehash(self._sp + 1, stream)
+
+    def clear(self, t: int = targets.device, stream=None, length: int = 0):
+        """clear(targets, s, len) (hashinator.h:491-518).  targets.device:
+        keys -> EMPTY and fill = 0 only; otherwise a fresh table and Info."""
+        if t == targets.device:
+            n = length or self.bucket_count()
+            check(self._fn("reset_all")(_ptr(self.buckets), _ptr(self._info), n, _stream(stream)), "clear")
+        else:
+            self.buckets = self._new_table(self._sp)
+            check(self._L.hn_info_init(_ptr(self._info), self._sp, _stream(stream)), "clear")
+        self._sync(stream)
+
+    def performCleanupTasks(self, stream=None):
+        """GPU performCleanupTasks (hashinator.h:642-650)."""
+        if self.tombstone_ratio() > 0.025:
+            self.clean_tombstones(stream)
+        while self.info().current_max_bucket_overflow > _lib.HN_BUCKET_OVERFLOW:
+            self.device_rehash(self._sp + 1, stream)
+
+    def download(self, stream=None):
+        """download() (hashinator.h:1457-1467): rehash when the overflow grew
+        beyond BUCKET_OVERFLOW, else clean tombstones if any."""
+        if self.info().current_max_bucket_overflow > _lib.HN_BUCKET_OVERFLOW:
+            self.device_rehash(self._sp + 1, stream)
+        elif self.tombstone_count() > 0:
+            self.clean_tombstones(stream)
+
+    # ----------------------------------------------------------- extraction (hashinator.h:1154-1259)
+    def extractPattern(self, pred: hn_pred | None = None, stream=None):
+        """Pairs of the bucket array selected by ``pred`` (default: valid keys),
+        in bucket order, as an (n, 2) device tensor."""
+        from .tools import copy_if_pairs
+
+        return copy_if_pairs(self.buckets, pred if pred is not None else valid_pred(), self.kb, stream)
+
+    def extractKeysByPattern(self, pred: hn_pred | None = None, stream=None):
+        from .tools import copy_keys_if_pairs
+
+        return copy_keys_if_pairs(self.buckets, pred if pred is not None else valid_pred(), self.kb, stream)
+
+    def extractAllKeys(self, stream=None):
+        return self.extractKeysByPattern(valid_pred(), stream)
+
+
+def valid_pred() -> hn_pred:
+    return hn_pred(_lib.HN_PRED_VALID, 0, 0, 0)

@@ -1,0 +1,140 @@
+"""Seeded synthetic inputs for the map, compaction and batched workloads.
+
+Keys come from bijections of consecutive indices, so a key set is unique by
+construction (no dedupe pass) and a disjoint "absent" set is just another
+index range of the same bijection.  Neither sentinel (EMPTYBUCKET = all ones,
+TOMBSTONE = all ones - 1) is ever produced.
+
+Seeds: ``SEED_BASE + workload id`` (0x48415348494E4154 = "HASHINAT").
+"""
+from __future__ import annotations
+
+import numpy as np
+
+SEED_BASE = 0x48415348494E4154
+M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+M63 = np.uint64((1 << 63) - 1)
+
+
+def _u64(x):
+    return np.asarray(x, dtype=np.uint64)
+
+
+def mix64(x):
+    """SplitMix64 finalizer: a bijection of uint64."""
+    x = _u64(x).copy()
+    with np.errstate(over="ignore"):
+        x ^= x >> np.uint64(30)
+        x *= np.uint64(0xBF58476D1CE4E5B9)
+        x ^= x >> np.uint64(27)
+        x *= np.uint64(0x94D049BB133111EB)
+        x ^= x >> np.uint64(31)
+    return x
+
+
+def _bij_bits(x, bits):
+    """A bijection of the integers [0, 2^bits) (odd multiplies and xorshifts mod 2^bits)."""
+    m = np.uint64((1 << bits) - 1)
+    x = _u64(x) & m
+    h = np.uint64(bits // 2)
+    with np.errstate(over="ignore"):
+        x ^= x >> h
+        x = (x * np.uint64(0x9E3779B97F4A7C15)) & m
+        x ^= x >> h
+        x = (x * np.uint64(0xC2B2AE3D27D4EB4F)) & m
+        x ^= x >> h
+    return x
+
+
+def mix32(x):
+    """A bijection of uint32 (xorshift-multiply), computed in 64-bit lanes."""
+    x = _u64(x) & np.uint64(0xFFFFFFFF)
+    m = np.uint64(0xFFFFFFFF)
+    with np.errstate(over="ignore"):
+        x ^= x >> np.uint64(16)
+        x = (x * np.uint64(0x7FEB352D)) & m
+        x ^= x >> np.uint64(15)
+        x = (x * np.uint64(0x846CA68B)) & m
+        x ^= x >> np.uint64(16)
+    return x.astype(np.uint32)
+
+
+def unique_keys(n, bits, seed, start=0):
+    """n distinct keys (no sentinels), uniform over the key space.  Index
+    ranges [start, start+n+2) of one seed never collide with other ranges."""
+    idx = np.arange(start, start + n + 2, dtype=np.uint64)
+    if bits == 32:
+        off = np.uint64(mix64(np.uint64(seed)) & np.uint64(0xFFFFFFFF))
+        with np.errstate(over="ignore"):
+            k = mix32((idx + off) & np.uint64(0xFFFFFFFF))
+        k = k[k < np.uint32(0xFFFFFFFE)]
+    else:
+        off = mix64(np.uint64(seed))
+        with np.errstate(over="ignore"):
+            k = mix64(idx + off)
+        k = k[k < np.uint64(0xFFFFFFFFFFFFFFFE)]
+    return k[:n]
+
+
+def values(keys, seed):
+    """Uniform values, a function of (key, seed)."""
+    v = mix64(_u64(keys) ^ mix64(np.uint64(seed) + np.uint64(1)))
+    return v.astype(keys.dtype) if keys.dtype == np.uint32 else v
+
+
+def mixed64(n, seed, start=0):
+    """Keys of the large-map workload: half uniform uint64 (upper half of the
+    key space, a 63-bit bijection), half contiguous runs of 8..512 ids in the
+    lower half, each run in its own 1024-aligned block (a 53-bit bijection), so
+    all keys are distinct by construction.  Returned unshuffled."""
+    n_u = n // 2
+    n_r = n - n_u
+    idx = np.arange(start, start + n_u + 2, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        u = _bij_bits(idx + (mix64(np.uint64(seed)) & M63), 63) | np.uint64(1 << 63)
+    u = u[u < np.uint64(0xFFFFFFFFFFFFFFFE)][:n_u]
+    rng = np.random.default_rng(int(seed) & 0xFFFFFFFF)
+    lens = rng.integers(8, 513, size=n_r // 8 + 2, dtype=np.int64)
+    csum = np.cumsum(lens)
+    n_runs = int(np.searchsorted(csum, n_r)) + 1
+    lens = lens[:n_runs]
+    lens[-1] -= int(csum[n_runs - 1] - n_r)
+    with np.errstate(over="ignore"):
+        starts = _bij_bits(np.arange(n_runs, dtype=np.uint64) + (mix64(np.uint64(seed) ^ np.uint64(0x5555)) &
+                                                                   np.uint64((1 << 53) - 1)), 53) << np.uint64(10)
+    run_id = np.repeat(np.arange(n_runs), lens)
+    within = np.arange(n_r, dtype=np.uint64) - np.repeat((np.cumsum(lens) - lens).astype(np.uint64), lens)
+    r = starts[run_id] + within
+    return np.concatenate([u, r])
+
+
+def absent64(n, seed, n_used):
+    """Keys guaranteed absent from mixed64(n_used, seed): the next index range
+    of the same 63-bit bijection (upper half)."""
+    return mixed64_uniform_range(n, seed, start=n_used // 2 + 2)
+
+
+def mixed64_uniform_range(n, seed, start):
+    idx = np.arange(start, start + n + 2, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        u = _bij_bits(idx + (mix64(np.uint64(seed)) & M63), 63) | np.uint64(1 << 63)
+    return u[u < np.uint64(0xFFFFFFFFFFFFFFFE)][:n]
+
+
+def shuffled(a, seed):
+    return a[np.random.default_rng(int(seed) & 0xFFFFFFFF).permutation(a.size)]
+
+
+def velocity_grid_keys(n, center, sigma, rng, grid=200):
+    """Keys of one batched-workload map: linear indices x + grid*y + grid^2*z
+    of cells drawn around `center` with i.i.d. N(0, sigma^2) offsets, inside
+    the grid, without duplicates."""
+    out = np.empty(0, dtype=np.uint32)
+    while out.size < n:
+        m = max(64, 2 * (n - out.size))
+        d = np.rint(rng.normal(0.0, sigma, size=(m, 3))).astype(np.int64) + np.asarray(center, dtype=np.int64)
+        ok = np.all((d >= 0) & (d < grid), axis=1)
+        d = d[ok]
+        k = (d[:, 0] + grid * d[:, 1] + grid * grid * d[:, 2]).astype(np.uint32)
+        out = np.unique(np.concatenate([out, k]))
+    return rng.permutation(out)[:n]

@@ -1,0 +1,250 @@
+/*
+ * hashinator_amd.h -- C-ABI of the MI355X (gfx950) accelerated hash-map path.
+ *
+ * Every entry point is asynchronous on the given HIP stream, takes plain
+ * device-accessible pointers (hipMalloc or hipMallocManaged) and returns an
+ * int: 0 on success, a hipError_t value on a HIP failure, or one of the
+ * negative HN_E* codes below for invalid arguments.  Entry points are
+ * reentrant across maps; calls on the same map must be ordered by the caller
+ * (same stream, or events).
+ *
+ * Key/value instantiations: (uint32_t, uint32_t) as *_u32 and
+ * (uint64_t, uint64_t) as *_u64.  Sentinels follow the reference:
+ * EMPTYBUCKET = all ones, TOMBSTONE = EMPTYBUCKET - 1 (hashinator.h:59-60);
+ * callers must not insert either as a key.
+ *
+ * Bucket layout is the reference's hash_pair<K,V> array (hash_pair.h:29-75):
+ * 2^size_power slots of {K first; V second;}, 8 B (u32) or 16 B (u64) each.
+ *
+ * The reference interface each entry point replaces is cited as
+ * file:line in kstppd/hashinator include/.
+ */
+#ifndef HASHINATOR_AMD_H
+#define HASHINATOR_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t* hn_stream_t; /* == hipStream_t */
+
+#if defined(__GNUC__)
+#define HN_API __attribute__((visibility("default")))
+#else
+#define HN_API
+#endif
+
+/* Map metadata; byte-for-byte Hashinator::Info (hashinator/defaults.h:44-54).
+ * The two reserved words must be zero between calls (hn_info_init does
+ * that); insert kernels use them to finalise `err` without a host round trip. */
+typedef struct hn_info {
+   int32_t size_power;
+   int32_t reserved0;
+   uint64_t fill;
+   uint64_t current_max_bucket_overflow;
+   uint64_t tombstone_counter;
+   int32_t err; /* HN_STATUS_* == Hashinator::status (common.h:85) */
+   int32_t reserved1;
+} hn_info;
+
+enum { HN_STATUS_SUCCESS = 0, HN_STATUS_FAIL = 1, HN_STATUS_INVALID = 2 };
+
+/* BUCKET_OVERFLOW of the wave64 build (defaults.h:28-31): the initial and
+ * post-clean value of current_max_bucket_overflow, and the unit it grows in. */
+#define HN_BUCKET_OVERFLOW 64
+
+/* Error codes (negative, so they never collide with hipError_t). */
+#define HN_EINVAL (-1)   /* bad pointer, size or size_power */
+#define HN_ESCRATCH (-2) /* scratch buffer too small */
+#define HN_ENOLIB (-3)   /* no gfx950 code object for the current device */
+
+/* Launch tuning.  NULL means "use the tuned defaults".  probe_width is the
+ * number of lanes that probe one key together (1, 2, 4, 8, 16, 32 or 64; a
+ * window of probe_width consecutive, aligned slots is read per step).
+ * block_size is threads per workgroup (64..1024, multiple of 64); grid_blocks
+ * 0 means derived from len and the device's CU count. */
+typedef struct hn_tune {
+   int32_t probe_width;
+   int32_t block_size;
+   int32_t grid_blocks;
+   int32_t reserved;
+} hn_tune;
+
+/* Built-in compaction predicates (the C-ABI cannot carry a functor; user
+ * rules are evaluated into a flags array by the header shim and passed to
+ * hn_copy_if_flags). */
+typedef struct hn_pred {
+   int32_t kind;  /* HN_PRED_* */
+   int32_t field; /* 0: key (first), 1: value (second) */
+   uint64_t mask;
+   uint64_t value;
+} hn_pred;
+
+enum {
+   HN_PRED_ALL = 0,    /* every element */
+   HN_PRED_VALID = 1,  /* key != EMPTYBUCKET && key != TOMBSTONE */
+   HN_PRED_MASKEQ = 2, /* (field & mask) == value */
+   HN_PRED_LT = 3,     /* field < value */
+   HN_PRED_GE = 4      /* field >= value */
+};
+
+/* ---------------------------------------------------------------- map setup */
+
+/* Info(int) (defaults.h:46-48) written to device memory: size_power, fill 0,
+ * overflow HN_BUCKET_OVERFLOW, tombstones 0, err HN_STATUS_INVALID. */
+HN_API int hn_info_init(hn_info* info, int size_power, hn_stream_t s);
+
+/* SplitVector<hash_pair>(1<<sp, {EMPTYBUCKET, V()}) construction
+ * (hashinator.h:111-114, splitvec.h:233-238), done on the device. */
+HN_API int hn_table_init_u32(void* buckets, size_t n_slots, hn_stream_t s);
+HN_API int hn_table_init_u64(void* buckets, size_t n_slots, hn_stream_t s);
+
+/* ---------------------------------------------------------------- insert */
+
+/* Hasher::insert(keys, vals, buckets, info, len, s) (hashers.h:46-65) ->
+ * insert_kernel (kernels_AMD.h:272-405).  An existing key has its value
+ * overwritten; a new key claims the first EMPTY in probe order (TOMBSTONEs are
+ * not reused).  fill += new keys; overflow = max(overflow,
+ * nextOverflow(longest probe, 64)); err = SUCCESS, or FAIL if a key found no
+ * EMPTY slot in the whole table. */
+HN_API int hn_insert_u32(const uint32_t* keys, const uint32_t* vals, void* buckets, hn_info* info, size_t len,
+                  hn_stream_t s, const hn_tune* t);
+HN_API int hn_insert_u64(const uint64_t* keys, const uint64_t* vals, void* buckets, hn_info* info, size_t len,
+                  hn_stream_t s, const hn_tune* t);
+
+/* Hasher::insertIndex (hashers.h:68-87) -> insert_index_kernel
+ * (kernels_AMD.h:507-641): value = input index. */
+HN_API int hn_insert_index_u32(const uint32_t* keys, void* buckets, hn_info* info, size_t len, hn_stream_t s,
+                        const hn_tune* t);
+HN_API int hn_insert_index_u64(const uint64_t* keys, void* buckets, hn_info* info, size_t len, hn_stream_t s,
+                        const hn_tune* t);
+
+/* Hasher::insert(src pairs, ...) (hashers.h:90-108) -> insert_kernel over
+ * hash_pair input (kernels_AMD.h:136-270). */
+HN_API int hn_insert_pairs_u32(const void* pairs, void* buckets, hn_info* info, size_t len, hn_stream_t s,
+                        const hn_tune* t);
+HN_API int hn_insert_pairs_u64(const void* pairs, void* buckets, hn_info* info, size_t len, hn_stream_t s,
+                        const hn_tune* t);
+
+/* ---------------------------------------------------------------- retrieve */
+
+/* Hasher::retrieve(keys, vals, ...) (hashers.h:111-119) -> retrieve_kernel
+ * (kernels_AMD.h:647-697).  Probe offsets below
+ * current_max_bucket_overflow; on a hit vals[i] = value, on a miss vals[i] is
+ * left untouched.  `hit` (nullable) receives 1/0 per key -- an addition, the
+ * reference has no hit output. */
+HN_API int hn_retrieve_u32(const uint32_t* keys, uint32_t* vals, uint8_t* hit, const void* buckets,
+                    const hn_info* info, size_t len, hn_stream_t s, const hn_tune* t);
+HN_API int hn_retrieve_u64(const uint64_t* keys, uint64_t* vals, uint8_t* hit, const void* buckets,
+                    const hn_info* info, size_t len, hn_stream_t s, const hn_tune* t);
+
+/* Hasher::retrieve(src pairs, ...) (hashers.h:121-129) -> retrieve_kernel over
+ * hash_pair (kernels_AMD.h:703-752): src[i].second is written on a hit. */
+HN_API int hn_retrieve_pairs_u32(void* pairs, uint8_t* hit, const void* buckets, const hn_info* info, size_t len,
+                          hn_stream_t s, const hn_tune* t);
+HN_API int hn_retrieve_pairs_u64(void* pairs, uint8_t* hit, const void* buckets, const hn_info* info, size_t len,
+                          hn_stream_t s, const hn_tune* t);
+
+/* ---------------------------------------------------------------- erase */
+
+/* Hashmap::erase(keys, len) (hashinator.h:1397-1409) -> Hasher::erase
+ * (hashers.h:132-142) -> delete_kernel (kernels_AMD.h:412-505).  A found key
+ * becomes TOMBSTONE (compare-and-swap, so a key repeated in one batch counts
+ * once); tombstone_counter += erased, fill -= erased (done on the device). */
+HN_API int hn_erase_u32(const uint32_t* keys, void* buckets, hn_info* info, size_t len, hn_stream_t s,
+                 const hn_tune* t);
+HN_API int hn_erase_u64(const uint64_t* keys, void* buckets, hn_info* info, size_t len, hn_stream_t s,
+                 const hn_tune* t);
+
+/* ---------------------------------------------------------------- clear */
+
+/* clear(targets::device) (hashinator.h:491-518) -> Hasher::reset_all
+ * (hashers.h:155-163) -> reset_all_to_empty (kernels_AMD.h:26-43).  Keys of
+ * the first `len` slots become EMPTYBUCKET, fill = 0; values, tombstone
+ * counter and overflow are left as they are. */
+HN_API int hn_reset_all_u32(void* buckets, hn_info* info, size_t len, hn_stream_t s);
+HN_API int hn_reset_all_u64(void* buckets, hn_info* info, size_t len, hn_stream_t s);
+
+/* ---------------------------------------------------------------- clean / rehash */
+
+/* Scratch needed by hn_clean_tombstones for a table of 2^size_power slots
+ * when the caller can spare `budget_bytes` (0 = no limit).  The pass runs in
+ * regions of at most (scratch - header) / slot_bytes slots, so any scratch of
+ * at least hn_clean_tombstones_min_scratch() bytes works; more scratch means
+ * fewer launches. */
+HN_API size_t hn_clean_tombstones_scratch_bytes_u32(int size_power, size_t budget_bytes);
+HN_API size_t hn_clean_tombstones_scratch_bytes_u64(int size_power, size_t budget_bytes);
+HN_API size_t hn_clean_tombstones_min_scratch(void);
+
+/* Hashmap::clean_tombstones (hashinator.h:1261-1315).  No-op when
+ * tombstone_counter == 0 (checked on the device).  Otherwise TOMBSTONEs
+ * become EMPTY, every live element not in its home slot is lifted out and
+ * reinserted, tombstone_counter = 0, overflow restarts at
+ * HN_BUCKET_OVERFLOW, fill is unchanged.  The table is processed region by
+ * region starting after an EMPTY slot, so scratch is bounded by the region
+ * size, not the table.  A table with no EMPTY slot at all needs scratch for
+ * the whole table; with less, err = FAIL and the table is left unchanged. */
+HN_API int hn_clean_tombstones_u32(void* buckets, hn_info* info, int size_power, void* scratch, size_t scratch_bytes,
+                            hn_stream_t s);
+HN_API int hn_clean_tombstones_u64(void* buckets, hn_info* info, int size_power, void* scratch, size_t scratch_bytes,
+                            hn_stream_t s);
+
+/* Hashmap::device_rehash (hashinator.h:284-336): every live element of
+ * old_buckets (2^old_sp slots) is inserted into new_buckets (2^new_sp slots,
+ * initialised here to {EMPTYBUCKET, 0}); *info becomes Info(new_sp) plus the
+ * inserted elements; err = SUCCESS iff the fill is preserved.  The buffers
+ * must not overlap.  `scratch` must hold hn_rehash_scratch_bytes() bytes. */
+HN_API size_t hn_rehash_scratch_bytes(void);
+HN_API int hn_rehash_u32(const void* old_buckets, int old_sp, void* new_buckets, int new_sp, hn_info* info, void* scratch,
+                  hn_stream_t s);
+HN_API int hn_rehash_u64(const void* old_buckets, int old_sp, void* new_buckets, int new_sp, hn_info* info, void* scratch,
+                  hn_stream_t s);
+
+/* ---------------------------------------------------------------- compaction and scan */
+
+/* Scratch for one compaction or scan of n elements
+ * (estimateMemoryForCompaction, split_tools.h:1064-1089, sized for the
+ * single-pass kernel instead). */
+HN_API size_t hn_compact_scratch_bytes(size_t n);
+
+/* split::tools::copy_if over hash_pair<K,K> (split_tools.h:1128-1223) with a
+ * built-in predicate: stable, out[0..count) = selected elements in input
+ * order; *d_count (device pointer) = count. */
+HN_API int hn_copy_if_pairs_u32(const void* in, size_t n, const hn_pred* pred, void* out, uint64_t* d_count, void* scratch,
+                         size_t scratch_bytes, hn_stream_t s);
+HN_API int hn_copy_if_pairs_u64(const void* in, size_t n, const hn_pred* pred, void* out, uint64_t* d_count, void* scratch,
+                         size_t scratch_bytes, hn_stream_t s);
+
+/* split::tools::copy_keys_if (split_tools.h:1094-1111, 1146-1159,
+ * 1176-1191): as above, writing only .first of each selected element. */
+HN_API int hn_copy_keys_if_pairs_u32(const void* in, size_t n, const hn_pred* pred, uint32_t* out, uint64_t* d_count,
+                              void* scratch, size_t scratch_bytes, hn_stream_t s);
+HN_API int hn_copy_keys_if_pairs_u64(const void* in, size_t n, const hn_pred* pred, uint64_t* out, uint64_t* d_count,
+                              void* scratch, size_t scratch_bytes, hn_stream_t s);
+
+/* copy_if with the rule already evaluated into flags (one byte per element,
+ * nonzero = keep): the path the header shim takes for user functors.
+ * elem_bytes must be 4, 8 or 16. */
+HN_API int hn_copy_if_flags(const void* in, const uint8_t* flags, size_t n, size_t elem_bytes, void* out,
+                     uint64_t* d_count, void* scratch, size_t scratch_bytes, hn_stream_t s);
+
+/* split::tools::split_prefix_scan (split_tools.h:363-418): exclusive prefix
+ * sum with wraparound in the element type.  Any n (the reference requires a
+ * power of two).  in == out is allowed. */
+HN_API int hn_exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t n, void* scratch, size_t scratch_bytes,
+                          hn_stream_t s);
+HN_API int hn_exclusive_scan_u64(const uint64_t* in, uint64_t* out, size_t n, void* scratch, size_t scratch_bytes,
+                          hn_stream_t s);
+
+/* ---------------------------------------------------------------- misc */
+
+/* Version string and the gfx target the library was built for. */
+HN_API const char* hn_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
