@@ -1,0 +1,334 @@
+"""Benchmark: Mops/s of insert+retrieve on one large map at load factor 0.9.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Workload (one step): a fresh Hashmap<uint64_t,uint64_t> of 2^28 slots
+(4 GiB of buckets), insert floor(0.9 * 2^28) = 241,591,910 distinct keys (half
+uniform uint64, half runs of 8..512 consecutive ids), then retrieve all of them
+plus as many absent keys (shuffled together).  Ops per step = 3 * n.  Inputs
+are synthetic, generated once, and resident in HBM before timing starts.
+
+Besides the headline line it reports, from the same process: per-stage
+timings, the rest of the large-map sequence (erase 25%, clean_tombstones,
+retrieve after clean), the other load factors, the HBM roofline of the
+dominant kernel (retrieve) and the CPU baseline (the reference's host-only
+algorithm on a bounded sample of the same workload, one core).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mops/s insert+retrieve @LF0.9; batched vs per-map; % HBM roofline"
+PEAK_HBM_GBS = 8000.0
+SECTOR = 64  # bytes per random HBM sector in the algorithmic model
+
+
+def knuth_hit(a):
+    return 0.5 * (1 + 1 / (1 - a))
+
+
+def knuth_miss(a):
+    return 0.5 * (1 + 1 / (1 - a) ** 2)
+
+
+def sectors(L, slot):
+    """Expected 64 B sectors spanned by L consecutive slots at a random start."""
+    return 1 + (L - 1) * slot / SECTOR
+
+
+def alg_bytes_retrieve(n_hit, n_miss, alpha, kv=16, slot=16):
+    return n_hit * (kv + SECTOR * sectors(knuth_hit(alpha), slot)) + \
+        n_miss * (kv + SECTOR * sectors(knuth_miss(alpha), slot))
+
+
+def alg_bytes_insert(n, alpha, kv=16, slot=16):
+    return n * (kv + SECTOR * sectors(knuth_hit(alpha), slot) + SECTOR)
+
+
+def read_traffic(name):
+    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE,
+    calibrated on a copy kernel) recorded in profiles/<name>."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("TRAFFIC_BYTES_PER_LAUNCH:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def cpu_baseline(sp_sample=24, alpha=0.9, seed=0x48415348494E4154 + 2):
+    """The reference's host-only algorithm (serial _at insert, const find) on
+    the same workload at 2^sp_sample slots, one core.  Uses the reference's own
+    CPU-only build when it was built into oracle/_ref, else the C restatement."""
+    import oracle as O
+    from hashinator_amd import workloads as Wl
+
+    n = int(alpha * (1 << sp_sample))
+    keys = Wl.shuffled(Wl.mixed64(n, seed), seed)
+    vals = Wl.values(keys, seed)
+    absent = Wl.absent64(n, seed, n)
+    q = Wl.shuffled(np.concatenate([keys, absent]), seed + 1)
+    kind = "reference" if O.ref_available() else "port"
+    m = O.ref_map(sp_sample, 64) if kind == "reference" else O.CpuMap(sp_sample, 64)
+    t0 = time.perf_counter()
+    m.insert(keys, vals)
+    t1 = time.perf_counter()
+    v, h = m.find(q)
+    t2 = time.perf_counter()
+    assert int(h.sum()) == n
+    ops = n + q.size
+    return {
+        "value": round(ops / (t2 - t0) / 1e6, 2),
+        "unit": "Mops/s",
+        "cores": 1,
+        "kind": kind,
+        "sample": (f"Hashmap<u64,u64>(sp={sp_sample}) at LF {alpha}: insert {n} keys (_at), then find {n} present + "
+                   f"{n} absent keys (const find); same key distribution as the GPU workload at 1/{1 << (28 - sp_sample)} "
+                   f"scale; insert {n / (t1 - t0) / 1e6:.2f} Mops/s, find {q.size / (t2 - t1) / 1e6:.2f} Mops/s; "
+                   f"{t2 - t0:.1f} s of CPU time"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--sp", type=int, default=28)
+    ap.add_argument("--lf", type=float, default=0.9)
+    ap.add_argument("--probe-width", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-extra", action="store_true", help="skip the erase/clean and load-factor extras")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run one plain 1 GiB device copy (counter calibration for rocprofv3 --pmc)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus > 1 is not supported "
+                                   "(maps shard by key hash across GPUs, see DESIGN.md)"}))
+        sys.exit(2)
+
+    import torch
+
+    from hashinator_amd import _lib, hn_tune
+    from hashinator_amd import workloads as Wl
+
+    L = _lib.load()
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    stream = torch.cuda.current_stream()
+    S = C.c_void_p(stream.cuda_stream)
+
+    def P(t):
+        return C.c_void_p(t.data_ptr())
+
+    tune = C.byref(hn_tune(args.probe_width, 0, 0, 0)) if args.probe_width else None
+    sp, alpha = args.sp, args.lf
+    N = 1 << sp
+    seed = Wl.SEED_BASE + 2
+
+    def make_inputs(alpha):
+        n = int(alpha * N)
+        keys = Wl.mixed64(n, seed)
+        vals = Wl.values(keys, seed)
+        absent = Wl.absent64(n, seed, n)
+        g = torch.Generator(device=dev).manual_seed(seed & 0x7FFFFFFF)
+        dk = torch.from_numpy(keys.view(np.int64)).to(dev)
+        dv = torch.from_numpy(vals.view(np.int64)).to(dev)
+        perm = torch.randperm(n, device=dev, generator=g)
+        dk, dv = dk[perm].contiguous(), dv[perm].contiguous()
+        q = torch.cat([dk, torch.from_numpy(absent.view(np.int64)).to(dev)])
+        q = q[torch.randperm(q.numel(), device=dev, generator=g)].contiguous()
+        return n, dk, dv, q
+
+    t_gen = time.perf_counter()
+    n, dk, dv, q = make_inputs(alpha)
+    buckets = torch.empty((N, 2), dtype=torch.int64, device=dev)
+    info = torch.zeros(5, dtype=torch.int64, device=dev)
+    out = torch.empty_like(q)
+    hit = torch.empty(q.numel(), dtype=torch.uint8, device=dev)
+    torch.cuda.synchronize()
+    t_gen = time.perf_counter() - t_gen
+
+    def ev():
+        return torch.cuda.Event(enable_timing=True)
+
+    def step(rec=None):
+        e = [ev() for _ in range(4)]
+        e[0].record(stream)
+        _lib.check(L.hn_table_init_u64(P(buckets), N, S), "table_init")
+        _lib.check(L.hn_info_init(P(info), sp, S), "info_init")
+        e[1].record(stream)
+        _lib.check(L.hn_insert_u64(P(dk), P(dv), P(buckets), P(info), n, S, tune), "insert")
+        e[2].record(stream)
+        _lib.check(L.hn_retrieve_u64(P(q), P(out), P(hit), P(buckets), P(info), q.numel(), S, tune), "retrieve")
+        e[3].record(stream)
+        if rec is not None:
+            rec.append(e)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    recs = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(recs)
+    torch.cuda.synchronize()
+    t_total = time.perf_counter() - t0
+
+    # correctness of the last step: every present key hit, no absent key hit, values match
+    h = hit.bool()
+    n_hit = int(h.sum().item())
+    ok = n_hit == n
+    if ok:
+        got = out[h]
+        qk = q[h]
+        ok = torch.equal(torch.sort(qk).values, torch.sort(dk).values)
+        if ok:
+            ik = torch.argsort(dk)
+            ok = torch.equal(got[torch.argsort(qk)], dv[ik])
+    inf = info.cpu().numpy()
+    assert ok and int(inf[1]) == n and (int(inf[4]) & 0xFFFFFFFF) == 0, "result check failed"
+
+    t_init = np.mean([a[0].elapsed_time(a[1]) for a in recs])
+    t_ins = np.mean([a[1].elapsed_time(a[2]) for a in recs])
+    t_ret = np.mean([a[2].elapsed_time(a[3]) for a in recs])
+    ms_step = t_total * 1e3 / args.steps
+    ops = 3 * n
+    value = ops / (ms_step * 1e-3) / 1e6
+
+    ret_bytes = alg_bytes_retrieve(n, n, n / N)
+    ins_bytes = alg_bytes_insert(n, n / N)
+    achieved = ret_bytes / (t_ret * 1e-3) / 1e9
+    traffic = read_traffic("retrieve_u64_sp28_lf09_traffic.md")
+    roofline = {
+        "bound": "hbm",
+        "kernel": "k_lookup<u64> (retrieve)",
+        "achieved": round(achieved, 1),
+        "peak": PEAK_HBM_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / PEAK_HBM_GBS, 4),
+        "traffic": traffic,
+        "alg_bytes_per_launch": round(ret_bytes),
+        "avg_launch_ms": round(float(t_ret), 3),
+    }
+    stages = {
+        "table_init_ms": round(float(t_init), 3),
+        "insert_ms": round(float(t_ins), 3),
+        "insert_Mops": round(n / t_ins / 1e3, 1),
+        "insert_GBps_alg": round(ins_bytes / (t_ins * 1e-3) / 1e9, 1),
+        "retrieve_ms": round(float(t_ret), 3),
+        "retrieve_Mops": round(2 * n / t_ret / 1e3, 1),
+        "overflow": int(inf[2]),
+    }
+
+    if args.calib_copy:
+        src = torch.ones(1 << 27, dtype=torch.int64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
+    extra = {}
+    if not args.no_extra:
+        extra = run_extras(L, torch, dev, S, P, tune, buckets, info, dk, dv, q, out, hit, n, N, sp, make_inputs)
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "Mops/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"single Hashmap<u64,u64> sp={sp} ({N} slots), LF {alpha}: insert {n} keys + retrieve "
+                        f"{n} present + {n} absent keys per step",
+            "size_power": sp,
+            "load_factor": alpha,
+            "keys": "50% uniform u64, 50% runs of 8-512 consecutive ids",
+            "probe_width": args.probe_width or "default",
+        },
+        "roofline": roofline,
+        "stages": stages,
+        "extra": extra,
+        "input_gen_s": round(t_gen, 1),
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(line), flush=True)
+
+
+def run_extras(L, torch, dev, S, P, tune, buckets, info, dk, dv, q, out, hit, n, N, sp, make_inputs):
+    """Rest of the large-map sequence on the map left by the last step, then
+    the other load factors (one step each)."""
+    from hashinator_amd import _lib
+
+    extra = {}
+    stream = torch.cuda.current_stream()
+
+    def timed(fn):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1)
+
+    g = torch.Generator(device=dev).manual_seed(12345)
+    er = dk[torch.randperm(n, device=dev, generator=g)[: n // 4]].contiguous()
+    t_er = timed(lambda: _lib.check(L.hn_erase_u64(P(er), P(buckets), P(info), er.numel(), S, tune), "erase"))
+    sb = L.hn_clean_tombstones_scratch_bytes_u64(sp, 1 << 30)
+    scratch = torch.empty(sb, dtype=torch.uint8, device=dev)
+    t_cl = timed(lambda: _lib.check(L.hn_clean_tombstones_u64(P(buckets), P(info), sp, P(scratch), sb, S), "clean"))
+    t_ra = timed(lambda: _lib.check(L.hn_retrieve_u64(P(q), P(out), P(hit), P(buckets), P(info), q.numel(), S, tune),
+                                    "retrieve"))
+    inf = info.cpu().numpy()
+    n_hit = int(hit.sum().item())
+    extra["erase25_clean_retrieve"] = {
+        "erase_ms": round(t_er, 3), "erase_Mops": round(er.numel() / t_er / 1e3, 1),
+        "clean_ms": round(t_cl, 3), "clean_scratch_MiB": round(sb / 2**20, 1),
+        "retrieve_after_ms": round(t_ra, 3), "retrieve_after_Mops": round(q.numel() / t_ra / 1e3, 1),
+        "hits_after": n_hit, "expected_hits": n - n // 4, "tombstones_after": int(inf[3]),
+        "ok": n_hit == n - n // 4 and int(inf[3]) == 0 and int(inf[1]) == n - n // 4,
+    }
+    del scratch
+    for a in (0.5, 0.75):
+        n2, dk2, dv2, q2 = make_inputs(a)
+        out2 = torch.empty_like(q2)
+        hit2 = torch.empty(q2.numel(), dtype=torch.uint8, device=dev)
+
+        def one():
+            L.hn_table_init_u64(P(buckets), N, S)
+            L.hn_info_init(P(info), sp, S)
+
+        timed(one)
+        ti = timed(lambda: _lib.check(L.hn_insert_u64(P(dk2), P(dv2), P(buckets), P(info), n2, S, tune), "insert"))
+        tr = timed(lambda: _lib.check(L.hn_retrieve_u64(P(q2), P(out2), P(hit2), P(buckets), P(info), q2.numel(),
+                                                        S, tune), "retrieve"))
+        extra[f"lf{a}"] = {"n": n2, "insert_Mops": round(n2 / ti / 1e3, 1),
+                           "retrieve_Mops": round(q2.numel() / tr / 1e3, 1),
+                           "insert_retrieve_Mops": round(3 * n2 / (ti + tr) / 1e3, 1),
+                           "hits_ok": int(hit2.sum().item()) == n2}
+        del dk2, dv2, q2, out2, hit2
+    return extra
+
+
+if __name__ == "__main__":
+    main()

@@ -18,11 +18,11 @@ struct Launch {
 // Compute units of the current device (cached per device ordinal).
 int cu_count();
 
-// Default probe width: one 64-byte window per step (4 slots of 16 B for
-// u64 maps, 8 slots of 8 B for u32 maps).
+// Default probe width: one 128-byte window per step (8 slots of 16 B for
+// u64 maps, 16 slots of 8 B for u32 maps), the fastest on MI355X at LF 0.9.
 template <typename K>
 constexpr int default_probe_width(OpKind) {
-   return sizeof(K) == 8 ? 4 : 8;
+   return sizeof(K) == 8 ? 8 : 16;
 }
 
 constexpr int kDefaultBlock = 256;

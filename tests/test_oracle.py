@@ -1,0 +1,228 @@
+"""The oracle pinned: against the reference's own CPU-only build (golden
+fixtures, and the live build when present), against known answers, and
+against what the mathematics fixes (order independence of linear-probing
+occupancy, brute-force compaction and scan).  CPU only."""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from hashinator_amd import workloads as W
+from tests.helpers import EMPTY, NDT, TOMB, live_pairs, occupancy, reachable_all
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+# ------------------------------------------------------------------ hash (hashfunctions.h:39-56)
+def test_fibhash_known_answers():
+    L = O.lib()
+    assert L.or_fibhash32(1, 20) == 648055
+    assert L.or_fibhash32(0xDEADBEEF, 20) == 609062230  # unmasked high bits kept (hashfunctions.h:41)
+    assert L.or_fibhash32(0xDEADBEEF, 20) & ((1 << 20) - 1) == 888150
+    assert L.or_fibhash64(1, 28) == 165902235
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_fibhash_golden(bits):
+    g = np.load(os.path.join(GOLD, f"fibhash{bits}.npz"))
+    assert np.array_equal(O.fibhash(g["keys"], g["sps"], bits), g["hash"])
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_fibhash_closed_form(bits):
+    rng = np.random.default_rng(bits)
+    keys = rng.integers(0, 2**bits - 1, size=2000, dtype=np.uint64, endpoint=True)
+    for sp in (1, 7, 20, 31):
+        got = O.fibhash(keys.astype(NDT[bits]), sp, bits)
+        for k, h in zip(keys[:200].tolist(), got[:200].tolist()):
+            if bits == 32:
+                k2 = k ^ (k >> (32 - sp))
+                exp = ((k2 * 2654435769) >> (32 - sp)) & 0xFFFFFFFF
+            else:
+                k2 = k ^ (k >> (64 - sp))
+                exp = ((k2 * 11400714819323198485) % 2**64) >> (64 - sp)
+            assert h == exp
+
+
+# ------------------------------------------------------------------ host-only map vs the reference build
+FIX = sorted(glob.glob(os.path.join(GOLD, "cpumap*_sp*.npz")))
+
+
+@pytest.mark.parametrize("path", FIX, ids=[os.path.basename(p) for p in FIX])
+def test_cpu_map_golden_images(path):
+    bits = 32 if "cpumap32" in path else 64
+    sp = int(path.split("_sp")[1].split(".")[0])
+    g = np.load(path)
+    m = O.CpuMap(sp, bits)
+    m.insert(g["k1"], g["v1"])
+    assert np.array_equal(m.buckets(), g["img_insert"])
+    assert np.array_equal(np.array(m.info().as_tuple(), dtype=np.uint64), g["info_insert"])
+    assert m.erase(g["er"]) == int(g["n_erased"][0])
+    assert np.array_equal(m.buckets(), g["img_erase"])
+    assert np.array_equal(np.array(m.info().as_tuple(), dtype=np.uint64), g["info_erase"])
+    m.insert(g["k2"], g["v2"])  # tombstone reuse with the look-ahead (hashinator.h:362-392)
+    assert np.array_equal(m.buckets(), g["img_reinsert"])
+    assert np.array_equal(np.array(m.info().as_tuple(), dtype=np.uint64), g["info_reinsert"])
+    v, h = m.find(g["q"])
+    assert np.array_equal(h, g["find_hit"]) and np.array_equal(np.where(h, v, 0), g["find_vals"])
+    m.rehash(sp + 1)
+    assert np.array_equal(m.buckets(), g["img_rehash"])
+    assert np.array_equal(np.array(m.info().as_tuple(), dtype=np.uint64), g["info_rehash"])
+    v, h = m.find(g["q"])
+    assert np.array_equal(h, g["find2_hit"]) and np.array_equal(np.where(h, v, 0), g["find2_vals"])
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference CPU-only build not present (build container only)")
+@pytest.mark.parametrize("bits", [32, 64])
+def test_cpu_map_vs_live_reference(bits):
+    rng = np.random.default_rng(bits)
+    for trial in range(25):
+        sp = int(rng.integers(3, 12))
+        a, b = O.CpuMap(sp, bits), O.ref_map(sp, bits)
+        for step in range(5):
+            n = int(rng.integers(1, 1 << sp))
+            keys = rng.integers(0, 4 << sp, n).astype(NDT[bits])
+            vals = rng.integers(0, 2**31, n).astype(NDT[bits])
+            a.insert(keys, vals)
+            b.insert(keys, vals)
+            er = keys[rng.random(n) < 0.4]
+            assert a.erase(er) == b.erase(er)
+            assert np.array_equal(a.buckets(), b.buckets())
+            assert a.info().as_tuple() == b.info().as_tuple()
+            if step == 2:
+                a.rehash(a.info().size_power + 1)
+                b.rehash(b.info().size_power + 1)
+
+
+# ------------------------------------------------------------------ accelerated-path model
+@pytest.mark.parametrize("path", FIX, ids=[os.path.basename(p) for p in FIX])
+def test_gpu_model_equals_reference_without_tombstones(path):
+    """With no tombstones, the accelerated insert (first EMPTY in probe order,
+    overwrite on match) applied serially is the reference's _at: same image."""
+    bits = 32 if "cpumap32" in path else 64
+    sp = int(path.split("_sp")[1].split(".")[0])
+    g = np.load(path)
+    m = O.GpuModel(sp, bits)
+    m.insert(g["k1"], g["v1"])
+    assert np.array_equal(m.b, g["img_insert"])
+    assert m.info.fill == int(g["info_insert"][1])
+    # lookups on the reference's post-erase image agree with the reference's find
+    m2 = O.GpuModel(sp, bits, buckets=g["img_reinsert"],
+                    info=O.Info(sp, 0, int(g["info_reinsert"][1]), 1 << sp, int(g["info_reinsert"][3]), 0, 0))
+    v, h = m2.retrieve(g["q"])
+    assert np.array_equal(h, g["find_hit"]) and np.array_equal(np.where(h, v, 0), g["find_vals"])
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_gpu_model_occupancy_is_order_independent(bits):
+    sp = 12
+    keys = W.unique_keys(int(0.93 * (1 << sp)), bits, 17)
+    base = O.GpuModel(sp, bits)
+    base.insert(keys, keys)
+    rng = np.random.default_rng(1)
+    for _ in range(3):
+        p = rng.permutation(keys.size)
+        m = O.GpuModel(sp, bits)
+        m.insert(keys[p], keys[p])
+        assert np.array_equal(occupancy(m.b, bits), occupancy(base.b, bits))
+        assert np.array_equal(live_pairs(m.b, bits), live_pairs(base.b, bits))
+        assert reachable_all(m.b, bits, sp)
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_gpu_model_set_semantics_vs_dict(bits):
+    sp = 12  # room for every key of the 0..2999 range
+    rng = np.random.default_rng(bits + 3)
+    m = O.GpuModel(sp, bits)
+    ref = {}
+    for rnd in range(6):
+        keys = rng.integers(0, 3000, 300).astype(NDT[bits])
+        vals = (keys.astype(np.uint64) * 7 + rnd).astype(NDT[bits])  # one value per key within a batch
+        m.insert(keys, vals)
+        ref.update(zip(keys.tolist(), vals.tolist()))
+        er = rng.integers(0, 3000, 100).astype(NDT[bits])
+        m.erase(er)
+        for k in set(er.tolist()):
+            ref.pop(k, None)
+        if rnd % 2:
+            m.clean_tombstones()
+            assert m.info.tombstones == 0 and not np.any(m.b.reshape(-1, 2)[:, 0] == TOMB[bits])
+        assert m.info.fill == len(ref)
+        lp = live_pairs(m.b, bits)
+        assert dict(zip(lp[:, 0].tolist(), lp[:, 1].tolist())) == ref
+        q = np.arange(3000, dtype=NDT[bits])
+        v, h = m.retrieve(q)
+        assert [int(x) for x in q[h]] == sorted(ref)
+        assert all(ref[int(k)] == int(x) for k, x in zip(q[h], v[h]))
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_gpu_model_clean_and_rehash_postconditions(bits):
+    sp = 12
+    keys = W.unique_keys(int(0.9 * (1 << sp)), bits, 23)
+    m = O.GpuModel(sp, bits)
+    m.insert(keys, keys)
+    m.erase(keys[::4])
+    live = live_pairs(m.b, bits)
+    m.clean_tombstones()
+    assert np.array_equal(live_pairs(m.b, bits), live)
+    fresh = O.GpuModel(sp, bits)
+    fresh.insert(live[:, 0].copy(), live[:, 1].copy())
+    assert np.array_equal(occupancy(m.b, bits), occupancy(fresh.b, bits))
+    assert reachable_all(m.b, bits, sp)
+    assert m.info.overflow == max(64, -(-m.max_probe() // 64) * 64)
+    assert m.rehash(sp + 1) == 0
+    assert np.array_equal(live_pairs(m.b, bits), live) and m.info.size_power == sp + 1
+    assert reachable_all(m.b, bits, sp + 1)
+
+
+def test_checks_catch_planted_mistakes():
+    """The invariants used on the GPU results fail on a broken image."""
+    bits, sp = 64, 10
+    keys = W.unique_keys(900, bits, 29)
+    m = O.GpuModel(sp, bits)
+    m.insert(keys, keys)
+    img = m.b.reshape(-1, 2).copy()
+    # an EMPTY hole planted inside a probe path breaks reachability
+    occ = np.nonzero(img[:, 0] != EMPTY[bits])[0]
+    home = O.fibhash(img[occ, 0], sp, bits).astype(np.int64) & ((1 << sp) - 1)
+    displaced = occ[home != occ]
+    victim = displaced[0] - 1
+    bad = img.copy()
+    bad[victim % (1 << sp), 0] = EMPTY[bits]
+    assert not reachable_all(bad, bits, sp)
+    # swapping two keys' values changes the live set
+    bad = img.copy()
+    a, b = occ[0], occ[1]
+    bad[a, 1], bad[b, 1] = img[b, 1] + 1, img[a, 1]
+    assert not np.array_equal(live_pairs(bad, bits), live_pairs(img, bits))
+    # a wrong hash (dropped xor-shift term) changes the image
+    wrong = np.empty_like(img)
+    wrong[:] = [EMPTY[bits], 0]
+    for k in keys:
+        h = ((int(k) * 11400714819323198485) % 2**64) >> (64 - sp)
+        while wrong[h, 0] != EMPTY[bits]:
+            h = (h + 1) % (1 << sp)
+        wrong[h] = [k, k]
+    assert not np.array_equal(wrong, img)
+
+
+# ------------------------------------------------------------------ split::tools
+@pytest.mark.parametrize("n", [0, 1, 1000, 100_001])
+def test_copy_if_and_scan_brute_force(n):
+    rng = np.random.default_rng(n)
+    pairs = rng.integers(0, 2**64 - 1, size=(n, 2), dtype=np.uint64, endpoint=True)
+    p = O.make_pred(O.PRED_MASKEQ, 1, 1, 1, 64)
+    out = O.copy_if_pairs(pairs.reshape(-1), p, 64).reshape(-1, 2)
+    assert np.array_equal(out, pairs[(pairs[:, 1] & np.uint64(1)) == 1])
+    keys = O.copy_keys_if_pairs(pairs.reshape(-1), p, 64)
+    assert np.array_equal(keys, pairs[(pairs[:, 1] & np.uint64(1)) == 1][:, 0])
+    x = rng.integers(0, 2**32 - 1, size=n, dtype=np.uint32, endpoint=True)
+    ex = O.exclusive_scan(x)
+    ref = np.concatenate([[0], np.cumsum(x.astype(np.uint64))[:-1]]).astype(np.uint64) & np.uint64(0xFFFFFFFF) \
+        if n else np.zeros(0, np.uint64)
+    assert np.array_equal(ex.astype(np.uint64), ref)
+    f = (rng.random(n) < 0.3).astype(np.uint8)
+    assert np.array_equal(O.copy_if_flags(x, f), x[f.astype(bool)])
