@@ -93,6 +93,8 @@ SIGNATURES = {
     "hn_exclusive_scan_u32": (I, [P, P, S, P, S, P]),
     "hn_exclusive_scan_u64": (I, [P, P, S, P, S, P]),
     "hn_version": (C.c_char_p, []),
+    # include/hashinator_amd_calib.h
+    "hn_calib_random_gather": (I, [P, S, I, S, I, C.c_uint64, P, P]),
 }
 
 _LIB = None

@@ -1,0 +1,63 @@
+// hn_calib.hip -- random-access HBM calibration (no reference counterpart).
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "hashinator_amd_calib.h"
+#include "hn_device.hpp"
+#include "hn_host.hpp"
+
+namespace hn {
+
+__device__ __forceinline__ uint64_t calib_mix(uint64_t x) {
+   x ^= x >> 30;
+   x *= 0xBF58476D1CE4E5B9ull;
+   x ^= x >> 27;
+   x *= 0x94D049BB133111EBull;
+   x ^= x >> 31;
+   return x;
+}
+
+// L lanes (gran / 16 bytes) read one aligned chunk together.
+template <int L>
+__global__ __launch_bounds__(256) void k_random_gather(const uint4* buf, uint64_t chunks, uint64_t n_access, int chain,
+                                                       uint64_t seed, uint64_t* sink) {
+   const int gl = __lane_id() & (L - 1);
+   const uint64_t groups = static_cast<uint64_t>(gridDim.x) * (blockDim.x / L);
+   uint64_t g = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / L;
+   uint64_t acc = 0;
+   for (uint64_t a = g; a * chain < n_access; a += groups) {
+      uint64_t h = calib_mix(a ^ seed);
+      for (int c = 0; c < chain; ++c) {
+         const uint64_t chunk = h % chunks;
+         const uint4 v = buf[chunk * L + gl];
+         acc += v.x ^ v.w;
+         // the next offset depends on what was read (a dependent chain)
+         h = calib_mix(h + (v.y & 1u));
+      }
+   }
+   acc = wave_sum(acc);
+   if (__lane_id() == 0 && acc == 0x123456789ull) *sink = acc; // practically never; keeps the loads
+}
+
+} // namespace hn
+
+using namespace hn;
+
+extern "C" int hn_calib_random_gather(const void* buf, size_t bytes, int gran, size_t n_access, int chain,
+                                      uint64_t seed, uint64_t* sink, hn_stream_t s) {
+   if (!buf || !sink || chain < 1) return HN_EINVAL;
+   const uint64_t chunks = bytes / static_cast<uint64_t>(gran);
+   if (chunks == 0) return HN_EINVAL;
+   const uint32_t blocks = static_cast<uint32_t>(cu_count()) * 8;
+   const uint4* b = static_cast<const uint4*>(buf);
+   switch (gran) {
+   case 16: hipLaunchKernelGGL(k_random_gather<1>, dim3(blocks), dim3(256), 0, s, b, chunks, n_access, chain, seed, sink); break;
+   case 32: hipLaunchKernelGGL(k_random_gather<2>, dim3(blocks), dim3(256), 0, s, b, chunks, n_access, chain, seed, sink); break;
+   case 64: hipLaunchKernelGGL(k_random_gather<4>, dim3(blocks), dim3(256), 0, s, b, chunks, n_access, chain, seed, sink); break;
+   case 128: hipLaunchKernelGGL(k_random_gather<8>, dim3(blocks), dim3(256), 0, s, b, chunks, n_access, chain, seed, sink); break;
+   case 256: hipLaunchKernelGGL(k_random_gather<16>, dim3(blocks), dim3(256), 0, s, b, chunks, n_access, chain, seed, sink); break;
+   default: return HN_EINVAL;
+   }
+   return static_cast<int>(hipGetLastError());
+}

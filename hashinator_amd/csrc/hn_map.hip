@@ -44,6 +44,13 @@ struct InsertArgs {
    int32_t* fail_sink;            // if set, failures are OR-ed here and err is left alone
 };
 
+// Work distribution.  Inputs are consumed in blocks of G consecutive
+// entries: lane gl of a group loads entry blk*G + gl with one coalesced read,
+// the group then resolves the block's entries one after another (the current
+// key is broadcast from its lane), and lookups keep each result in the lane
+// that owns the entry, so the block's outputs leave in one coalesced write.
+// Blocks are handed out grid-stride per group.
+
 // insert_kernel / insert_index_kernel semantics (kernels_AMD.h:136-405, 507-641)
 template <typename K, int G, int SRC>
 __global__ __launch_bounds__(1024) void k_insert(InsertArgs<K> a) {
@@ -59,45 +66,55 @@ __global__ __launch_bounds__(1024) void k_insert(InsertArgs<K> a) {
    const uint64_t W = nsl < static_cast<uint64_t>(G) ? nsl : static_cast<uint64_t>(G);
    const uint64_t len = !sp_ok ? 0 : (a.d_len ? *a.d_len : a.len);
    const uint64_t gstride = static_cast<uint64_t>(gridDim.x) * (blockDim.x / G);
-   uint64_t q = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / G;
+   uint64_t blk = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / G;
 
    uint64_t placed = 0, max_probe = 0;
    int failed = sp_ok ? 0 : 1;
-   K key = kEmpty, val = 0;
-   uint64_t base = 0, kw = 0;
+   K my_key = kEmpty, my_val = 0; // this lane's entry of the current block
+   K key = kEmpty, val = 0;       // the entry being inserted by the group
+   uint64_t pending = 0, base = 0, kw = 0;
    int64_t d0 = 0;
    bool active = false;
 
-   auto fetch = [&]() {
+   auto load_block = [&]() {
       active = false;
-      while (q < len) {
-         if constexpr (SRC == SRC_SOA) {
-            key = a.keys[q];
-            val = a.vals[q];
-         } else if constexpr (SRC == SRC_INDEX) {
-            key = a.keys[q];
-            val = static_cast<K>(q);
-         } else {
-            const Slot<K> p = a.pairs[q];
-            key = p.key;
-            val = p.val;
-         }
-         if constexpr (SRC == SRC_TABLE) {
-            if (key == kEmpty || key == kTomb) {
-               q += gstride;
-               continue;
+      while (blk * G < len) {
+         const uint64_t q = blk * G + static_cast<uint64_t>(gl);
+         bool ok = q < len;
+         if (ok) {
+            if constexpr (SRC == SRC_SOA) {
+               my_key = a.keys[q];
+               my_val = a.vals[q];
+            } else if constexpr (SRC == SRC_INDEX) {
+               my_key = a.keys[q];
+               my_val = static_cast<K>(q);
+            } else {
+               const Slot<K> p = a.pairs[q];
+               my_key = p.key;
+               my_val = p.val;
             }
+            if constexpr (SRC == SRC_TABLE) ok = my_key != kEmpty && my_key != kTomb;
          }
-         const uint64_t home = fib_home(key, sp) & mask;
-         base = home & ~(W - 1);
-         d0 = static_cast<int64_t>(home - base);
-         kw = 0;
-         active = true;
-         return;
+         pending = group_bits<G>(__ballot(ok), gbase);
+         if (pending) {
+            active = true;
+            return;
+         }
+         blk += gstride;
       }
    };
+   auto start_entry = [&]() {
+      const int j = first_bit(pending);
+      key = __shfl(my_key, gbase + j, kWave);
+      val = __shfl(my_val, gbase + j, kWave);
+      const uint64_t home = fib_home(key, sp) & mask;
+      base = home & ~(W - 1);
+      d0 = static_cast<int64_t>(home - base);
+      kw = 0;
+   };
 
-   fetch();
+   load_block();
+   if (active) start_entry();
    while (__ballot(active)) {
       if (active) {
          const uint64_t pos = kw * W + static_cast<uint64_t>(gl);
@@ -147,8 +164,12 @@ __global__ __launch_bounds__(1024) void k_insert(InsertArgs<K> a) {
             }
          }
          if (done) {
-            q += gstride;
-            fetch();
+            pending &= pending - 1;
+            if (!pending) {
+               blk += gstride;
+               load_block();
+            }
+            if (active) start_entry();
          }
       }
    }
@@ -198,29 +219,57 @@ __global__ __launch_bounds__(1024) void k_lookup(LookupArgs<K> a) {
    const uint64_t bound = ovf < nsl ? ovf : nsl;
    const uint64_t len = sp_ok ? a.len : 0;
    const uint64_t gstride = static_cast<uint64_t>(gridDim.x) * (blockDim.x / G);
-   uint64_t q = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / G;
+   uint64_t blk = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / G;
 
    uint64_t erased = 0;
+   K my_key = kEmpty, res_val = 0;
+   bool res_hit = false;
    K key = kEmpty;
-   uint64_t base = 0, kw = 0;
+   uint64_t pending = 0, base = 0, kw = 0;
    int64_t d0 = 0;
    bool active = false;
 
-   auto fetch = [&]() {
-      active = q < len;
-      if (!active) return;
-      if constexpr (PAIRS) {
-         key = a.pairs[q].key;
-      } else {
-         key = a.keys[q];
+   auto load_block = [&]() {
+      active = false;
+      if (blk * G >= len) return;
+      const uint64_t q = blk * G + static_cast<uint64_t>(gl);
+      const bool ok = q < len;
+      if (ok) {
+         if constexpr (PAIRS) {
+            my_key = a.pairs[q].key;
+         } else {
+            my_key = a.keys[q];
+         }
       }
+      res_hit = false;
+      pending = group_bits<G>(__ballot(ok), gbase);
+      active = pending != 0;
+   };
+   auto flush_block = [&]() {
+      if constexpr (!ERASE) {
+         const uint64_t q = blk * G + static_cast<uint64_t>(gl);
+         if (q < len) {
+            if (res_hit) {
+               if constexpr (PAIRS) {
+                  a.pairs[q].val = res_val;
+               } else {
+                  a.vals[q] = res_val;
+               }
+            }
+            if (a.hit) a.hit[q] = res_hit ? 1 : 0;
+         }
+      }
+   };
+   auto start_entry = [&]() {
+      key = __shfl(my_key, gbase + first_bit(pending), kWave);
       const uint64_t home = fib_home(key, sp) & mask;
       base = home & ~(W - 1);
       d0 = static_cast<int64_t>(home - base);
       kw = 0;
    };
 
-   fetch();
+   load_block();
+   if (active) start_entry();
    while (__ballot(active)) {
       if (active) {
          const uint64_t pos = kw * W + static_cast<uint64_t>(gl);
@@ -234,32 +283,30 @@ __global__ __launch_bounds__(1024) void k_lookup(LookupArgs<K> a) {
          const int fe = be ? first_bit(be) : G;
          bool done = true;
          if (fm < fe) {
-            if (gl == fm) {
-               if constexpr (ERASE) {
-                  if (atomic_cas(&sl->key, key, kTomb) == key) ++erased;
-               } else {
-                  if constexpr (PAIRS) {
-                     a.pairs[q].val = s.val;
-                  } else {
-                     a.vals[q] = s.val;
-                  }
-                  if (a.hit) a.hit[q] = 1;
+            if constexpr (ERASE) {
+               if (gl == fm && atomic_cas(&sl->key, key, kTomb) == key) ++erased;
+            } else {
+               const K v = __shfl(s.val, gbase + fm, kWave);
+               if (gl == first_bit(pending)) {
+                  res_val = v;
+                  res_hit = true;
                }
             }
          } else {
             const uint64_t covered = (kw + 1) * W - static_cast<uint64_t>(d0);
-            if (be || covered >= bound) {
-               if constexpr (!ERASE) {
-                  if (gl == 0 && a.hit) a.hit[q] = 0;
-               }
-            } else {
+            if (!be && covered < bound) {
                ++kw;
                done = false;
             }
          }
          if (done) {
-            q += gstride;
-            fetch();
+            pending &= pending - 1;
+            if (!pending) {
+               flush_block();
+               blk += gstride;
+               load_block();
+            }
+            if (active) start_entry();
          }
       }
    }

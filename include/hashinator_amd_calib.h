@@ -1,0 +1,28 @@
+/*
+ * hashinator_amd_calib.h -- measurement aids shipped with the library.
+ *
+ * No counterpart in the reference: these exist so that the random-access
+ * HBM ceiling the hash-map kernels are held against can be measured on the
+ * same device, in the same process, with the same launch machinery.
+ */
+#ifndef HASHINATOR_AMD_CALIB_H
+#define HASHINATOR_AMD_CALIB_H
+
+#include "hashinator_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Random gather: n_access reads of `gran` bytes (16, 32, 64, 128 or 256) at
+ * uniformly random gran-aligned offsets of buf[0, bytes), each read issued by
+ * gran/16 lanes as one coalesced request.  `chain` > 1 makes every lane group
+ * issue `chain` dependent reads (the next offset depends on the data read),
+ * which models a probe sequence.  The reads are folded into *sink (device). */
+HN_API int hn_calib_random_gather(const void* buf, size_t bytes, int gran, size_t n_access, int chain, uint64_t seed,
+                                  uint64_t* sink, hn_stream_t s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

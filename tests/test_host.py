@@ -12,11 +12,11 @@ from hashinator_amd import _lib
 from hashinator_amd import workloads as W
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADER = os.path.join(ROOT, "include", "hashinator_amd.h")
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("hashinator_amd.h", "hashinator_amd_calib.h")]
 
 
 def declared_symbols():
-    txt = open(HEADER).read()
+    txt = "".join(open(h).read() for h in HEADERS)
     return sorted(set(re.findall(r"^HN_API\s+[\w\s\*]+?\b(hn_\w+)\s*\(", txt, flags=re.M)))
 
 
