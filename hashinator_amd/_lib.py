@@ -76,6 +76,8 @@ SIGNATURES = {
     "hn_erase_u64": (I, [P, P, P, S, P, TUNE]),
     "hn_reset_all_u32": (I, [P, P, S, P]),
     "hn_reset_all_u64": (I, [P, P, S, P]),
+    "hn_reset_u32": (I, [P, P, P, S, P]),
+    "hn_reset_u64": (I, [P, P, P, S, P]),
     "hn_clean_tombstones_scratch_bytes_u32": (S, [I, S]),
     "hn_clean_tombstones_scratch_bytes_u64": (S, [I, S]),
     "hn_clean_tombstones_min_scratch": (S, []),
@@ -90,6 +92,11 @@ SIGNATURES = {
     "hn_copy_keys_if_pairs_u32": (I, [P, S, PRED, P, P, P, S, P]),
     "hn_copy_keys_if_pairs_u64": (I, [P, S, PRED, P, P, P, S, P]),
     "hn_copy_if_flags": (I, [P, P, S, S, P, P, P, S, P]),
+    "hn_copy_keys_if_flags_u32": (I, [P, P, S, P, P, P, S, P]),
+    "hn_copy_if_bits": (I, [P, P, S, S, P, P, P, S, P]),
+    "hn_copy_keys_if_bits_u32": (I, [P, P, S, P, P, P, S, P]),
+    "hn_copy_keys_if_bits_u64": (I, [P, P, S, P, P, P, S, P]),
+    "hn_copy_keys_if_flags_u64": (I, [P, P, S, P, P, P, S, P]),
     "hn_exclusive_scan_u32": (I, [P, P, S, P, S, P]),
     "hn_exclusive_scan_u64": (I, [P, P, S, P, S, P]),
     "hn_version": (C.c_char_p, []),

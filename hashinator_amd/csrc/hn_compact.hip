@@ -125,6 +125,13 @@ struct FlagSel {
    __device__ bool operator()(uint64_t i, const T&) const { return flags[i] != 0; }
 };
 
+// Rule already evaluated into a bit mask (bit i of word i/32, LSB first).
+template <typename T>
+struct BitSel {
+   const uint32_t* bits;
+   __device__ bool operator()(uint64_t i, const T&) const { return (bits[i >> 5] >> (i & 31)) & 1u; }
+};
+
 template <typename T>
 struct Whole {
    __device__ T operator()(const T& x) const { return x; }
@@ -365,6 +372,48 @@ int hn_copy_if_flags(const void* in, const uint8_t* flags, size_t n, size_t eb, 
                          FlagSel<Slot<uint64_t>>{flags}, Whole<Slot<uint64_t>>{});
    default: return HN_EINVAL;
    }
+}
+
+int hn_copy_if_bits(const void* in, const uint32_t* bits, size_t n, size_t eb, void* out, uint64_t* c, void* sc,
+                    size_t sb, hn_stream_t s) {
+   if (n && !bits) return HN_EINVAL;
+   switch (eb) {
+   case 4:
+      return run_compact(static_cast<const uint32_t*>(in), n, static_cast<uint32_t*>(out), c, sc, sb, s,
+                         BitSel<uint32_t>{bits}, Whole<uint32_t>{});
+   case 8:
+      return run_compact(static_cast<const uint64_t*>(in), n, static_cast<uint64_t*>(out), c, sc, sb, s,
+                         BitSel<uint64_t>{bits}, Whole<uint64_t>{});
+   case 16:
+      return run_compact(static_cast<const Slot<uint64_t>*>(in), n, static_cast<Slot<uint64_t>*>(out), c, sc, sb, s,
+                         BitSel<Slot<uint64_t>>{bits}, Whole<Slot<uint64_t>>{});
+   default: return HN_EINVAL;
+   }
+}
+int hn_copy_keys_if_bits_u32(const void* in, const uint32_t* bits, size_t n, uint32_t* out, uint64_t* c, void* sc,
+                             size_t sb, hn_stream_t s) {
+   if (n && !bits) return HN_EINVAL;
+   return run_compact(static_cast<const Slot<uint32_t>*>(in), n, out, c, sc, sb, s, BitSel<Slot<uint32_t>>{bits},
+                      KeyOf<uint32_t>{});
+}
+int hn_copy_keys_if_bits_u64(const void* in, const uint32_t* bits, size_t n, uint64_t* out, uint64_t* c, void* sc,
+                             size_t sb, hn_stream_t s) {
+   if (n && !bits) return HN_EINVAL;
+   return run_compact(static_cast<const Slot<uint64_t>*>(in), n, out, c, sc, sb, s, BitSel<Slot<uint64_t>>{bits},
+                      KeyOf<uint64_t>{});
+}
+
+int hn_copy_keys_if_flags_u32(const void* in, const uint8_t* flags, size_t n, uint32_t* out, uint64_t* c, void* sc,
+                              size_t sb, hn_stream_t s) {
+   if (n && !flags) return HN_EINVAL;
+   return run_compact(static_cast<const Slot<uint32_t>*>(in), n, out, c, sc, sb, s, FlagSel<Slot<uint32_t>>{flags},
+                      KeyOf<uint32_t>{});
+}
+int hn_copy_keys_if_flags_u64(const void* in, const uint8_t* flags, size_t n, uint64_t* out, uint64_t* c, void* sc,
+                              size_t sb, hn_stream_t s) {
+   if (n && !flags) return HN_EINVAL;
+   return run_compact(static_cast<const Slot<uint64_t>*>(in), n, out, c, sc, sb, s, FlagSel<Slot<uint64_t>>{flags},
+                      KeyOf<uint64_t>{});
 }
 
 int hn_exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t n, void* sc, size_t sb, hn_stream_t s) {

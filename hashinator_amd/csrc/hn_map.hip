@@ -200,10 +200,17 @@ struct LookupArgs {
    uint64_t len;
 };
 
-// retrieve_kernel (kernels_AMD.h:647-752) for ERASE == false,
-// delete_kernel (kernels_AMD.h:412-505) with CAS for ERASE == true.
-template <typename K, int G, bool PAIRS, bool ERASE>
+enum LookupMode { LK_RETRIEVE = 0, LK_ERASE = 1, LK_RESET = 2 };
+
+// LK_RETRIEVE: retrieve_kernel (kernels_AMD.h:647-752).
+// LK_ERASE:    delete_kernel (kernels_AMD.h:412-505), with CAS key -> TOMBSTONE.
+// LK_RESET:    reset_to_empty (kernels_AMD.h:65-134): the first slot holding
+//              the key (whole table, EMPTY does not stop the probe) -> EMPTY,
+//              fill -= len.
+template <typename K, int G, bool PAIRS, int MODE>
 __global__ __launch_bounds__(1024) void k_lookup(LookupArgs<K> a) {
+   constexpr bool ERASE = MODE == LK_ERASE;
+   constexpr bool RESET = MODE == LK_RESET;
    constexpr K kEmpty = empty_key<K>();
    constexpr K kTomb = tomb_key<K>();
    const int lane = __lane_id();
@@ -215,7 +222,7 @@ __global__ __launch_bounds__(1024) void k_lookup(LookupArgs<K> a) {
    const uint64_t mask = nsl - 1;
    const uint64_t W = nsl < static_cast<uint64_t>(G) ? nsl : static_cast<uint64_t>(G);
    // Offsets past 2^sp only revisit slots already seen, so the bound is clamped there.
-   const uint64_t ovf = a.info->current_max_bucket_overflow;
+   const uint64_t ovf = RESET ? nsl : a.info->current_max_bucket_overflow;
    const uint64_t bound = ovf < nsl ? ovf : nsl;
    const uint64_t len = sp_ok ? a.len : 0;
    const uint64_t gstride = static_cast<uint64_t>(gridDim.x) * (blockDim.x / G);
@@ -246,7 +253,7 @@ __global__ __launch_bounds__(1024) void k_lookup(LookupArgs<K> a) {
       active = pending != 0;
    };
    auto flush_block = [&]() {
-      if constexpr (!ERASE) {
+      if constexpr (MODE == LK_RETRIEVE) {
          const uint64_t q = blk * G + static_cast<uint64_t>(gl);
          if (q < len) {
             if (res_hit) {
@@ -278,13 +285,15 @@ __global__ __launch_bounds__(1024) void k_lookup(LookupArgs<K> a) {
          Slot<K>* sl = a.buckets + ((base + pos) & mask);
          const Slot<K> s = *sl;
          const uint64_t bm = group_bits<G>(__ballot(valid && s.key == key), gbase);
-         const uint64_t be = group_bits<G>(__ballot(valid && s.key == kEmpty), gbase);
+         const uint64_t be = RESET ? 0 : group_bits<G>(__ballot(valid && s.key == kEmpty), gbase);
          const int fm = bm ? first_bit(bm) : G;
          const int fe = be ? first_bit(be) : G;
          bool done = true;
          if (fm < fe) {
             if constexpr (ERASE) {
                if (gl == fm && atomic_cas(&sl->key, key, kTomb) == key) ++erased;
+            } else if constexpr (RESET) {
+               if (gl == fm) atomic_cas(&sl->key, key, kEmpty);
             } else {
                const K v = __shfl(s.val, gbase + fm, kWave);
                if (gl == first_bit(pending)) {
@@ -317,6 +326,10 @@ __global__ __launch_bounds__(1024) void k_lookup(LookupArgs<K> a) {
          atomic_add_relaxed(&a.info->tombstone_counter, erased);
          __hip_atomic_fetch_sub(&a.info->fill, erased, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
+   }
+   if constexpr (RESET) {
+      if (blockIdx.x == 0 && threadIdx.x == 0)
+         __hip_atomic_fetch_sub(&a.info->fill, a.len, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
    }
 }
 
@@ -521,19 +534,19 @@ InsertKernel<K, SRC> pick_insert(int G) {
    }
 }
 
-template <typename K, bool PAIRS, bool ERASE>
+template <typename K, bool PAIRS, int MODE>
 using LookupKernel = void (*)(LookupArgs<K>);
 
-template <typename K, bool PAIRS, bool ERASE>
-LookupKernel<K, PAIRS, ERASE> pick_lookup(int G) {
+template <typename K, bool PAIRS, int MODE>
+LookupKernel<K, PAIRS, MODE> pick_lookup(int G) {
    switch (G) {
-   case 1: return k_lookup<K, 1, PAIRS, ERASE>;
-   case 2: return k_lookup<K, 2, PAIRS, ERASE>;
-   case 4: return k_lookup<K, 4, PAIRS, ERASE>;
-   case 8: return k_lookup<K, 8, PAIRS, ERASE>;
-   case 16: return k_lookup<K, 16, PAIRS, ERASE>;
-   case 32: return k_lookup<K, 32, PAIRS, ERASE>;
-   case 64: return k_lookup<K, 64, PAIRS, ERASE>;
+   case 1: return k_lookup<K, 1, PAIRS, MODE>;
+   case 2: return k_lookup<K, 2, PAIRS, MODE>;
+   case 4: return k_lookup<K, 4, PAIRS, MODE>;
+   case 8: return k_lookup<K, 8, PAIRS, MODE>;
+   case 16: return k_lookup<K, 16, PAIRS, MODE>;
+   case 32: return k_lookup<K, 32, PAIRS, MODE>;
+   case 64: return k_lookup<K, 64, PAIRS, MODE>;
    default: return nullptr;
    }
 }
@@ -549,12 +562,12 @@ int launch_insert(InsertArgs<K> a, uint64_t grid_len, hipStream_t s, const hn_tu
    return static_cast<int>(hipGetLastError());
 }
 
-template <typename K, bool PAIRS, bool ERASE>
+template <typename K, bool PAIRS, int MODE>
 int launch_lookup(LookupArgs<K> a, hipStream_t s, const hn_tune* t, OpKind op) {
    Launch L;
    int rc = resolve_launch<K>(t, op, a.len, &L);
    if (rc) return rc;
-   LookupKernel<K, PAIRS, ERASE> k = pick_lookup<K, PAIRS, ERASE>(L.probe_width);
+   LookupKernel<K, PAIRS, MODE> k = pick_lookup<K, PAIRS, MODE>(L.probe_width);
    if (!k) return HN_EINVAL;
    hipLaunchKernelGGL(k, dim3(L.blocks), dim3(L.block_size), 0, s, a);
    return static_cast<int>(hipGetLastError());
@@ -590,7 +603,7 @@ int retrieve_soa(const K* keys, K* vals, uint8_t* hit, const void* buckets, cons
    if (len == 0) return 0;
    LookupArgs<K> a{keys, vals, nullptr, hit, const_cast<Slot<K>*>(static_cast<const Slot<K>*>(buckets)),
                    const_cast<hn_info*>(info), len};
-   return launch_lookup<K, false, false>(a, s, t, OP_RETRIEVE);
+   return launch_lookup<K, false, LK_RETRIEVE>(a, s, t, OP_RETRIEVE);
 }
 
 template <typename K>
@@ -600,7 +613,7 @@ int retrieve_pairs(void* pairs, uint8_t* hit, const void* buckets, const hn_info
    if (len == 0) return 0;
    LookupArgs<K> a{nullptr, nullptr, static_cast<Slot<K>*>(pairs), hit,
                    const_cast<Slot<K>*>(static_cast<const Slot<K>*>(buckets)), const_cast<hn_info*>(info), len};
-   return launch_lookup<K, true, false>(a, s, t, OP_RETRIEVE);
+   return launch_lookup<K, true, LK_RETRIEVE>(a, s, t, OP_RETRIEVE);
 }
 
 template <typename K>
@@ -608,7 +621,16 @@ int erase_keys(const K* keys, void* buckets, hn_info* info, size_t len, hipStrea
    if (!buckets || !info || (len && !keys)) return HN_EINVAL;
    if (len == 0) return 0;
    LookupArgs<K> a{keys, nullptr, nullptr, nullptr, static_cast<Slot<K>*>(buckets), info, len};
-   return launch_lookup<K, false, true>(a, s, t, OP_ERASE);
+   return launch_lookup<K, false, LK_ERASE>(a, s, t, OP_ERASE);
+}
+
+template <typename K>
+int reset_keys(const void* pairs, void* buckets, hn_info* info, size_t len, hipStream_t s) {
+   if (!buckets || !info || (len && !pairs)) return HN_EINVAL;
+   if (len == 0) return 0;
+   LookupArgs<K> a{nullptr, nullptr, const_cast<Slot<K>*>(static_cast<const Slot<K>*>(pairs)), nullptr,
+                   static_cast<Slot<K>*>(buckets), info, len};
+   return launch_lookup<K, true, LK_RESET>(a, s, nullptr, OP_ERASE);
 }
 
 template <typename K>
@@ -768,6 +790,8 @@ int hn_erase_u64(const uint64_t* k, void* b, hn_info* i, size_t n, hn_stream_t s
    return erase_keys<uint64_t>(k, b, i, n, s, t);
 }
 int hn_reset_all_u32(void* b, hn_info* i, size_t n, hn_stream_t s) { return reset_all<uint32_t>(b, i, n, s); }
+int hn_reset_u32(const void* p, void* b, hn_info* i, size_t n, hn_stream_t s) { return reset_keys<uint32_t>(p, b, i, n, s); }
+int hn_reset_u64(const void* p, void* b, hn_info* i, size_t n, hn_stream_t s) { return reset_keys<uint64_t>(p, b, i, n, s); }
 int hn_reset_all_u64(void* b, hn_info* i, size_t n, hn_stream_t s) { return reset_all<uint64_t>(b, i, n, s); }
 
 size_t hn_clean_tombstones_min_scratch(void) { return 64 * 1024; }

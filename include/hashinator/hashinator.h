@@ -1,0 +1,517 @@
+/* hashinator.h -- Hashinator::Hashmap for the MI355X (gfx950) build.
+ *
+ * Drop-in for the accelerated host API of kstppd/hashinator
+ * include/hashinator/hashinator.h (Hashmap, 59-1802): the same class
+ * template, method names, argument meaning, synchronous behaviour and
+ * auto-grow rule, over libhashinator_amd's C-ABI through the DeviceHasher
+ * plugin (hashers.h).
+ *
+ * Residency.  The reference keeps the buckets and the Info in unified
+ * memory and reads them from the host.  MI355X without HMM serves
+ * hipMallocManaged from host memory (hipDeviceAttributeManagedMemory = 0),
+ * where the probe kernels run three orders of magnitude slower, so here the
+ * bucket array and the Info live in device memory.  The host keeps a mirror
+ * of the Info, refreshed after every call (each call synchronises its stream,
+ * as in the reference), and host-side lookups (find, count, iteration) read
+ * a host copy of the buckets taken on first use after a change.
+ * expose_bucketdata() returns the device pointer; expose_mapinfo() the host
+ * mirror (call pushInfo() after editing it).
+ *
+ * Keys, values and pairs handed to insert/retrieve/erase may be device
+ * memory (fastest) or unified memory such as a split::SplitVector.
+ *
+ * Instantiations: Hashmap<uint32_t,uint32_t> and Hashmap<uint64_t,uint64_t>
+ * with the default sentinels and hash.
+ *
+ * Not part of this build (the reference's device-only API and host-side
+ * mutation): warpInsert/warpFind/warpErase, device_insert/device_find/
+ * device_erase and device iterators, upload/download handles, and the host
+ * insertion path (_at, operator[], erase(key) on the host).
+ */
+#pragma once
+#include <cmath>
+#include <cstdio>
+#include <limits>
+#include <stdexcept>
+#include <vector>
+
+#include "../common.h"
+#include "../splitvector/split_tools.h"
+#include "../splitvector/splitvec.h"
+#include "defaults.h"
+#include "hash_pair.h"
+#include "hashers.h"
+#include "hashfunctions.h"
+
+namespace Hashinator {
+
+template <typename T>
+using DefaultMetaAllocator = split::split_unified_allocator<T>;
+#define DefaultHasher                                                                                                  \
+   Hashers::Hasher<KEY_TYPE, VAL_TYPE, HashFunction, EMPTYBUCKET, TOMBSTONE, defaults::WARPSIZE,                       \
+                   defaults::elementsPerWarp>
+
+using MapInfo = Hashinator::Info;
+
+template <typename KEY_TYPE, typename VAL_TYPE, KEY_TYPE EMPTYBUCKET = std::numeric_limits<KEY_TYPE>::max(),
+          KEY_TYPE TOMBSTONE = EMPTYBUCKET - 1, class HashFunction = HashFunctions::Fibonacci<KEY_TYPE>,
+          class DeviceHasher = DefaultHasher, class Meta_Allocator = DefaultMetaAllocator<MapInfo>>
+class Hashmap {
+   using pair_t = hash_pair<KEY_TYPE, VAL_TYPE>;
+   static constexpr bool W64 = sizeof(KEY_TYPE) == 8;
+
+   pair_t* _buckets = nullptr; // device memory, 2^sizePower slots
+   size_t _nbuckets = 0;
+   MapInfo* _dinfo = nullptr;  // device memory
+   mutable MapInfo _info;      // host mirror of *_dinfo
+   mutable std::vector<pair_t> _view;
+   mutable bool _view_ok = false;
+   hn_tune _tune{0, 0, 0, 0};
+   bool _has_tune = false;
+
+   uint32_t hash(KEY_TYPE in) const { return static_cast<uint32_t>(HashFunction::_hash(in, _info.sizePower)); }
+   const hn_tune* tune() const { return _has_tune ? &_tune : nullptr; }
+   hn_info* dinfo() const { return reinterpret_cast<hn_info*>(_dinfo); }
+   static hn_stream_t st(hipStream_t s) { return reinterpret_cast<hn_stream_t>(s); }
+   static void chk(int rc) { Hashers::detail::check(rc); }
+
+   static pair_t* new_table(size_t n, hipStream_t s) {
+      pair_t* p = nullptr;
+      SPLIT_CHECK_ERR(hipMalloc(&p, n * sizeof(pair_t)));
+      chk(W64 ? hn_table_init_u64(p, n, st(s)) : hn_table_init_u32(p, n, st(s)));
+      return p;
+   }
+   void pull(hipStream_t s) const {
+      SPLIT_CHECK_ERR(hipStreamSynchronize(s));
+      SPLIT_CHECK_ERR(hipMemcpy(&_info, _dinfo, sizeof(MapInfo), hipMemcpyDeviceToHost));
+   }
+   void changed(hipStream_t s) {
+      _view_ok = false;
+      pull(s);
+   }
+   void set_status(status code) {
+      _info.err = code;
+      pushInfo();
+   }
+   void alloc(int sizePower) {
+      _nbuckets = size_t(1) << sizePower;
+      _buckets = new_table(_nbuckets, 0);
+      SPLIT_CHECK_ERR(hipMalloc(&_dinfo, sizeof(MapInfo)));
+      _info = MapInfo(sizePower);
+      pushInfo();
+   }
+   void release() {
+      if (_buckets) SPLIT_CHECK_ERR(hipFree(_buckets));
+      if (_dinfo) SPLIT_CHECK_ERR(hipFree(_dinfo));
+      _buckets = nullptr;
+      _dinfo = nullptr;
+      _nbuckets = 0;
+   }
+   const std::vector<pair_t>& view() const {
+      if (!_view_ok) {
+         _view.resize(_nbuckets);
+         SPLIT_CHECK_ERR(hipDeviceSynchronize());
+         SPLIT_CHECK_ERR(hipMemcpy(_view.data(), _buckets, _nbuckets * sizeof(pair_t), hipMemcpyDeviceToHost));
+         _view_ok = true;
+      }
+      return _view;
+   }
+
+public:
+   Hashmap() : Hashmap(5) {}
+   Hashmap(int sizepower) { alloc(sizepower); }
+   Hashmap(const Hashmap& other) {
+      alloc(other._info.sizePower);
+      SPLIT_CHECK_ERR(hipMemcpy(_buckets, other._buckets, _nbuckets * sizeof(pair_t), hipMemcpyDeviceToDevice));
+      other.pull(0);
+      _info = other._info;
+      pushInfo();
+      _tune = other._tune;
+      _has_tune = other._has_tune;
+   }
+   Hashmap(Hashmap&& other) noexcept
+       : _buckets(other._buckets), _nbuckets(other._nbuckets), _dinfo(other._dinfo), _info(other._info),
+         _tune(other._tune), _has_tune(other._has_tune) {
+      other._buckets = nullptr;
+      other._dinfo = nullptr;
+      other._nbuckets = 0;
+   }
+   Hashmap& operator=(const Hashmap& other) {
+      if (this == &other) return *this;
+      overwrite(other, 0);
+      SPLIT_CHECK_ERR(hipStreamSynchronize(0));
+      return *this;
+   }
+   Hashmap& operator=(Hashmap&& other) noexcept {
+      if (this == &other) return *this;
+      release();
+      std::swap(_buckets, other._buckets);
+      std::swap(_dinfo, other._dinfo);
+      std::swap(_nbuckets, other._nbuckets);
+      _info = other._info;
+      _view_ok = false;
+      return *this;
+   }
+   ~Hashmap() { release(); }
+
+   /* Copy-assign on a stream (hashinator.h:165-174). */
+   void overwrite(const Hashmap& other, hipStream_t stream = 0) {
+      if (this == &other) return;
+      if (_nbuckets != other._nbuckets) {
+         if (_buckets) SPLIT_CHECK_ERR(hipFree(_buckets));
+         _nbuckets = other._nbuckets;
+         SPLIT_CHECK_ERR(hipMalloc(&_buckets, _nbuckets * sizeof(pair_t)));
+      }
+      SPLIT_CHECK_ERR(hipMemcpyAsync(_buckets, other._buckets, _nbuckets * sizeof(pair_t), hipMemcpyDeviceToDevice,
+                                     stream));
+      SPLIT_CHECK_ERR(hipMemcpyAsync(_dinfo, other._dinfo, sizeof(MapInfo), hipMemcpyDeviceToDevice, stream));
+      changed(stream);
+   }
+   /* Copy of the Info to any host or device address (hashinator.h:228-230). */
+   void copyMetadata(MapInfo* dst, hipStream_t s = 0) {
+      SPLIT_CHECK_ERR(hipMemcpyAsync(dst, _dinfo, sizeof(MapInfo), hipMemcpyDefault, s));
+   }
+   /* Write the host mirror back to the device (after editing expose_mapinfo()). */
+   void pushInfo() { SPLIT_CHECK_ERR(hipMemcpy(_dinfo, &_info, sizeof(MapInfo), hipMemcpyHostToDevice)); }
+
+   /* Launch tuning for this map's calls (an addition): probe width 1..64
+    * lanes, block size, grid cap; zeros mean the library defaults. */
+   void setTune(const hn_tune& t) {
+      _tune = t;
+      _has_tune = true;
+   }
+
+   /* ------------------------------------------------------------ rehash / resize */
+
+   /* Host rehash (hashinator.h:236-279): the serial algorithm with the
+    * 32-probe placement limit, retried one size larger on failure; the result
+    * is written back to the device. */
+   void rehash(int newSizePower) {
+      const std::vector<pair_t>& old = view();
+      for (;;) {
+         if (newSizePower > 32)
+            throw std::out_of_range("Hashmap ran into rehashing catastrophe and exceeded 32bit buckets.");
+         std::vector<pair_t> nb(size_t(1) << newSizePower, pair_t(EMPTYBUCKET, VAL_TYPE()));
+         const size_t bitMask = nb.size() - 1;
+         bool ok = true;
+         for (size_t e = 0; e < old.size() && ok; ++e) {
+            const pair_t el = old[e];
+            if (el.first == EMPTYBUCKET || el.first == TOMBSTONE) continue;
+            const uint32_t h = static_cast<uint32_t>(HashFunction::_hash(el.first, newSizePower));
+            bool found = false;
+            for (uint32_t i = 0; i < 32; ++i) { // the CPU BUCKET_OVERFLOW of the reference
+               pair_t& c = nb[static_cast<uint32_t>(h + i) & bitMask];
+               if (c.first == EMPTYBUCKET) {
+                  c = el;
+                  found = true;
+                  break;
+               }
+            }
+            ok = found;
+         }
+         if (!ok) {
+            ++newSizePower;
+            continue;
+         }
+         SPLIT_CHECK_ERR(hipFree(_buckets));
+         _nbuckets = nb.size();
+         SPLIT_CHECK_ERR(hipMalloc(&_buckets, _nbuckets * sizeof(pair_t)));
+         SPLIT_CHECK_ERR(hipMemcpy(_buckets, nb.data(), _nbuckets * sizeof(pair_t), hipMemcpyHostToDevice));
+         _info.sizePower = newSizePower;
+         _info.currentMaxBucketOverflow = defaults::BUCKET_OVERFLOW;
+         _info.tombstoneCounter = 0;
+         pushInfo();
+         _view.swap(nb);
+         _view_ok = true;
+         return;
+      }
+   }
+
+   /* device_rehash (hashinator.h:284-336): every live element moves into a
+    * fresh table of 2^newSizePower slots (a new allocation, also when the
+    * size is unchanged); err = success iff the fill is preserved. */
+   template <bool prefetches = true>
+   void device_rehash(int newSizePower, hipStream_t s = 0) {
+      if (newSizePower > 32)
+         throw std::out_of_range("Hashmap ran into rehashing catastrophe and exceeded 32bit buckets.");
+      pull(s);
+      const size_t fill = _info.fill;
+      // insert(valid, n, targetLF = 1) grows further when the elements do not fit (hashinator.h:332, 1329)
+      if (fill > 0) {
+         const int need = static_cast<int>(std::ceil(std::log2(static_cast<double>(fill))));
+         if (need > newSizePower) newSizePower = need;
+      }
+      pair_t* fresh = nullptr;
+      SPLIT_CHECK_ERR(hipMalloc(&fresh, (size_t(1) << newSizePower) * sizeof(pair_t)));
+      void* scratch = nullptr;
+      SPLIT_CHECK_ERR(hipMallocAsync(&scratch, hn_rehash_scratch_bytes(), s));
+      chk(W64 ? hn_rehash_u64(_buckets, _info.sizePower, fresh, newSizePower, dinfo(), scratch, st(s))
+              : hn_rehash_u32(_buckets, _info.sizePower, fresh, newSizePower, dinfo(), scratch, st(s)));
+      SPLIT_CHECK_ERR(hipFreeAsync(scratch, s));
+      SPLIT_CHECK_ERR(hipStreamSynchronize(s));
+      SPLIT_CHECK_ERR(hipFree(_buckets));
+      _buckets = fresh;
+      _nbuckets = size_t(1) << newSizePower;
+      changed(s);
+   }
+
+   template <bool prefetches = true>
+   void clear(targets t = targets::host, hipStream_t s = 0, size_t len = 0) {
+      if (t == targets::device) {
+         if (len == 0) len = _nbuckets;
+         DeviceHasher::reset_all(_buckets, _dinfo, len, s);
+      } else {
+         chk(W64 ? hn_table_init_u64(_buckets, _nbuckets, st(s)) : hn_table_init_u32(_buckets, _nbuckets, st(s)));
+         chk(hn_info_init(dinfo(), _info.sizePower, st(s)));
+      }
+      changed(s);
+   }
+
+   void resize_to_lf(float targetLF = 0.5, targets t = targets::host, hipStream_t s = 0) {
+      while (load_factor() > targetLF) resize(_info.sizePower + 1, t, s);
+   }
+   void resize(int newSizePower, targets t = targets::host, hipStream_t s = 0) {
+      if (t == targets::device)
+         device_rehash(newSizePower, s);
+      else
+         rehash(newSizePower);
+   }
+
+   /* ------------------------------------------------------------ metadata (hashinator.h:434-620) */
+   status peek_status(void) noexcept {
+      const status r = _info.err;
+      _info.err = status::invalid;
+      pushInfo();
+      return r;
+   }
+   int getSizePower(void) const noexcept { return _info.sizePower; }
+   size_t size() const { return _info.fill; }
+   size_t bucket_count() const { return _nbuckets; }
+   constexpr KEY_TYPE get_emptybucket() const { return EMPTYBUCKET; }
+   constexpr KEY_TYPE get_tombstone() const { return TOMBSTONE; }
+   float load_factor() const { return (float)size() / bucket_count(); }
+   size_t tombstone_count() const { return _info.tombstoneCounter; }
+   float tombstone_ratio() const {
+      if (tombstone_count() == 0) return 0.0;
+      return (float)_info.tombstoneCounter / (float)_nbuckets;
+   }
+   template <bool warn = true>
+   MapInfo* expose_mapinfo() noexcept {
+      if (warn) printf("Warning, exposing Hashmap internal info struct!\n");
+      return &_info;
+   }
+   template <bool warn = true>
+   pair_t* expose_bucketdata() noexcept {
+      if (warn) printf("Warning, exposing Hashmap internal bucket data!\n");
+      _view_ok = false; // the caller may write through it
+      return _buckets;
+   }
+   MapInfo* device_info() noexcept { return _dinfo; }
+   void stats() const {
+      printf("Hashinator Stats \n");
+      printf("Bucket size= %lu\n", (unsigned long)_nbuckets);
+      printf("Fill= %lu, LoadFactor=%f \n", (unsigned long)_info.fill, load_factor());
+      printf("Tombstones= %lu\n", (unsigned long)_info.tombstoneCounter);
+      printf("Overflow= %lu\n", (unsigned long)_info.currentMaxBucketOverflow);
+   }
+   void swap(Hashmap& other) noexcept {
+      std::swap(_buckets, other._buckets);
+      std::swap(_nbuckets, other._nbuckets);
+      std::swap(_dinfo, other._dinfo);
+      std::swap(_info, other._info);
+      _view_ok = other._view_ok = false;
+   }
+
+   /* GPU performCleanupTasks (hashinator.h:642-650). */
+   template <bool prefetches = true>
+   void performCleanupTasks(hipStream_t s = 0) {
+      if (tombstone_ratio() > 0.025) clean_tombstones<prefetches>(s);
+      while (_info.currentMaxBucketOverflow > static_cast<size_t>(defaults::BUCKET_OVERFLOW))
+         device_rehash<prefetches>(_info.sizePower + 1, s);
+   }
+
+   /* ------------------------------------------------------------ host read-only lookups */
+   class const_iterator {
+      const Hashmap* h;
+      size_t index;
+
+   public:
+      explicit const_iterator(const Hashmap& hashtable, size_t i) : h(&hashtable), index(i) {}
+      const_iterator& operator++() {
+         const std::vector<pair_t>& b = h->view();
+         index++;
+         while (index < b.size() && (b[index].first == EMPTYBUCKET || b[index].first == TOMBSTONE)) index++;
+         return *this;
+      }
+      const_iterator operator++(int) {
+         const_iterator t = *this;
+         ++(*this);
+         return t;
+      }
+      bool operator==(const_iterator o) const { return h == o.h && index == o.index; }
+      bool operator!=(const_iterator o) const { return !(*this == o); }
+      const pair_t& operator*() const { return h->view()[index]; }
+      const pair_t* operator->() const { return &h->view()[index]; }
+      size_t getIndex() { return index; }
+   };
+
+   /* const find (hashinator.h:742-768) over the host copy of the buckets. */
+   const const_iterator find(KEY_TYPE key) const {
+      const std::vector<pair_t>& b = view();
+      const size_t bitMask = b.size() - 1;
+      const uint32_t h = hash(key);
+      for (size_t i = 0; i < b.size(); i++) {
+         const size_t idx = (static_cast<size_t>(h) + i) & bitMask;
+         const KEY_TYPE c = b[idx].first;
+         if (c == TOMBSTONE) continue;
+         if (c == key) return const_iterator(*this, idx);
+         if (c == EMPTYBUCKET) return end();
+      }
+      return end();
+   }
+   size_t count(const KEY_TYPE& key) const { return find(key) != end() ? 1 : 0; }
+   const_iterator begin() const {
+      const std::vector<pair_t>& b = view();
+      for (size_t i = 0; i < b.size(); i++)
+         if (b[i].first != EMPTYBUCKET && b[i].first != TOMBSTONE) return const_iterator(*this, i);
+      return end();
+   }
+   const_iterator end() const { return const_iterator(*this, _nbuckets); }
+
+   /* ------------------------------------------------------------ residency (hashinator.h:1127-1444)
+    * The table never leaves device memory; these keep the reference's names. */
+   void optimizeGPU(hipStream_t stream = 0) noexcept { (void)stream; }
+   void optimizeCPU(hipStream_t stream = 0) noexcept {
+      SPLIT_CHECK_ERR(hipStreamSynchronize(stream));
+      _view_ok = false;
+      (void)view();
+   }
+   void memAdvise(hipMemoryAdvise, int, hipStream_t = 0) {}
+   void streamAttach(hipStream_t, uint32_t = hipMemAttachSingle) {}
+
+   /* ------------------------------------------------------------ extraction (hashinator.h:1154-1259) */
+   template <bool prefetches = true, typename Rule>
+   size_t extractPattern(split::SplitVector<pair_t>& elements, Rule rule, hipStream_t s = 0) {
+      elements.resize(_info.fill + 1, true);
+      split::tools::splitStackArena mPool(split::tools::estimateMemoryForCompaction(_nbuckets), s);
+      const size_t len = split::tools::copy_if_raw(_buckets, elements.data(), _nbuckets, rule, mPool, s);
+      elements.erase(elements.data() + len, elements.end());
+      return elements.size();
+   }
+   template <typename Rule, int BLOCKSIZE = 1024>
+   size_t extractPattern(pair_t* elements, Rule rule, hipStream_t s = 0) {
+      split::tools::splitStackArena mPool(split::tools::estimateMemoryForCompaction(_nbuckets), s);
+      return split::tools::copy_if_raw(_buckets, elements, _nbuckets, rule, mPool, s);
+   }
+   template <bool prefetches = true, typename Rule>
+   size_t extractKeysByPattern(split::SplitVector<KEY_TYPE>& elements, Rule rule, hipStream_t s = 0) {
+      elements.resize(_info.fill + 1, true);
+      split::tools::splitStackArena mPool(split::tools::estimateMemoryForCompaction(_nbuckets), s);
+      const size_t len = split::tools::copy_keys_if_raw(_buckets, elements.data(), _nbuckets, rule, mPool, s);
+      elements.erase(elements.data() + len, elements.end());
+      return elements.size();
+   }
+   template <bool prefetches = true, typename Rule>
+   size_t extractKeysByPattern(split::SplitVector<KEY_TYPE>& elements, Rule rule, void* stack, size_t max_size,
+                               hipStream_t s = 0) {
+      elements.resize(_info.fill + 1, true);
+      split::tools::splitStackArena mPool(stack, max_size);
+      const size_t len = split::tools::copy_keys_if_raw(_buckets, elements.data(), _nbuckets, rule, mPool, s);
+      elements.erase(elements.data() + len, elements.end());
+      return elements.size();
+   }
+   /* All live keys, through the library's built-in predicate (no rule staging). */
+   template <bool prefetches = true>
+   size_t extractAllKeys(split::SplitVector<KEY_TYPE>& elements, hipStream_t s = 0) {
+      elements.resize(_info.fill + 1, true);
+      const size_t sb = hn_compact_scratch_bytes(_nbuckets);
+      split::tools::splitStackArena mPool(sb + 512, s);
+      uint64_t* d_count = static_cast<uint64_t*>(mPool.allocate(8));
+      void* scratch = mPool.allocate(sb);
+      const hn_pred p{HN_PRED_VALID, 0, 0, 0};
+      chk(W64 ? hn_copy_keys_if_pairs_u64(_buckets, _nbuckets, &p, reinterpret_cast<uint64_t*>(elements.data()),
+                                          d_count, scratch, sb, st(s))
+              : hn_copy_keys_if_pairs_u32(_buckets, _nbuckets, &p, reinterpret_cast<uint32_t*>(elements.data()),
+                                          d_count, scratch, sb, st(s)));
+      const size_t len = split::tools::detail::read_count(d_count, s);
+      elements.erase(elements.data() + len, elements.end());
+      return elements.size();
+   }
+   template <bool prefetches = true>
+   size_t extractAllKeys(split::SplitVector<KEY_TYPE>& elements, void* stack, size_t max_size, hipStream_t s = 0) {
+      auto rule = [] __host__ __device__(const pair_t& kval) -> bool {
+         return kval.first != EMPTYBUCKET && kval.first != TOMBSTONE;
+      };
+      return extractKeysByPattern<prefetches>(elements, rule, stack, max_size, s);
+   }
+
+   /* ------------------------------------------------------------ accelerated API (hashinator.h:1261-1409) */
+   template <bool prefetches = true>
+   void clean_tombstones(hipStream_t s = 0) {
+      if (_info.tombstoneCounter == 0) return;
+      const int sp = _info.sizePower;
+      const size_t full =
+          W64 ? hn_clean_tombstones_scratch_bytes_u64(sp, 0) : hn_clean_tombstones_scratch_bytes_u32(sp, 0);
+      // A table with no EMPTY slot is cleaned in one region; otherwise an eighth of it is enough.
+      const bool no_empty = _info.fill + _info.tombstoneCounter >= _nbuckets;
+      const size_t budget = no_empty ? 0 : std::max<size_t>(full / 8, size_t(64) << 20);
+      const size_t sb = W64 ? hn_clean_tombstones_scratch_bytes_u64(sp, budget)
+                            : hn_clean_tombstones_scratch_bytes_u32(sp, budget);
+      void* scratch = nullptr;
+      SPLIT_CHECK_ERR(hipMallocAsync(&scratch, sb, s));
+      chk(W64 ? hn_clean_tombstones_u64(_buckets, dinfo(), sp, scratch, sb, st(s))
+              : hn_clean_tombstones_u32(_buckets, dinfo(), sp, scratch, sb, st(s)));
+      SPLIT_CHECK_ERR(hipFreeAsync(scratch, s));
+      changed(s);
+   }
+
+   template <bool prefetches = true>
+   void insert(KEY_TYPE* keys, VAL_TYPE* vals, size_t len, float targetLF = 0.5, hipStream_t s = 0) {
+      if (len == 0) {
+         set_status(status::success);
+         return;
+      }
+      const int64_t neededPowerSize = std::ceil(std::log2((_info.fill + len) * (1.0 / targetLF)));
+      if (neededPowerSize > _info.sizePower) resize(static_cast<int>(neededPowerSize), targets::device, s);
+      DeviceHasher::insert(keys, vals, _buckets, _dinfo, len, s, tune());
+      changed(s);
+   }
+   template <bool prefetches = true>
+   void insertIndex(KEY_TYPE* keys, size_t len, float targetLF = 0.5, hipStream_t s = 0) {
+      if (len == 0) {
+         set_status(status::success);
+         return;
+      }
+      const int64_t neededPowerSize = std::ceil(std::log2((_info.fill + len) * (1.0 / targetLF)));
+      if (neededPowerSize > _info.sizePower) resize(static_cast<int>(neededPowerSize), targets::device, s);
+      DeviceHasher::insertIndex(keys, _buckets, _dinfo, len, s, tune());
+      changed(s);
+   }
+   template <bool prefetches = true>
+   void insert(pair_t* src, size_t len, float targetLF = 0.5, hipStream_t s = 0) {
+      if (len == 0) {
+         set_status(status::success);
+         return;
+      }
+      const int64_t neededPowerSize = std::ceil(std::log2(((_info.fill) + len) * (1.0 / targetLF)));
+      if (neededPowerSize > _info.sizePower) resize(static_cast<int>(neededPowerSize), targets::device, s);
+      DeviceHasher::insert(src, _buckets, _dinfo, len, s, tune());
+      changed(s);
+   }
+   /* `hit` (optional, an addition) receives 1/0 per key. */
+   template <bool prefetches = true>
+   void retrieve(KEY_TYPE* keys, VAL_TYPE* vals, size_t len, hipStream_t s = 0, uint8_t* hit = nullptr) {
+      DeviceHasher::retrieve(keys, vals, _buckets, _dinfo, len, s, tune(), hit);
+   }
+   template <bool prefetches = true>
+   void retrieve(pair_t* src, size_t len, hipStream_t s = 0, uint8_t* hit = nullptr) {
+      DeviceHasher::retrieve(src, _buckets, _dinfo, len, s, tune(), hit);
+   }
+   template <bool prefetches = true>
+   void erase(KEY_TYPE* keys, size_t len, hipStream_t s = 0) {
+      DeviceHasher::erase(keys, _buckets, _dinfo, len, s, tune());
+      changed(s);
+   }
+};
+
+} // namespace Hashinator

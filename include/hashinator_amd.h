@@ -168,6 +168,13 @@ HN_API int hn_erase_u64(const uint64_t* keys, void* buckets, hn_info* info, size
 HN_API int hn_reset_all_u32(void* buckets, hn_info* info, size_t len, hn_stream_t s);
 HN_API int hn_reset_all_u64(void* buckets, hn_info* info, size_t len, hn_stream_t s);
 
+/* Hasher::reset(src, dst, info, len, s) (hashers.h:144-152) -> reset_to_empty
+ * (kernels_AMD.h:65-134): for each src pair, the first slot holding its key
+ * (probing the whole table; EMPTY does not end the probe) becomes EMPTY;
+ * fill -= len. */
+HN_API int hn_reset_u32(const void* src_pairs, void* buckets, hn_info* info, size_t len, hn_stream_t s);
+HN_API int hn_reset_u64(const void* src_pairs, void* buckets, hn_info* info, size_t len, hn_stream_t s);
+
 /* ---------------------------------------------------------------- clean / rehash */
 
 /* Scratch needed by hn_clean_tombstones for a table of 2^size_power slots
@@ -230,6 +237,24 @@ HN_API int hn_copy_keys_if_pairs_u64(const void* in, size_t n, const hn_pred* pr
  * elem_bytes must be 4, 8 or 16. */
 HN_API int hn_copy_if_flags(const void* in, const uint8_t* flags, size_t n, size_t elem_bytes, void* out,
                      uint64_t* d_count, void* scratch, size_t scratch_bytes, hn_stream_t s);
+
+/* copy_keys_if with the rule already evaluated into flags: .first of each
+ * flagged hash_pair, in input order (the shim's path for user functors in
+ * copy_keys_if / Hashmap::extractKeysByPattern). */
+HN_API int hn_copy_keys_if_flags_u32(const void* in_pairs, const uint8_t* flags, size_t n, uint32_t* out,
+                                     uint64_t* d_count, void* scratch, size_t scratch_bytes, hn_stream_t s);
+HN_API int hn_copy_keys_if_flags_u64(const void* in_pairs, const uint8_t* flags, size_t n, uint64_t* out,
+                                     uint64_t* d_count, void* scratch, size_t scratch_bytes, hn_stream_t s);
+
+/* As hn_copy_if_flags / hn_copy_keys_if_flags with the rule packed one bit
+ * per element (bit i is bit i%32 of word i/32) -- n/8 bytes of staging
+ * instead of n. */
+HN_API int hn_copy_if_bits(const void* in, const uint32_t* bits, size_t n, size_t elem_bytes, void* out,
+                           uint64_t* d_count, void* scratch, size_t scratch_bytes, hn_stream_t s);
+HN_API int hn_copy_keys_if_bits_u32(const void* in_pairs, const uint32_t* bits, size_t n, uint32_t* out,
+                                    uint64_t* d_count, void* scratch, size_t scratch_bytes, hn_stream_t s);
+HN_API int hn_copy_keys_if_bits_u64(const void* in_pairs, const uint32_t* bits, size_t n, uint64_t* out,
+                                    uint64_t* d_count, void* scratch, size_t scratch_bytes, hn_stream_t s);
 
 /* split::tools::split_prefix_scan (split_tools.h:363-418): exclusive prefix
  * sum with wraparound in the element type.  Any n (the reference requires a

@@ -1,0 +1,258 @@
+// test_shim.cpp -- the drop-in C++ headers (include/hashinator, include/splitvector)
+// exercised the way the reference's own unit tests use them
+// (kstppd/hashinator unit_tests/hashmap_unit_test/main.cu,
+// unit_tests/stream_compaction/unit.cu), on the GPU, against host-side
+// expectations.  Prints one line per case; exits nonzero on any failure.
+#include <hashinator/hashinator.h>
+#include <splitvector/split_tools.h>
+
+#include <cstdio>
+#include <random>
+#include <unordered_map>
+#include <vector>
+
+using namespace Hashinator;
+
+static int failures = 0;
+#define CHECK(cond)                                                                                                    \
+   do {                                                                                                                \
+      if (!(cond)) {                                                                                                   \
+         printf("  check failed: %s (line %d)\n", #cond, __LINE__);                                                    \
+         ++failures;                                                                                                   \
+         return false;                                                                                                 \
+      }                                                                                                                \
+   } while (0)
+
+template <typename K>
+struct IsOdd {
+   __host__ __device__ bool operator()(hash_pair<K, K>& e) const { return (e.second & 1) != 0; }
+};
+template <typename K>
+struct KeyBelow {
+   K limit;
+   __host__ __device__ bool operator()(hash_pair<K, K>& e) const {
+      return e.first < limit && e.first != std::numeric_limits<K>::max() && e.first != std::numeric_limits<K>::max() - 1;
+   }
+};
+struct Below1000 {
+   __host__ __device__ bool operator()(uint32_t& x) const { return x < 1000; }
+};
+
+template <typename K>
+bool insert_retrieve_find(int power, float lf) {
+   using Map = Hashmap<K, K>;
+   const size_t N = size_t(1) << power;
+   const size_t n = static_cast<size_t>(lf * N);
+   split::SplitVector<K> keys(n), vals(n), out(2 * n), q(2 * n);
+   std::mt19937_64 rng(power);
+   for (size_t i = 0; i < n; ++i) {
+      keys[i] = static_cast<K>(i * 2654435761ull + 17) & (std::numeric_limits<K>::max() >> 2);
+      vals[i] = static_cast<K>(rng());
+   }
+   for (size_t i = 0; i < n; ++i) {
+      q[i] = keys[i];
+      q[n + i] = keys[i] | (K(1) << (sizeof(K) * 8 - 2)); // never inserted
+   }
+   Map hmap(power);
+   keys.optimizeGPU();
+   vals.optimizeGPU();
+   hmap.insert(keys.data(), vals.data(), n, 1.0f);
+   CHECK(hmap.peek_status() == status::success);
+   CHECK(hmap.size() == n);
+   uint8_t* hit = nullptr;
+   SPLIT_CHECK_ERR(hipMallocManaged(&hit, 2 * n));
+   for (size_t i = 0; i < 2 * n; ++i) out[i] = K(7);
+   q.optimizeGPU();
+   out.optimizeGPU();
+   hmap.retrieve(q.data(), out.data(), 2 * n, 0, hit);
+   SPLIT_CHECK_ERR(hipDeviceSynchronize());
+   for (size_t i = 0; i < n; ++i) {
+      CHECK(hit[i] == 1 && out[i] == vals[i]);
+      CHECK(hit[n + i] == 0 && out[n + i] == K(7)); // a miss leaves the value untouched
+   }
+   SPLIT_CHECK_ERR(hipFree(hit));
+   // host-side const find on the same buckets (recover_all_elements)
+   hmap.optimizeCPU();
+   for (size_t i = 0; i < n; i += 7) {
+      auto it = hmap.find(keys[i]);
+      CHECK(it != hmap.end() && it->second == vals[i]);
+      CHECK(hmap.find(q[n + i]) == hmap.end());
+   }
+   size_t iterated = 0;
+   for (auto it = hmap.begin(); it != hmap.end(); ++it) ++iterated;
+   CHECK(iterated == n);
+   return true;
+}
+
+template <typename K>
+bool erase_clean_extract(int power) {
+   using Map = Hashmap<K, K>;
+   const size_t n = (size_t(1) << power) / 2;
+   split::SplitVector<K> keys(n), vals(n);
+   for (size_t i = 0; i < n; ++i) {
+      keys[i] = static_cast<K>(i);
+      vals[i] = static_cast<K>(i);
+   }
+   Map hmap(power);
+   hmap.insert(keys.data(), vals.data(), n, 1.0f);
+   // erase the even keys (test_hashmap_* "no even-valued survivors")
+   split::SplitVector<K> even(n / 2);
+   for (size_t i = 0; i < n / 2; ++i) even[i] = static_cast<K>(2 * i);
+   hmap.erase(even.data(), n / 2);
+   CHECK(hmap.size() == n - n / 2);
+   CHECK(hmap.tombstone_count() == n / 2);
+   hmap.clean_tombstones();
+   CHECK(hmap.tombstone_count() == 0);
+   CHECK(hmap.size() == n - n / 2);
+   hmap.optimizeCPU();
+   for (size_t i = 0; i < n; ++i) {
+      const bool present = hmap.find(static_cast<K>(i)) != hmap.end();
+      CHECK(present == (i % 2 == 1));
+   }
+   // extractPattern with a user rule == host filter of the bucket array, in bucket order
+   split::SplitVector<hash_pair<K, K>> elements;
+   const size_t m = hmap.extractPattern(elements, IsOdd<K>());
+   CHECK(m == n - n / 2);
+   std::vector<hash_pair<K, K>> expect, b(hmap.bucket_count());
+   SPLIT_CHECK_ERR(hipMemcpy(b.data(), hmap.template expose_bucketdata<false>(), b.size() * sizeof(b[0]),
+                             hipMemcpyDeviceToHost)); // the table lives in device memory
+   for (size_t i = 0; i < hmap.bucket_count(); ++i)
+      if (b[i].first != hmap.get_emptybucket() && b[i].first != hmap.get_tombstone() && (b[i].second & 1))
+         expect.push_back(b[i]);
+   CHECK(expect.size() == m);
+   for (size_t i = 0; i < m; ++i) CHECK(elements[i] == expect[i]);
+   // keys by pattern and all keys (extractKeysByPattern / extractAllKeys)
+   split::SplitVector<K> ks;
+   const size_t kb = hmap.extractKeysByPattern(ks, KeyBelow<K>{K(1000)});
+   size_t eb = 0;
+   for (auto& e : expect) eb += e.first < 1000;
+   CHECK(kb == eb);
+   for (size_t i = 0; i < kb; ++i) CHECK(ks[i] < 1000 && (ks[i] & 1));
+   split::SplitVector<K> all;
+   CHECK(hmap.extractAllKeys(all) == m);
+   for (size_t i = 0; i < m; ++i) CHECK(all[i] == expect[i].first);
+   return true;
+}
+
+template <typename K>
+bool resize_and_grow() {
+   using Map = Hashmap<K, K>;
+   Map hmap; // sizePower 5
+   const size_t n = 5000;
+   split::SplitVector<K> keys(n), vals(n);
+   for (size_t i = 0; i < n; ++i) {
+      keys[i] = static_cast<K>(3 * i + 1);
+      vals[i] = static_cast<K>(i);
+   }
+   hmap.insert(keys.data(), vals.data(), n); // targetLF 0.5 grows to 2^14
+   CHECK(hmap.getSizePower() == 14);
+   CHECK(hmap.size() == n);
+   hmap.resize(16, targets::device);
+   CHECK(hmap.getSizePower() == 16 && hmap.peek_status() == status::success && hmap.size() == n);
+   hmap.resize(17, targets::host);
+   CHECK(hmap.getSizePower() >= 17 && hmap.size() == n);
+   hmap.optimizeCPU();
+   for (size_t i = 0; i < n; ++i) {
+      auto it = hmap.find(keys[i]);
+      CHECK(it != hmap.end() && it->second == vals[i]);
+   }
+   hmap.clear(targets::device);
+   CHECK(hmap.size() == 0);
+   hmap.optimizeCPU();
+   CHECK(hmap.find(keys[0]) == hmap.end());
+   // insertIndex: value = position
+   Map h2(13);
+   h2.insertIndex(keys.data(), n, 1.0f);
+   h2.optimizeCPU();
+   for (size_t i = 0; i < n; i += 13) CHECK(h2.find(keys[i])->second == static_cast<K>(i));
+   // duplicate-heavy insert (hashmap_unit_test main.cu:1064-1085)
+   Map h3(10);
+   split::SplitVector<K> dk(1 << 12), dv(1 << 12);
+   for (size_t i = 0; i < dk.size(); ++i) {
+      dk[i] = static_cast<K>(i % 1024);
+      dv[i] = static_cast<K>(i % 1024);
+   }
+   h3.insert(dk.data(), dv.data(), dk.size(), 1.0f);
+   CHECK(h3.size() == 1024);
+   return true;
+}
+
+bool compaction_and_scan() {
+   const size_t n = (1 << 20) + 333;
+   split::SplitVector<uint32_t> in(n), out(n);
+   std::mt19937 rng(3);
+   for (size_t i = 0; i < n; ++i) in[i] = rng() % 4000;
+   split::tools::copy_if(in, out, Below1000());
+   std::vector<uint32_t> expect;
+   for (size_t i = 0; i < n; ++i)
+      if (in[i] < 1000) expect.push_back(in[i]);
+   CHECK(out.size() == expect.size());
+   for (size_t i = 0; i < expect.size(); ++i) CHECK(out[i] == expect[i]);
+   // preallocated arena (unit_tests/stream_compaction/preallocated.cu)
+   split::SplitVector<uint32_t> out2(n);
+   const size_t need = split::tools::estimateMemoryForCompaction(in);
+   void* stack = nullptr;
+   SPLIT_CHECK_ERR(hipMalloc(&stack, need));
+   split::tools::copy_if(in, out2, Below1000(), stack, need);
+   SPLIT_CHECK_ERR(hipFree(stack));
+   CHECK(out2 == out);
+   // exclusive scan, any size, wrapping
+   split::SplitVector<uint32_t> flags(n), sum(n);
+   for (size_t i = 0; i < n; ++i) flags[i] = rng();
+   split::tools::split_prefix_scan(flags, sum);
+   uint32_t acc = 0;
+   for (size_t i = 0; i < n; ++i) {
+      CHECK(sum[i] == acc);
+      acc += flags[i];
+   }
+   return true;
+}
+
+bool splitvector_host_semantics() {
+   split::SplitVector<int> v{1, 2, 3};
+   v.push_back(4);
+   CHECK(v.size() == 4 && v.back() == 4 && v.front() == 1);
+   v.insert(v.begin() + 1, 2, 9);
+   CHECK(v.size() == 6 && v[1] == 9 && v[2] == 9 && v[3] == 2);
+   v.erase(v.begin(), v.begin() + 2);
+   CHECK(v.size() == 4 && v[0] == 9 && v[1] == 2);
+   v.resize(100);
+   CHECK(v.size() == 100 && v.capacity() >= 100);
+   v.resize(3);
+   CHECK(v.size() == 3);
+   split::SplitVector<int> w(v);
+   CHECK(w == v);
+   w[0] = -1;
+   CHECK(w != v);
+   v.clear();
+   CHECK(v.empty());
+   split::SplitVector<int> big(1 << 20, 5);
+   big.optimizeCPU();
+   CHECK(big[(1 << 20) - 1] == 5 && big[0] == 5);
+   return true;
+}
+
+int main() {
+   struct Case {
+      const char* name;
+      bool (*fn)();
+   } cases[] = {
+       {"insert_retrieve_find<u32> sp=16 lf=0.9", [] { return insert_retrieve_find<uint32_t>(16, 0.9f); }},
+       {"insert_retrieve_find<u64> sp=18 lf=0.9", [] { return insert_retrieve_find<uint64_t>(18, 0.9f); }},
+       {"erase_clean_extract<u32> sp=14", [] { return erase_clean_extract<uint32_t>(14); }},
+       {"erase_clean_extract<u64> sp=14", [] { return erase_clean_extract<uint64_t>(14); }},
+       {"resize_and_grow<u32>", [] { return resize_and_grow<uint32_t>(); }},
+       {"resize_and_grow<u64>", [] { return resize_and_grow<uint64_t>(); }},
+       {"compaction_and_scan", compaction_and_scan},
+       {"splitvector_host_semantics", splitvector_host_semantics},
+   };
+   for (auto& c : cases) {
+      printf("%s ...\n", c.name);
+      fflush(stdout);
+      const bool ok = c.fn();
+      printf("%s %s\n", ok ? "PASS" : "FAIL", c.name);
+      fflush(stdout);
+   }
+   printf("%d failure(s)\n", failures);
+   return failures ? 1 : 0;
+}

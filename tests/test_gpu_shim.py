@@ -1,0 +1,18 @@
+"""The drop-in C++ headers (include/hashinator, include/splitvector) on the GPU:
+runs tests/cpp/_bin/test_shim, built by __graft_entry__.build()."""
+import os
+import subprocess
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+BIN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "_bin", "test_shim")
+
+
+def test_cpp_dropin_headers(gpu):
+    assert os.path.exists(BIN), "tests/cpp/_bin/test_shim missing: run __graft_entry__.build()"
+    r = subprocess.run([BIN], capture_output=True, text=True, timeout=300)
+    print(r.stdout[-4000:], r.stderr[-2000:])
+    assert r.returncode == 0, r.stdout[-4000:]
+    assert "0 failure(s)" in r.stdout
