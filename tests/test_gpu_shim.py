@@ -16,3 +16,12 @@ def test_cpp_dropin_headers(gpu):
     print(r.stdout[-4000:], r.stderr[-2000:])
     assert r.returncode == 0, r.stdout[-4000:]
     assert "0 failure(s)" in r.stdout
+
+
+def test_bucket_residency_results_agree(gpu):
+    """Device-resident and unified-memory bucket arrays give the same results
+    (the timings it prints are the reason the table lives in device memory)."""
+    exe = os.path.join(os.path.dirname(BIN), "residency")
+    r = subprocess.run([exe, "18"], capture_output=True, text=True, timeout=300)
+    print(r.stdout)
+    assert r.returncode == 0 and "residency ok" in r.stdout, r.stdout + r.stderr
