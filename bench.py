@@ -111,6 +111,7 @@ def main():
     ap.add_argument("--probe-width", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extra", action="store_true", help="skip the erase/clean and load-factor extras")
+    ap.add_argument("--no-batched", action="store_true", help="skip the many-maps (batched vs per-map) comparison")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one plain 1 GiB device copy (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -244,6 +245,11 @@ def main():
     if not args.no_extra:
         extra = run_extras(L, torch, dev, S, P, tune, buckets, info, dk, dv, q, out, hit, n, N, sp, make_inputs)
 
+    if not args.no_batched:
+        del buckets, out, hit, q, dk, dv
+        torch.cuda.empty_cache()
+        extra["batched"] = run_batched(torch)
+
     line = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -273,6 +279,81 @@ def main():
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)
+
+
+def run_batched(torch):
+    """Many uneven maps: 4096 Hashmap<u32,u32> of 2^8..2^16 slots at LF 0.6
+    (velocity-grid keys): insert -> retrieve (keys + as many absent cells)
+    -> erase 20% -> clean_tombstones, batched in one persistent launch (eager
+    and replayed from a HIP graph) against the per-map calls."""
+    from hashinator_amd import _lib
+    from hashinator_amd import workloads as Wl
+    from hashinator_amd.autotune import tune_batch
+    from hashinator_amd.batch import BatchedMaps
+
+    t0 = time.perf_counter()
+    wl = Wl.batched_workload()
+    bm = BatchedMaps(*wl, key_bits=32)
+    gen_s = time.perf_counter() - t0
+    n_ins, n_qry, n_ers = bm.counts()
+    ops = n_ins + n_qry + n_ers
+    stream = torch.cuda.current_stream()
+
+    def timed(fn, reps=3):
+        best = float("inf")
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            fn()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            best = min(best, e0.elapsed_time(e1))
+        return best
+
+    tune, table = tune_batch(bm, candidates=[(w, b) for w in (4, 8, 16) for b in (256, 512, 1024)])
+
+    def per_map():
+        bm.init_per_map()
+        bm.run_per_map()
+
+    t_per = timed(per_map, reps=2)
+    ok_per = _batch_ok(bm, n_ins, n_ers)
+
+    def batched():
+        bm.init()
+        bm.run(tune=tune)
+
+    t_bat = timed(batched)
+    ok_bat = _batch_ok(bm, n_ins, n_ers)
+    g = torch.cuda.CUDAGraph()
+    side = torch.cuda.Stream()
+    side.wait_stream(stream)
+    with torch.cuda.stream(side):
+        batched()
+    stream.wait_stream(side)
+    torch.cuda.synchronize()
+    with torch.cuda.graph(g):
+        batched()
+    t_graph = timed(g.replay)
+    ok_graph = _batch_ok(bm, n_ins, n_ers)
+    return {
+        "workload": f"{bm.n} Hashmap<u32,u32>, sizes 2^8..2^16 ({bm.total_slots} slots), LF 0.6, velocity-grid keys: "
+                    f"insert {n_ins} + retrieve {n_qry} + erase {n_ers} + clean",
+        "per_map_ms": round(t_per, 3), "batched_ms": round(t_bat, 3), "batched_graph_ms": round(t_graph, 3),
+        "per_map_Mops": round(ops / t_per / 1e3, 1), "batched_Mops": round(ops / t_bat / 1e3, 1),
+        "batched_graph_Mops": round(ops / t_graph / 1e3, 1),
+        "speedup_vs_per_map": round(t_per / min(t_bat, t_graph), 2),
+        "tune": {"probe_width": tune.probe_width, "block_size": tune.block_size},
+        "tune_table_ms": {k: round(v, 3) for k, v in table.items()},
+        "ok": bool(ok_per and ok_bat and ok_graph), "input_gen_s": round(gen_s, 1),
+    }
+
+
+def _batch_ok(bm, n_ins, n_ers):
+    inf = bm.all_infos()
+    hits = int(bm.qry_hit.sum().item())
+    return (hits == n_ins and int(inf[:, 1].sum()) == n_ins - n_ers and int(inf[:, 3].sum()) == 0
+            and not np.any(inf[:, 4] & 0xFFFFFFFF))
 
 
 def run_extras(L, torch, dev, S, P, tune, buckets, info, dk, dv, q, out, hit, n, N, sp, make_inputs):

@@ -40,6 +40,25 @@ class hn_tune(C.Structure):
     ]
 
 
+class hn_batch_map(C.Structure):
+    _fields_ = [
+        ("buckets", C.c_void_p),
+        ("info", C.c_void_p),
+        ("ins_off", C.c_uint64),
+        ("ins_len", C.c_uint64),
+        ("qry_off", C.c_uint64),
+        ("qry_len", C.c_uint64),
+        ("ers_off", C.c_uint64),
+        ("ers_len", C.c_uint64),
+        ("scratch_off", C.c_uint64),
+        ("scratch_cap", C.c_uint64),
+    ]
+
+
+HN_BATCH_INSERT, HN_BATCH_RETRIEVE, HN_BATCH_ERASE, HN_BATCH_CLEAN = 1, 2, 4, 8
+HN_BATCH_ALL = 15
+
+
 class hn_pred(C.Structure):
     _fields_ = [
         ("kind", C.c_int32),
@@ -99,6 +118,11 @@ SIGNATURES = {
     "hn_copy_keys_if_flags_u64": (I, [P, P, S, P, P, P, S, P]),
     "hn_exclusive_scan_u32": (I, [P, P, S, P, S, P]),
     "hn_exclusive_scan_u64": (I, [P, P, S, P, S, P]),
+    "hn_batch_scratch_bytes": (S, []),
+    "hn_batch_init_u32": (I, [P, C.c_uint32, P, P]),
+    "hn_batch_init_u64": (I, [P, C.c_uint32, P, P]),
+    "hn_batch_run_u32": (I, [P, C.c_uint32, C.c_uint32, P, P, P, P, P, P, P, P, P, TUNE]),
+    "hn_batch_run_u64": (I, [P, C.c_uint32, C.c_uint32, P, P, P, P, P, P, P, P, P, TUNE]),
     "hn_version": (C.c_char_p, []),
     # include/hashinator_amd_calib.h
     "hn_calib_random_gather": (I, [P, S, I, S, I, C.c_uint64, P, P]),

@@ -1,0 +1,295 @@
+// hn_batch.hip -- many independent maps in one persistent launch.
+//
+// No counterpart kernel in the reference (its API is one launch per map per
+// operation, hashers.h:46-163); the per-map semantics are the reference's:
+// for each map, in order, insert (insert_kernel, kernels_AMD.h:272-405),
+// retrieve (retrieve_kernel 647-697), erase (delete_kernel 412-505), then
+// clean_tombstones (hashinator.h:1261-1315).
+//
+// One workgroup owns one map at a time and runs the requested phases on it
+// back to back, separated by workgroup barriers with agent-scope fences;
+// maps are handed out by an atomic ticket in the order the caller lists them
+// (largest first balances the grid).  The probe loops are the single-map
+// ones (hn_probe.hpp), so a map's results are those of the plain calls.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "hn_device.hpp"
+#include "hn_host.hpp"
+#include "hn_probe.hpp"
+
+namespace hn {
+
+template <typename K>
+struct BatchArgs {
+   const hn_batch_map* maps;
+   uint32_t n_maps;
+   uint32_t ops;
+   const K* ins_keys;
+   const K* ins_vals;
+   const K* qry_keys;
+   K* qry_vals;
+   uint8_t* qry_hit;
+   const K* ers_keys;
+   Slot<K>* clean_scratch;
+   uint32_t* ticket;
+};
+
+__device__ __forceinline__ void phase_barrier() {
+   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+   __syncthreads();
+   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+template <typename K, int G>
+__global__ __launch_bounds__(1024) void k_batch(BatchArgs<K> a) {
+   __shared__ uint32_t s_map;
+   __shared__ unsigned long long s_placed, s_max, s_erased, s_lifted;
+   __shared__ int s_failed;
+   const uint64_t groups = blockDim.x / G;
+   const uint64_t g = threadIdx.x / G;
+   const int lane = __lane_id();
+
+   auto reset_shared = [&]() {
+      s_placed = 0;
+      s_max = 0;
+      s_erased = 0;
+      s_lifted = 0;
+      s_failed = 0;
+   };
+   auto fold_insert = [&](const InsertCounters& c) {
+      const uint64_t p = wave_sum(c.placed);
+      const uint64_t m = wave_max(c.max_probe);
+      const int f = wave_max(c.failed);
+      if (lane == 0) {
+         if (p) atomicAdd(&s_placed, static_cast<unsigned long long>(p));
+         if (m) atomicMax(&s_max, static_cast<unsigned long long>(m));
+         if (f) atomicOr(&s_failed, 1);
+      }
+   };
+
+   for (;;) {
+      if (threadIdx.x == 0) {
+         s_map = __hip_atomic_fetch_add(a.ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+         reset_shared();
+      }
+      __syncthreads();
+      const uint32_t mi = s_map;
+      if (mi >= a.n_maps) break;
+      const hn_batch_map M = a.maps[mi];
+      hn_info* info = M.info;
+      Slot<K>* b = static_cast<Slot<K>*>(M.buckets);
+      const int sp = info->size_power;
+      const bool sp_ok = sp >= 1 && sp <= max_size_power<K>();
+      const uint64_t nsl = sp_ok ? (1ull << sp) : 0;
+
+      if (a.ops & HN_BATCH_INSERT) {
+         InsertCounters c;
+         if (sp_ok) {
+            InsertArgs<K> ia{a.ins_keys + M.ins_off, a.ins_vals + M.ins_off, nullptr, b, info, M.ins_len,
+                             nullptr, nullptr, nullptr};
+            insert_range<K, G, SRC_SOA>(ia, sp, M.ins_len, g, groups, c);
+         }
+         fold_insert(c);
+         __syncthreads();
+         if (threadIdx.x == 0) {
+            info->fill += s_placed;
+            if (s_max) {
+               const uint64_t o = next_overflow(s_max, kBucketOverflow);
+               if (o > info->current_max_bucket_overflow) info->current_max_bucket_overflow = o;
+            }
+            info->err = (s_failed || !sp_ok) ? HN_STATUS_FAIL : HN_STATUS_SUCCESS;
+            s_placed = 0;
+            s_max = 0;
+            s_failed = 0;
+         }
+         phase_barrier();
+      }
+      if ((a.ops & HN_BATCH_RETRIEVE) && sp_ok) {
+         LookupArgs<K> la{a.qry_keys + M.qry_off, a.qry_vals + M.qry_off, nullptr,
+                          a.qry_hit ? a.qry_hit + M.qry_off : nullptr, b, info, M.qry_len};
+         lookup_range<K, G, false, LK_RETRIEVE>(la, sp, info->current_max_bucket_overflow, M.qry_len, g, groups);
+         phase_barrier();
+      }
+      if ((a.ops & HN_BATCH_ERASE) && sp_ok) {
+         LookupArgs<K> la{a.ers_keys + M.ers_off, nullptr, nullptr, nullptr, b, info, M.ers_len};
+         const uint64_t e =
+             wave_sum(lookup_range<K, G, false, LK_ERASE>(la, sp, info->current_max_bucket_overflow, M.ers_len, g, groups));
+         if (lane == 0 && e) atomicAdd(&s_erased, static_cast<unsigned long long>(e));
+         __syncthreads();
+         if (threadIdx.x == 0) {
+            info->tombstone_counter += s_erased;
+            info->fill -= s_erased;
+         }
+         phase_barrier();
+      }
+      if ((a.ops & HN_BATCH_CLEAN) && sp_ok && info->tombstone_counter > 0) {
+         if (info->fill > M.scratch_cap) {
+            // not enough scratch to lift the displaced elements out: leave the map as it is
+            if (threadIdx.x == 0) info->err = HN_STATUS_FAIL;
+         } else {
+            // TOMBSTONE -> EMPTY; live elements away from home are lifted into scratch (one region: the map)
+            Slot<K>* out = a.clean_scratch + M.scratch_off;
+            const uint64_t mask = nsl - 1;
+            for (uint64_t base = 0; base < nsl; base += blockDim.x) {
+               const uint64_t i = base + threadIdx.x;
+               bool take = false;
+               Slot<K> s{};
+               if (i < nsl) {
+                  s = b[i];
+                  if (s.key == tomb_key<K>()) {
+                     Slot<K> e = s;
+                     e.key = empty_key<K>();
+                     b[i] = e;
+                  } else if (s.key != empty_key<K>() && (fib_home(s.key, sp) & mask) != i) {
+                     take = true;
+                  }
+               }
+               const uint64_t m = __ballot(take);
+               if (m) {
+                  unsigned long long wb = 0;
+                  const int leader = first_bit(m);
+                  if (lane == leader) wb = atomicAdd(&s_lifted, static_cast<unsigned long long>(__popcll(m)));
+                  wb = __shfl(wb, leader, kWave);
+                  if (take) {
+                     out[wb + __popcll(m & ((1ull << lane) - 1ull))] = s;
+                     Slot<K> e = s;
+                     e.key = empty_key<K>();
+                     b[i] = e;
+                  }
+               }
+            }
+            phase_barrier();
+            const uint64_t lifted = s_lifted;
+            if (threadIdx.x == 0) {
+               info->tombstone_counter = 0;
+               info->current_max_bucket_overflow = kBucketOverflow;
+               info->fill -= lifted;
+            }
+            phase_barrier();
+            InsertCounters c;
+            if (lifted) {
+               InsertArgs<K> ia{nullptr, nullptr, out, b, info, lifted, nullptr, nullptr, nullptr};
+               insert_range<K, G, SRC_PAIRS>(ia, sp, lifted, g, groups, c);
+            }
+            fold_insert(c);
+            __syncthreads();
+            if (threadIdx.x == 0) {
+               info->fill += s_placed;
+               if (s_max) {
+                  const uint64_t o = next_overflow(s_max, kBucketOverflow);
+                  if (o > info->current_max_bucket_overflow) info->current_max_bucket_overflow = o;
+               }
+               if (lifted) info->err = s_failed ? HN_STATUS_FAIL : HN_STATUS_SUCCESS;
+            }
+         }
+         phase_barrier();
+      }
+      __syncthreads(); // s_map is rewritten by thread 0 next
+   }
+}
+
+__global__ void k_zero_u32(uint32_t* p) { *p = 0; }
+
+// Fresh tables and Info(sp) for every map of a batch, one launch.
+template <typename K>
+__global__ __launch_bounds__(256) void k_batch_init(const hn_batch_map* maps, uint32_t n_maps, const int32_t* sps) {
+   for (uint32_t m = blockIdx.x; m < n_maps; m += gridDim.x) {
+      const hn_batch_map M = maps[m];
+      const int sp = sps[m];
+      Slot<K>* b = static_cast<Slot<K>*>(M.buckets);
+      const uint64_t n = 1ull << sp;
+      for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) {
+         Slot<K> s;
+         s.key = empty_key<K>();
+         s.val = 0;
+         b[i] = s;
+      }
+      if (threadIdx.x == 0) {
+         M.info->size_power = sp;
+         M.info->reserved0 = 0;
+         M.info->fill = 0;
+         M.info->current_max_bucket_overflow = kBucketOverflow;
+         M.info->tombstone_counter = 0;
+         M.info->err = HN_STATUS_INVALID;
+         M.info->reserved1 = 0;
+      }
+   }
+}
+
+template <typename K>
+using BatchKernel = void (*)(BatchArgs<K>);
+
+template <typename K>
+BatchKernel<K> pick_batch(int G) {
+   switch (G) {
+   case 1: return k_batch<K, 1>;
+   case 2: return k_batch<K, 2>;
+   case 4: return k_batch<K, 4>;
+   case 8: return k_batch<K, 8>;
+   case 16: return k_batch<K, 16>;
+   case 32: return k_batch<K, 32>;
+   case 64: return k_batch<K, 64>;
+   default: return nullptr;
+   }
+}
+
+template <typename K>
+int batch_run(const hn_batch_map* maps, uint32_t n_maps, uint32_t ops, const K* ik, const K* iv, const K* qk, K* qv,
+              uint8_t* qh, const K* ek, void* clean_scratch, void* scratch, hipStream_t s, const hn_tune* t) {
+   if (!maps || !scratch) return HN_EINVAL;
+   if ((ops & HN_BATCH_INSERT) && (!ik || !iv)) return HN_EINVAL;
+   if ((ops & HN_BATCH_RETRIEVE) && (!qk || !qv)) return HN_EINVAL;
+   if ((ops & HN_BATCH_ERASE) && !ek) return HN_EINVAL;
+   if ((ops & HN_BATCH_CLEAN) && !clean_scratch) return HN_EINVAL;
+   if (n_maps == 0) return 0;
+   const int G = (t && t->probe_width) ? t->probe_width : default_probe_width<K>(OP_INSERT);
+   const int bs = (t && t->block_size) ? t->block_size : 512;
+   if (G < 1 || G > 64 || (G & (G - 1)) || bs < 64 || bs > 1024 || (bs % 64)) return HN_EINVAL;
+   uint32_t blocks = (t && t->grid_blocks > 0) ? static_cast<uint32_t>(t->grid_blocks)
+                                               : static_cast<uint32_t>(cu_count()) * (2048 / bs);
+   if (blocks > n_maps) blocks = n_maps;
+   BatchKernel<K> k = pick_batch<K>(G);
+   uint32_t* ticket = static_cast<uint32_t*>(scratch);
+   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, ticket);
+   BatchArgs<K> a{maps, n_maps, ops, ik, iv, qk, qv, qh, ek, static_cast<Slot<K>*>(clean_scratch), ticket};
+   hipLaunchKernelGGL(k, dim3(blocks), dim3(bs), 0, s, a);
+   return static_cast<int>(hipGetLastError());
+}
+
+template <typename K>
+int batch_init(const hn_batch_map* maps, uint32_t n_maps, const int32_t* sps, hipStream_t s) {
+   if (!maps || !sps) return HN_EINVAL;
+   if (n_maps == 0) return 0;
+   const uint32_t blocks = n_maps < 65535u ? n_maps : 65535u;
+   hipLaunchKernelGGL(k_batch_init<K>, dim3(blocks), dim3(256), 0, s, maps, n_maps, sps);
+   return static_cast<int>(hipGetLastError());
+}
+
+} // namespace hn
+
+using namespace hn;
+
+extern "C" {
+
+size_t hn_batch_scratch_bytes(void) { return 256; }
+
+int hn_batch_init_u32(const hn_batch_map* maps, uint32_t n, const int32_t* sps, hn_stream_t s) {
+   return batch_init<uint32_t>(maps, n, sps, s);
+}
+int hn_batch_init_u64(const hn_batch_map* maps, uint32_t n, const int32_t* sps, hn_stream_t s) {
+   return batch_init<uint64_t>(maps, n, sps, s);
+}
+int hn_batch_run_u32(const hn_batch_map* maps, uint32_t n, uint32_t ops, const uint32_t* ik, const uint32_t* iv,
+                     const uint32_t* qk, uint32_t* qv, uint8_t* qh, const uint32_t* ek, void* cs, void* sc,
+                     hn_stream_t s, const hn_tune* t) {
+   return batch_run<uint32_t>(maps, n, ops, ik, iv, qk, qv, qh, ek, cs, sc, s, t);
+}
+int hn_batch_run_u64(const hn_batch_map* maps, uint32_t n, uint32_t ops, const uint64_t* ik, const uint64_t* iv,
+                     const uint64_t* qk, uint64_t* qv, uint8_t* qh, const uint64_t* ek, void* cs, void* sc,
+                     hn_stream_t s, const hn_tune* t) {
+   return batch_run<uint64_t>(maps, n, ops, ik, iv, qk, qv, qh, ek, cs, sc, s, t);
+}
+
+} // extern "C"

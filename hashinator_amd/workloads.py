@@ -138,3 +138,35 @@ def velocity_grid_keys(n, center, sigma, rng, grid=200):
         k = (d[:, 0] + grid * d[:, 1] + grid * grid * d[:, 2]).astype(np.uint32)
         out = np.unique(np.concatenate([out, k]))
     return rng.permutation(out)[:n]
+
+
+def batched_workload(n_maps=4096, sp_lo=8, sp_hi=16, lf=0.6, grid=200, sigma=20.0, erase_frac=0.2,
+                     seed=SEED_BASE + 4):
+    """Many uneven uint32 maps (the batched workload): size powers uniform in
+    [sp_lo, sp_hi] (log-uniform slot counts), each filled to `lf` with
+    velocity-grid cell ids drawn around a per-map random centre
+    (Maxwellian-like shell, `sigma` cells); queries are the keys plus as many
+    absent in-grid cells, shuffled; `erase_frac` of the keys are erased.
+    Returns (sps, ins_keys, ins_vals, qry_keys, ers_keys) as lists of arrays."""
+    rng = np.random.default_rng(int(seed) & 0xFFFFFFFF)
+    sps = rng.integers(sp_lo, sp_hi + 1, size=n_maps)
+    ins, vals, qry, ers = [], [], [], []
+    cells = grid ** 3
+    for m in range(n_maps):
+        n = int(lf * (1 << int(sps[m])))
+        center = rng.integers(0, grid, size=3)
+        k = velocity_grid_keys(n, center, sigma, rng, grid)
+        v = values(k, int(seed) + m)
+        a = np.empty(0, dtype=np.uint32)
+        while a.size < n:
+            cand = rng.integers(0, cells, size=n - a.size + n // 4 + 16).astype(np.uint32)
+            cand = cand[~np.isin(cand, k)]
+            a = np.unique(np.concatenate([a, cand]))
+        a = rng.permutation(a)[:n]
+        q = rng.permutation(np.concatenate([k, a]))
+        e = rng.choice(k, int(erase_frac * n), replace=False) if n else np.zeros(0, np.uint32)
+        ins.append(k)
+        vals.append(v.astype(np.uint32))
+        qry.append(q)
+        ers.append(e.astype(np.uint32))
+    return sps, ins, vals, qry, ers

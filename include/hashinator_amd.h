@@ -264,6 +264,50 @@ HN_API int hn_exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t n, vo
 HN_API int hn_exclusive_scan_u64(const uint64_t* in, uint64_t* out, size_t n, void* scratch, size_t scratch_bytes,
                           hn_stream_t s);
 
+/* ---------------------------------------------------------------- many maps, one launch */
+
+/* No counterpart in the reference (its API is one launch per map per
+ * operation).  A batch is an array of independent maps, each with its own
+ * bucket array and Info, and slices of shared input/output arrays.  For every
+ * map the requested phases run in the reference's order -- insert
+ * (hashers.h:46-65), retrieve (111-119), erase (132-142), clean_tombstones
+ * (hashinator.h:1261-1315) -- with the single-map semantics above; one
+ * workgroup owns one map at a time, maps are taken in array order (list the
+ * largest first), all in one persistent launch (graph-capturable: no host
+ * synchronisation, no allocation, no memset). */
+typedef struct hn_batch_map {
+   void* buckets;        /* 2^size_power slots; size_power is read from *info */
+   hn_info* info;
+   uint64_t ins_off, ins_len;       /* slice of ins_keys / ins_vals */
+   uint64_t qry_off, qry_len;       /* slice of qry_keys / qry_vals / qry_hit */
+   uint64_t ers_off, ers_len;       /* slice of ers_keys */
+   uint64_t scratch_off, scratch_cap; /* slice of clean_scratch, in slots; clean needs cap >= fill */
+} hn_batch_map;
+
+enum { HN_BATCH_INSERT = 1, HN_BATCH_RETRIEVE = 2, HN_BATCH_ERASE = 4, HN_BATCH_CLEAN = 8 };
+
+/* Device scratch of hn_batch_run (the map ticket). */
+HN_API size_t hn_batch_scratch_bytes(void);
+
+/* Fresh tables ({EMPTYBUCKET, 0}) and Info(sps[m]) for all maps, one launch.
+ * `maps` and `sps` are device arrays. */
+HN_API int hn_batch_init_u32(const hn_batch_map* maps, uint32_t n_maps, const int32_t* sps, hn_stream_t s);
+HN_API int hn_batch_init_u64(const hn_batch_map* maps, uint32_t n_maps, const int32_t* sps, hn_stream_t s);
+
+/* Run the phases in `ops` (HN_BATCH_* bits) on every map.  `maps` is a device
+ * array.  Unused input pointers may be NULL; qry_hit is optional.  A map
+ * whose clean would need more than scratch_cap slots gets err = FAIL and is
+ * left as it is.  tune: probe_width, block_size (threads per map), grid_blocks
+ * (resident workgroups); NULL = defaults. */
+HN_API int hn_batch_run_u32(const hn_batch_map* maps, uint32_t n_maps, uint32_t ops, const uint32_t* ins_keys,
+                            const uint32_t* ins_vals, const uint32_t* qry_keys, uint32_t* qry_vals, uint8_t* qry_hit,
+                            const uint32_t* ers_keys, void* clean_scratch, void* scratch, hn_stream_t s,
+                            const hn_tune* t);
+HN_API int hn_batch_run_u64(const hn_batch_map* maps, uint32_t n_maps, uint32_t ops, const uint64_t* ins_keys,
+                            const uint64_t* ins_vals, const uint64_t* qry_keys, uint64_t* qry_vals, uint8_t* qry_hit,
+                            const uint64_t* ers_keys, void* clean_scratch, void* scratch, hn_stream_t s,
+                            const hn_tune* t);
+
 /* ---------------------------------------------------------------- misc */
 
 /* Version string and the gfx target the library was built for. */
