@@ -112,6 +112,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extra", action="store_true", help="skip the erase/clean and load-factor extras")
     ap.add_argument("--no-batched", action="store_true", help="skip the many-maps (batched vs per-map) comparison")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop measurement")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one plain 1 GiB device copy (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -249,6 +250,9 @@ def main():
         del buckets, out, hit, q, dk, dv
         torch.cuda.empty_cache()
         extra["batched"] = run_batched(torch)
+    if not args.no_serving:
+        torch.cuda.empty_cache()
+        extra["serving"] = run_serving(torch)
 
     line = {
         "metric": METRIC,
@@ -347,6 +351,42 @@ def run_batched(torch):
         "tune_table_ms": {k: round(v, 3) for k, v in table.items()},
         "ok": bool(ok_per and ok_bat and ok_graph), "input_gen_s": round(gen_s, 1),
     }
+
+
+def run_serving(torch, rates=(1000.0, 4000.0), n_requests=300):
+    """The serving loop: 64 resident Hashmap<u64,u64> of 2^20..2^26 slots
+    preloaded to LF 0.4; a Poisson stream of insert/retrieve/erase batches
+    (45/45/10%, log-uniform 1k..4M keys, 30% repeats of live keys) from host
+    memory; growth at LF 0.85; admission by a 64 GiB in-flight budget."""
+    from hashinator_amd import workloads as Wl
+    from hashinator_amd.serving import ServingEngine, latency_report, make_requests
+
+    rng = np.random.default_rng(5)
+    sps = [int(x) for x in rng.integers(20, 27, size=64)]
+    out = {"maps": "64 Hashmap<u64,u64>, 2^20..2^26 slots, preloaded to LF 0.4", "runs": []}
+    t0 = time.perf_counter()
+    pre = []
+    for m, sp in enumerate(sps):
+        k = Wl.unique_keys(int(0.4 * (1 << sp)), 64, 7000 + m)
+        pre.append(k)
+    gen = time.perf_counter() - t0
+    for rate in rates:
+        eng = ServingEngine(sps, key_bits=64, n_streams=8, hbm_budget_bytes=64 << 30)
+        for m, k in enumerate(pre):
+            eng.preload(m, k, Wl.values(k, 1))
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        reqs = make_requests(sps, n_requests, rate_hz=rate, seed=int(rate), live_keys=[[k] for k in pre])
+        gen += time.perf_counter() - t1
+        wall = eng.serve(reqs, realtime=True)
+        rep = latency_report(reqs, wall)
+        rep.update({"rate_hz": rate, "rehashes": eng.rehashes, "launches": eng.launches,
+                    "merged_requests": n_requests - eng.launches})
+        out["runs"].append(rep)
+        del eng
+        torch.cuda.empty_cache()
+    out["input_gen_s"] = round(gen, 1)
+    return out
 
 
 def _batch_ok(bm, n_ins, n_ers):

@@ -1,0 +1,320 @@
+"""A continuous-batching request loop over maps resident in HBM.
+
+No counterpart in the reference; each request is one of the reference's
+accelerated calls (insert / retrieve / erase, hashinator.h:1318-1409) on one
+map, with the reference's auto-grow (a resize before an insert that would
+cross the load-factor ceiling, 1329-1332 -- here at 0.85).
+
+* Requests arrive over time, each for one map.  Per map they run in arrival
+  order (a map has at most one request on the device), so every result is
+  the one the same request gives when the stream is replayed alone.
+* Requests for different maps run concurrently on a pool of HIP streams; a
+  request is admitted when a stream is free and its device bytes (inputs,
+  outputs and, for a growing insert, the new table) fit in the HBM budget
+  next to the requests already in flight.
+* Queued retrieves (or erases) for the same map are merged into one launch
+  when they are next in line together -- both are order-insensitive among
+  themselves -- which is the batching; inserts are not merged (a key repeated
+  across two inserts must keep the later value).
+
+Latency is completion minus arrival; the loop reports p50/p99 and Mops/s.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+from collections import deque
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check
+
+INSERT, RETRIEVE, ERASE = "insert", "retrieve", "erase"
+_TDT = {32: torch.int32, 64: torch.int64}
+_NDT = {32: np.uint32, 64: np.uint64}
+
+
+@dataclass
+class Request:
+    rid: int
+    map_id: int
+    kind: str
+    keys: np.ndarray
+    vals: np.ndarray | None = None
+    arrival: float = 0.0  # seconds from the start of the run
+    # filled in by the engine
+    out_vals: np.ndarray | None = None
+    out_hit: np.ndarray | None = None
+    t_admit: float = -1.0
+    t_done: float = -1.0
+
+    @property
+    def latency(self):
+        return self.t_done - self.arrival
+
+
+@dataclass
+class _Map:
+    sp: int
+    buckets: torch.Tensor
+    info: torch.Tensor
+    fill_upper: int  # an upper bound of the fill, known without reading the device
+    info_host: torch.Tensor | None = None  # pinned copy of the Info, written after each request
+    busy: bool = False
+    queue: deque = field(default_factory=deque)
+
+
+class ServingEngine:
+    def __init__(self, sps, key_bits=64, n_streams=8, hbm_budget_bytes=32 << 30, grow_lf=0.85, device="cuda",
+                 tune=None):
+        self.kb = key_bits
+        self.sfx = "u32" if key_bits == 32 else "u64"
+        self.slot = 8 if key_bits == 32 else 16
+        self.es = 4 if key_bits == 32 else 8
+        self.L = _lib.load()
+        self.dev = torch.device(device)
+        self.grow_lf = grow_lf
+        self.budget = hbm_budget_bytes
+        self.tune = C.byref(tune) if tune is not None else None
+        self.maps = []
+        s0 = torch.cuda.current_stream()
+        for sp in sps:
+            b = torch.empty((1 << int(sp), 2), dtype=_TDT[key_bits], device=self.dev)
+            info = torch.zeros(5, dtype=torch.int64, device=self.dev)
+            check(getattr(self.L, f"hn_table_init_{self.sfx}")(b.data_ptr(), 1 << int(sp), s0.cuda_stream), "init")
+            check(self.L.hn_info_init(info.data_ptr(), int(sp), s0.cuda_stream), "info")
+            self.maps.append(_Map(int(sp), b, info, 0, torch.zeros(5, dtype=torch.int64, pin_memory=True)))
+        torch.cuda.synchronize()
+        self.streams = [torch.cuda.Stream(device=self.dev) for _ in range(n_streams)]
+        self.free = list(range(n_streams))
+        self.inflight = []  # (event, stream index, map id, [requests], bytes, keepalive tensors, t_admit)
+        self.bytes_inflight = 0
+        self.rehashes = 0
+        self.launches = 0
+        self.completed = 0
+
+    # ------------------------------------------------------------ preload (not timed)
+    def preload(self, map_id, keys, vals):
+        m = self.maps[map_id]
+        s = torch.cuda.current_stream()
+        dk = torch.from_numpy(np.ascontiguousarray(keys).view(np.int64 if self.kb == 64 else np.int32)).to(self.dev)
+        dv = torch.from_numpy(np.ascontiguousarray(vals).view(np.int64 if self.kb == 64 else np.int32)).to(self.dev)
+        check(getattr(self.L, f"hn_insert_{self.sfx}")(dk.data_ptr(), dv.data_ptr(), m.buckets.data_ptr(),
+                                                        m.info.data_ptr(), dk.numel(), s.cuda_stream, self.tune),
+              "preload")
+        m.fill_upper += dk.numel()
+
+    # ------------------------------------------------------------ loop
+    def _request_bytes(self, m: _Map, reqs):
+        n = sum(r.keys.size for r in reqs)
+        kind = reqs[0].kind
+        b = n * self.es * (2 if kind != ERASE else 1) + (n if kind == RETRIEVE else 0)
+        if kind == INSERT and self._needs_grow(m, n):
+            b += (2 << m.sp) * self.slot + _lib.load().hn_rehash_scratch_bytes()
+        return b
+
+    def _needs_grow(self, m: _Map, n):
+        return m.fill_upper + n > self.grow_lf * (1 << m.sp)
+
+    def _take(self, m: _Map):
+        """Next group of requests for a map: one insert, or a run of same-kind retrieves/erases."""
+        head = m.queue[0]
+        group = [head]
+        if head.kind != INSERT:
+            for r in list(m.queue)[1:]:
+                if r.kind != head.kind:
+                    break
+                group.append(r)
+        return group
+
+    def _launch(self, map_id, reqs, si, now, nbytes):
+        m = self.maps[map_id]
+        s = self.streams[si]
+        L, sfx = self.L, self.sfx
+        kind = reqs[0].kind
+        keep = []
+        with torch.cuda.stream(s):
+            keys = np.concatenate([r.keys for r in reqs])
+            n = keys.size
+            hk = torch.from_numpy(keys.view(np.int64 if self.kb == 64 else np.int32)).pin_memory()
+            dk = hk.to(self.dev, non_blocking=True)
+            keep += [hk, dk]
+            if kind == INSERT:
+                if self._needs_grow(m, n):
+                    new_sp = m.sp + 1
+                    while m.fill_upper + n > self.grow_lf * (1 << new_sp):
+                        new_sp += 1
+                    nb = torch.empty((1 << new_sp, 2), dtype=_TDT[self.kb], device=self.dev)
+                    sc = torch.empty(L.hn_rehash_scratch_bytes(), dtype=torch.uint8, device=self.dev)
+                    check(getattr(L, f"hn_rehash_{sfx}")(m.buckets.data_ptr(), m.sp, nb.data_ptr(), new_sp,
+                                                         m.info.data_ptr(), sc.data_ptr(), s.cuda_stream), "rehash")
+                    m.buckets.record_stream(s)
+                    keep += [m.buckets, sc]
+                    m.buckets, m.sp = nb, new_sp
+                    self.rehashes += 1
+                hv = torch.from_numpy(np.concatenate([r.vals for r in reqs]).view(
+                    np.int64 if self.kb == 64 else np.int32)).pin_memory()
+                dv = hv.to(self.dev, non_blocking=True)
+                keep += [hv, dv]
+                check(getattr(L, f"hn_insert_{sfx}")(dk.data_ptr(), dv.data_ptr(), m.buckets.data_ptr(),
+                                                      m.info.data_ptr(), n, s.cuda_stream, self.tune), "insert")
+                m.fill_upper += n
+            elif kind == RETRIEVE:
+                dv = torch.zeros(n, dtype=_TDT[self.kb], device=self.dev)
+                dh = torch.empty(n, dtype=torch.uint8, device=self.dev)
+                check(getattr(L, f"hn_retrieve_{sfx}")(dk.data_ptr(), dv.data_ptr(), dh.data_ptr(),
+                                                        m.buckets.data_ptr(), m.info.data_ptr(), n, s.cuda_stream,
+                                                        self.tune), "retrieve")
+                hv = torch.empty(n, dtype=_TDT[self.kb], pin_memory=True)
+                hh = torch.empty(n, dtype=torch.uint8, pin_memory=True)
+                hv.copy_(dv, non_blocking=True)
+                hh.copy_(dh, non_blocking=True)
+                keep += [dv, dh, hv, hh]
+            else:
+                check(getattr(L, f"hn_erase_{sfx}")(dk.data_ptr(), m.buckets.data_ptr(), m.info.data_ptr(), n,
+                                                     s.cuda_stream, self.tune), "erase")
+            m.info_host.copy_(m.info, non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record(s)
+        self.launches += 1
+        for r in reqs:
+            r.t_admit = now
+            m.queue.popleft()
+        m.busy = True
+        self.bytes_inflight += nbytes
+        self.inflight.append((ev, si, map_id, reqs, nbytes, keep))
+
+    def _complete(self, now):
+        still = []
+        for item in self.inflight:
+            ev, si, map_id, reqs, nbytes, keep = item
+            if not ev.query():
+                still.append(item)
+                continue
+            if reqs[0].kind == RETRIEVE:
+                hv, hh = keep[-2], keep[-1]
+                vals = hv.numpy().view(_NDT[self.kb])
+                hit = hh.numpy().astype(bool)
+                off = 0
+                for r in reqs:
+                    k = r.keys.size
+                    r.out_vals = vals[off:off + k].copy()
+                    r.out_hit = hit[off:off + k].copy()
+                    off += k
+            for r in reqs:
+                r.t_done = now
+            self.completed += len(reqs)
+            m = self.maps[map_id]
+            m.fill_upper = int(m.info_host[1])  # the device's fill, now that the map is idle
+            m.busy = False
+            self.free.append(si)
+            self.bytes_inflight -= nbytes
+        self.inflight = still
+
+    def serve(self, requests, realtime=True):
+        """Run `requests` (sorted by arrival) through the loop; returns wall seconds."""
+        pending = deque(sorted(requests, key=lambda r: r.arrival))
+        t0 = time.perf_counter()
+        done0 = self.completed
+        done = 0
+        total = len(pending)
+        while done < total:
+            now = time.perf_counter() - t0
+            while pending and (pending[0].arrival <= now or not realtime):
+                r = pending.popleft()
+                self.maps[r.map_id].queue.append(r)
+            self._complete(now)
+            dispatched = False
+            for map_id, m in enumerate(self.maps):
+                if m.busy or not m.queue or not self.free:
+                    continue
+                reqs = self._take(m)
+                need = self._request_bytes(m, reqs)
+                if self.inflight and self.bytes_inflight + need > self.budget:
+                    continue  # wait for memory to free up
+                self._launch(map_id, reqs, self.free.pop(), now, need)
+                dispatched = True
+            done = self.completed - done0
+            if not dispatched and self.inflight:
+                self.inflight[0][0].synchronize() if not pending else time.sleep(0)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    def serve_alone(self, requests):
+        """Every request by itself, in arrival order, one at a time (the reference's synchronous use)."""
+        t0 = time.perf_counter()
+        for r in sorted(requests, key=lambda r: r.arrival):
+            m = self.maps[r.map_id]
+            self.maps[r.map_id].queue.append(r)
+            need = self._request_bytes(m, [r])
+            self._launch(r.map_id, [r], self.free.pop(), time.perf_counter() - t0, need)
+            self.inflight[-1][0].synchronize()
+            self._complete(time.perf_counter() - t0)
+            assert not m.busy
+        return time.perf_counter() - t0
+
+    # ------------------------------------------------------------ reporting
+    def live_set(self, map_id):
+        m = self.maps[map_id]
+        img = m.buckets.cpu().numpy().view(_NDT[self.kb]).reshape(-1, 2)
+        k = img[:, 0]
+        e = np.iinfo(_NDT[self.kb]).max
+        rows = img[(k != e) & (k != e - 1)]
+        return rows[np.argsort(rows[:, 0], kind="stable")]
+
+    def infos(self):
+        return np.stack([m.info.cpu().numpy() for m in self.maps])
+
+
+def make_requests(engine_sps, n_requests, rate_hz, seed, key_bits=64, min_batch=1000, max_batch=4_000_000,
+                  mix=(0.45, 0.45, 0.10), repeat_frac=0.3, live_keys=None):
+    """A Poisson stream of requests over the maps: kind by `mix`, batch size
+    log-uniform in [min_batch, max_batch], keys uniform (unique within a
+    batch) with `repeat_frac` drawn from keys already live in the map;
+    values are a function of (key, request id)."""
+    from . import workloads as W
+
+    rng = np.random.default_rng(seed)
+    nd = _NDT[key_bits]
+    t = 0.0
+    reqs = []
+    live = [list(x) if x is not None else [] for x in (live_keys or [None] * len(engine_sps))]
+    next_idx = 10_000_000
+    for rid in range(n_requests):
+        t += rng.exponential(1.0 / rate_hz)
+        mid = int(rng.integers(0, len(engine_sps)))
+        kind = rng.choice([INSERT, RETRIEVE, ERASE], p=list(mix))
+        n = int(math.exp(rng.uniform(math.log(min_batch), math.log(max_batch))))
+        n_rep = int(repeat_frac * n) if live[mid] else 0
+        fresh = W.unique_keys(n - n_rep, key_bits, seed * 7919 + mid, start=next_idx)
+        next_idx += n + 2
+        if n_rep:
+            pool = live[mid][int(rng.integers(0, len(live[mid])))]
+            rep = pool[rng.integers(0, pool.size, size=min(n_rep, pool.size))]
+            keys = np.unique(np.concatenate([fresh, rep]).astype(nd))
+            keys = rng.permutation(keys)
+        else:
+            keys = fresh.astype(nd)
+        vals = W.values(keys, rid).astype(nd) if kind == INSERT else None
+        if kind == INSERT:
+            live[mid].append(keys)
+        reqs.append(Request(rid, mid, str(kind), keys, vals, t))
+    return reqs
+
+
+def latency_report(requests, wall_s):
+    lat = np.array([r.latency for r in requests])
+    ops = sum(r.keys.size for r in requests)
+    return {
+        "requests": len(requests),
+        "ops": int(ops),
+        "wall_s": round(wall_s, 4),
+        "Mops": round(ops / wall_s / 1e6, 1),
+        "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+        "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+        "max_ms": round(float(lat.max()) * 1e3, 3),
+    }

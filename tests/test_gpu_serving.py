@@ -1,0 +1,80 @@
+"""The serving loop: every request's result is the same in the concurrent mix
+as when the stream is replayed one request at a time, and both agree with the
+oracle's sequential replay (retrieve values and hit flags are exact; the
+final key/value sets and fills are exact)."""
+import numpy as np
+import pytest
+
+import oracle as O
+from hashinator_amd import workloads as W
+from hashinator_amd.serving import INSERT, RETRIEVE, ERASE, ServingEngine, latency_report, make_requests
+from tests.helpers import live_pairs
+
+pytestmark = pytest.mark.gpu
+
+SPS = [10, 11, 12, 13, 12, 11]
+
+
+def build(seed=1):
+    pre = []
+    for m, sp in enumerate(SPS):
+        k = W.unique_keys(int(0.4 * (1 << sp)), 64, 500 + m)
+        pre.append((k, W.values(k, 99)))
+    reqs = make_requests(SPS, 240, rate_hz=2e5, seed=seed, min_batch=20, max_batch=3000,
+                         live_keys=[[k] for k, _ in pre])
+    return pre, reqs
+
+
+def engine(pre, n_streams=8, budget=1 << 30):
+    e = ServingEngine(SPS, key_bits=64, n_streams=n_streams, hbm_budget_bytes=budget)
+    for m, (k, v) in enumerate(pre):
+        e.preload(m, k, v)
+    return e
+
+
+def clone(reqs):
+    from hashinator_amd.serving import Request
+
+    return [Request(r.rid, r.map_id, r.kind, r.keys, r.vals, r.arrival) for r in reqs]
+
+
+@pytest.mark.parametrize("realtime,budget", [(False, 1 << 30), (True, 1 << 30), (False, 200_000)])
+def test_mix_equals_alone_equals_oracle(gpu, realtime, budget):
+    pre, reqs = build(seed=3)
+    mixed = clone(reqs)
+    alone = clone(reqs)
+    e1 = engine(pre, budget=budget)
+    wall = e1.serve(mixed, realtime=realtime)
+    rep = latency_report(mixed, wall)
+    assert rep["requests"] == len(reqs) and all(r.t_done >= r.arrival for r in mixed)
+    e2 = engine(pre, n_streams=1)
+    e2.serve_alone(alone)
+    # oracle: sequential per map on a table large enough never to grow
+    models = []
+    for m, (k, v) in enumerate(pre):
+        g = O.GpuModel(18, 64)
+        g.insert(k, v)
+        models.append(g)
+    expect = {}
+    for r in sorted(reqs, key=lambda r: r.arrival):
+        g = models[r.map_id]
+        if r.kind == INSERT:
+            g.insert(r.keys, r.vals)
+        elif r.kind == ERASE:
+            g.erase(r.keys)
+        else:
+            expect[r.rid] = g.retrieve(r.keys)
+    for a, b in zip(mixed, alone):
+        assert a.rid == b.rid
+        if a.kind == RETRIEVE:
+            ev, eh = expect[a.rid]
+            assert np.array_equal(a.out_hit, eh) and np.array_equal(b.out_hit, eh)
+            assert np.array_equal(a.out_vals[eh], ev[eh]) and np.array_equal(b.out_vals[eh], ev[eh])
+    i1, i2 = e1.infos(), e2.infos()
+    for m in range(len(SPS)):
+        la, lb = e1.live_set(m), e2.live_set(m)
+        assert np.array_equal(la, lb)
+        assert np.array_equal(la, live_pairs(models[m].b, 64))
+        assert i1[m][1] == i2[m][1] == models[m].info.fill
+        assert (i1[m][4] & 0xFFFFFFFF) == 0
+    assert e1.rehashes >= 1  # the stream crosses the 0.85 ceiling somewhere
