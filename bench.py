@@ -4,9 +4,12 @@
 
 Workload (one step): a fresh Hashmap<uint64_t,uint64_t> of 2^28 slots
 (4 GiB of buckets), insert floor(0.9 * 2^28) = 241,591,910 distinct keys (half
-uniform uint64, half runs of 8..512 consecutive ids), then retrieve all of them
-plus as many absent keys (shuffled together).  Ops per step = 3 * n.  Inputs
-are synthetic, generated once, and resident in HBM before timing starts.
+uniform uint64, half runs of 8..512 consecutive ids), the Robin Hood reorder
+pass (which the library's insert policy runs after a batch this large), then
+retrieve all of them plus as many absent keys (shuffled together).  Ops per
+step = 3 * n.  Inputs are synthetic, generated once, and resident in HBM
+before timing starts.  --no-reorder times the arrival-order table instead
+(the reference's layout; every miss then probes to an EMPTY slot).
 
 Besides the headline line it reports, from the same process: per-stage
 timings, the rest of the large-map sequence (erase 25%, clean_tombstones,
@@ -42,14 +45,23 @@ def knuth_miss(a):
     return 0.5 * (1 + 1 / (1 - a) ** 2)
 
 
+def rh_miss(a):
+    """Expected slots examined by a miss in a Robin Hood ordered table: the
+    home slot, the elements hashed to it (Poisson, mean a) and the elements
+    carried past it from earlier homes (total displacement / N =
+    a * (L_hit - 1) = a^2 / (2 (1 - a)))."""
+    return 1 + a + a * a / (2 * (1 - a))
+
+
 def sectors(L, slot):
     """Expected 64 B sectors spanned by L consecutive slots at a random start."""
     return 1 + (L - 1) * slot / SECTOR
 
 
-def alg_bytes_retrieve(n_hit, n_miss, alpha, kv=16, slot=16):
+def alg_bytes_retrieve(n_hit, n_miss, alpha, kv=16, slot=16, ordered=False):
+    miss = rh_miss(alpha) if ordered else knuth_miss(alpha)
     return n_hit * (kv + SECTOR * sectors(knuth_hit(alpha), slot)) + \
-        n_miss * (kv + SECTOR * sectors(knuth_miss(alpha), slot))
+        n_miss * (kv + SECTOR * sectors(miss, slot))
 
 
 def alg_bytes_insert(n, alpha, kv=16, slot=16):
@@ -109,6 +121,8 @@ def main():
     ap.add_argument("--sp", type=int, default=28)
     ap.add_argument("--lf", type=float, default=0.9)
     ap.add_argument("--probe-width", type=int, default=0)
+    ap.add_argument("--no-reorder", action="store_true",
+                    help="skip the Robin Hood reorder pass (arrival-order table, as in the reference)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extra", action="store_true", help="skip the erase/clean and load-factor extras")
     ap.add_argument("--no-batched", action="store_true", help="skip the many-maps (batched vs per-map) comparison")
@@ -160,6 +174,9 @@ def main():
     info = torch.zeros(5, dtype=torch.int64, device=dev)
     out = torch.empty_like(q)
     hit = torch.empty(q.numel(), dtype=torch.uint8, device=dev)
+    rsb = L.hn_reorder_scratch_bytes(sp)
+    rscratch = torch.empty(rsb, dtype=torch.uint8, device=dev)
+    reorder = not args.no_reorder
     torch.cuda.synchronize()
     t_gen = time.perf_counter() - t_gen
 
@@ -167,15 +184,18 @@ def main():
         return torch.cuda.Event(enable_timing=True)
 
     def step(rec=None):
-        e = [ev() for _ in range(4)]
+        e = [ev() for _ in range(5)]
         e[0].record(stream)
         _lib.check(L.hn_table_init_u64(P(buckets), N, S), "table_init")
         _lib.check(L.hn_info_init(P(info), sp, S), "info_init")
         e[1].record(stream)
         _lib.check(L.hn_insert_u64(P(dk), P(dv), P(buckets), P(info), n, S, tune), "insert")
         e[2].record(stream)
-        _lib.check(L.hn_retrieve_u64(P(q), P(out), P(hit), P(buckets), P(info), q.numel(), S, tune), "retrieve")
+        if reorder:
+            _lib.check(L.hn_reorder_u64(P(buckets), P(info), sp, P(rscratch), rsb, S), "reorder")
         e[3].record(stream)
+        _lib.check(L.hn_retrieve_u64(P(q), P(out), P(hit), P(buckets), P(info), q.numel(), S, tune), "retrieve")
+        e[4].record(stream)
         if rec is not None:
             rec.append(e)
 
@@ -202,21 +222,25 @@ def main():
             ok = torch.equal(got[torch.argsort(qk)], dv[ik])
     inf = info.cpu().numpy()
     assert ok and int(inf[1]) == n and (int(inf[4]) & 0xFFFFFFFF) == 0, "result check failed"
+    ordered = bool((int(inf[4]) >> 32) & _lib.HN_INFO_ORDERED)
+    assert ordered == reorder, "reorder flag"
 
     t_init = np.mean([a[0].elapsed_time(a[1]) for a in recs])
     t_ins = np.mean([a[1].elapsed_time(a[2]) for a in recs])
-    t_ret = np.mean([a[2].elapsed_time(a[3]) for a in recs])
+    t_ro = np.mean([a[2].elapsed_time(a[3]) for a in recs])
+    t_ret = np.mean([a[3].elapsed_time(a[4]) for a in recs])
     ms_step = t_total * 1e3 / args.steps
     ops = 3 * n
     value = ops / (ms_step * 1e-3) / 1e6
 
-    ret_bytes = alg_bytes_retrieve(n, n, n / N)
+    ret_bytes = alg_bytes_retrieve(n, n, n / N, ordered=ordered)
     ins_bytes = alg_bytes_insert(n, n / N)
     achieved = ret_bytes / (t_ret * 1e-3) / 1e9
-    traffic = read_traffic("retrieve_u64_sp28_lf09_traffic.md")
+    traffic = read_traffic("retrieve_u64_sp28_lf09_rh_traffic.md" if ordered else
+                           "retrieve_u64_sp28_lf09_traffic.md")
     roofline = {
         "bound": "hbm",
-        "kernel": "k_lookup<u64> (retrieve)",
+        "kernel": "k_lookup<u64> (retrieve, " + ("Robin Hood ordered table)" if ordered else "arrival order)"),
         "achieved": round(achieved, 1),
         "peak": PEAK_HBM_GBS,
         "unit": "GB/s",
@@ -230,6 +254,8 @@ def main():
         "insert_ms": round(float(t_ins), 3),
         "insert_Mops": round(n / t_ins / 1e3, 1),
         "insert_GBps_alg": round(ins_bytes / (t_ins * 1e-3) / 1e9, 1),
+        "reorder_ms": round(float(t_ro), 3),
+        "reorder_GBps_alg": round(2 * N * 16 / (t_ro * 1e-3) / 1e9, 1) if reorder else None,
         "retrieve_ms": round(float(t_ret), 3),
         "retrieve_Mops": round(2 * n / t_ret / 1e3, 1),
         "overflow": int(inf[2]),
@@ -244,7 +270,8 @@ def main():
 
     extra = {}
     if not args.no_extra:
-        extra = run_extras(L, torch, dev, S, P, tune, buckets, info, dk, dv, q, out, hit, n, N, sp, make_inputs)
+        extra = run_extras(L, torch, dev, S, P, tune, buckets, info, dk, dv, q, out, hit, n, N, sp, make_inputs,
+                           rscratch if reorder else None)
 
     if not args.no_batched:
         del buckets, out, hit, q, dk, dv
@@ -274,6 +301,7 @@ def main():
             "load_factor": alpha,
             "keys": "50% uniform u64, 50% runs of 8-512 consecutive ids",
             "probe_width": args.probe_width or "default",
+            "robin_hood_reorder": reorder,
         },
         "roofline": roofline,
         "stages": stages,
@@ -396,7 +424,7 @@ def _batch_ok(bm, n_ins, n_ers):
             and not np.any(inf[:, 4] & 0xFFFFFFFF))
 
 
-def run_extras(L, torch, dev, S, P, tune, buckets, info, dk, dv, q, out, hit, n, N, sp, make_inputs):
+def run_extras(L, torch, dev, S, P, tune, buckets, info, dk, dv, q, out, hit, n, N, sp, make_inputs, rscratch):
     """Rest of the large-map sequence on the map left by the last step, then
     the other load factors (one step each)."""
     from hashinator_amd import _lib
@@ -418,13 +446,19 @@ def run_extras(L, torch, dev, S, P, tune, buckets, info, dk, dv, q, out, hit, n,
     sb = L.hn_clean_tombstones_scratch_bytes_u64(sp, 1 << 30)
     scratch = torch.empty(sb, dtype=torch.uint8, device=dev)
     t_cl = timed(lambda: _lib.check(L.hn_clean_tombstones_u64(P(buckets), P(info), sp, P(scratch), sb, S), "clean"))
+
+    def ro():
+        if rscratch is not None:
+            _lib.check(L.hn_reorder_u64(P(buckets), P(info), sp, P(rscratch), rscratch.numel(), S), "reorder")
+
+    t_ro = timed(ro)
     t_ra = timed(lambda: _lib.check(L.hn_retrieve_u64(P(q), P(out), P(hit), P(buckets), P(info), q.numel(), S, tune),
                                     "retrieve"))
     inf = info.cpu().numpy()
     n_hit = int(hit.sum().item())
     extra["erase25_clean_retrieve"] = {
         "erase_ms": round(t_er, 3), "erase_Mops": round(er.numel() / t_er / 1e3, 1),
-        "clean_ms": round(t_cl, 3), "clean_scratch_MiB": round(sb / 2**20, 1),
+        "clean_ms": round(t_cl, 3), "clean_scratch_MiB": round(sb / 2**20, 1), "reorder_ms": round(t_ro, 3),
         "retrieve_after_ms": round(t_ra, 3), "retrieve_after_Mops": round(q.numel() / t_ra / 1e3, 1),
         "hits_after": n_hit, "expected_hits": n - n // 4, "tombstones_after": int(inf[3]),
         "ok": n_hit == n - n // 4 and int(inf[3]) == 0 and int(inf[1]) == n - n // 4,
@@ -441,6 +475,7 @@ def run_extras(L, torch, dev, S, P, tune, buckets, info, dk, dv, q, out, hit, n,
 
         timed(one)
         ti = timed(lambda: _lib.check(L.hn_insert_u64(P(dk2), P(dv2), P(buckets), P(info), n2, S, tune), "insert"))
+        ti += timed(ro)
         tr = timed(lambda: _lib.check(L.hn_retrieve_u64(P(q2), P(out2), P(hit2), P(buckets), P(info), q2.numel(),
                                                         S, tune), "retrieve"))
         extra[f"lf{a}"] = {"n": n2, "insert_Mops": round(n2 / ti / 1e3, 1),

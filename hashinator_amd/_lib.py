@@ -12,6 +12,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libhashinator_amd.so")
 
 HN_STATUS_SUCCESS, HN_STATUS_FAIL, HN_STATUS_INVALID = 0, 1, 2
+HN_INFO_ORDERED = 2
 HN_BUCKET_OVERFLOW = 64
 HN_EINVAL, HN_ESCRATCH, HN_ENOLIB = -1, -2, -3
 HN_PRED_ALL, HN_PRED_VALID, HN_PRED_MASKEQ, HN_PRED_LT, HN_PRED_GE = 0, 1, 2, 3, 4
@@ -105,6 +106,9 @@ SIGNATURES = {
     "hn_rehash_scratch_bytes": (S, []),
     "hn_rehash_u32": (I, [P, I, P, I, P, P, P]),
     "hn_rehash_u64": (I, [P, I, P, I, P, P, P]),
+    "hn_reorder_scratch_bytes": (S, [I]),
+    "hn_reorder_u32": (I, [P, P, I, P, S, P]),
+    "hn_reorder_u64": (I, [P, P, I, P, S, P]),
     "hn_compact_scratch_bytes": (S, [S]),
     "hn_copy_if_pairs_u32": (I, [P, S, PRED, P, P, P, S, P]),
     "hn_copy_if_pairs_u64": (I, [P, S, PRED, P, P, P, S, P]),

@@ -100,6 +100,7 @@ __global__ __launch_bounds__(1024) void k_batch(BatchArgs<K> a) {
                if (o > info->current_max_bucket_overflow) info->current_max_bucket_overflow = o;
             }
             info->err = (s_failed || !sp_ok) ? HN_STATUS_FAIL : HN_STATUS_SUCCESS;
+            info->reserved1 = 0; // inserts do not keep Robin Hood order
             s_placed = 0;
             s_max = 0;
             s_failed = 0;
@@ -109,13 +110,15 @@ __global__ __launch_bounds__(1024) void k_batch(BatchArgs<K> a) {
       if ((a.ops & HN_BATCH_RETRIEVE) && sp_ok) {
          LookupArgs<K> la{a.qry_keys + M.qry_off, a.qry_vals + M.qry_off, nullptr,
                           a.qry_hit ? a.qry_hit + M.qry_off : nullptr, b, info, M.qry_len};
-         lookup_range<K, G, false, LK_RETRIEVE>(la, sp, info->current_max_bucket_overflow, M.qry_len, g, groups);
+         lookup_range<K, G, false, LK_RETRIEVE>(la, sp, info->current_max_bucket_overflow, M.qry_len, g, groups,
+                                                (info->reserved1 & kInfoOrdered) != 0);
          phase_barrier();
       }
       if ((a.ops & HN_BATCH_ERASE) && sp_ok) {
          LookupArgs<K> la{a.ers_keys + M.ers_off, nullptr, nullptr, nullptr, b, info, M.ers_len};
          const uint64_t e =
-             wave_sum(lookup_range<K, G, false, LK_ERASE>(la, sp, info->current_max_bucket_overflow, M.ers_len, g, groups));
+             wave_sum(lookup_range<K, G, false, LK_ERASE>(la, sp, info->current_max_bucket_overflow, M.ers_len, g, groups,
+                                                          (info->reserved1 & kInfoOrdered) != 0));
          if (lane == 0 && e) atomicAdd(&s_erased, static_cast<unsigned long long>(e));
          __syncthreads();
          if (threadIdx.x == 0) {
@@ -166,6 +169,7 @@ __global__ __launch_bounds__(1024) void k_batch(BatchArgs<K> a) {
                info->tombstone_counter = 0;
                info->current_max_bucket_overflow = kBucketOverflow;
                info->fill -= lifted;
+               info->reserved1 = 0; // the reinsertion does not keep Robin Hood order
             }
             phase_barrier();
             InsertCounters c;

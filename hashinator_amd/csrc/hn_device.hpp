@@ -14,6 +14,11 @@
 namespace hn {
 
 constexpr int kWave = 64;
+// hn_info::reserved1 bits: 1 = an insert kernel saw a failure (transient,
+// cleared by the kernel's last wave); 2 = the table is in Robin Hood order
+// (set by the reorder pass, cleared by every insert).
+constexpr int32_t kInfoFail = 1;
+constexpr int32_t kInfoOrdered = 2;
 constexpr uint64_t kBucketOverflow = HN_BUCKET_OVERFLOW;
 
 template <typename K>
@@ -115,7 +120,7 @@ __device__ __forceinline__ void insert_epilogue(hn_info* info, uint64_t placed, 
       const int32_t waves = static_cast<int32_t>(gridDim.x * (blockDim.x / kWave));
       const int32_t prev = __hip_atomic_fetch_add(&info->reserved0, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
       if (prev == waves - 1) {
-         int32_t f = __hip_atomic_load(&info->reserved1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+         int32_t f = __hip_atomic_load(&info->reserved1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) & kInfoFail;
          if (expected_fill) {
             uint64_t now = __hip_atomic_load(&info->fill, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
             if (now != *expected_fill) f = 1;

@@ -55,7 +55,8 @@ __global__ __launch_bounds__(1024) void k_lookup(LookupArgs<K> a) {
    const uint64_t blk = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / G;
    uint64_t erased = 0;
    if (sp_ok)
-      erased = lookup_range<K, G, PAIRS, MODE>(a, sp, a.info->current_max_bucket_overflow, a.len, blk, gstride);
+      erased = lookup_range<K, G, PAIRS, MODE>(a, sp, a.info->current_max_bucket_overflow, a.len, blk, gstride,
+                                               (a.info->reserved1 & kInfoOrdered) != 0);
    if constexpr (MODE == LK_ERASE) {
       erased = wave_sum(erased);
       if (__lane_id() == 0 && erased) {
@@ -158,6 +159,7 @@ __global__ void k_clean_commit(hn_info* info, CleanHeader* h, uint64_t n, uint64
    }
    info->tombstone_counter = 0;
    info->current_max_bucket_overflow = kBucketOverflow;
+   info->reserved1 = 0; // the reinsertion does not keep Robin Hood order
 }
 
 // One region: TOMBSTONE -> EMPTY; a live element away from its home slot is

@@ -193,9 +193,15 @@ enum LookupMode { LK_RETRIEVE = 0, LK_ERASE = 1, LK_RESET = 2 };
 // Entries [0, len), starting at block `blk`, stepping `gstride` blocks; `ovf`
 // is the map's currentMaxBucketOverflow.  Returns this lane's erase count.
 // Must be called by whole waves.
+//
+// `ordered`: the table is in Robin Hood order (every cluster sorted by home,
+// see hn_reorder_*).  Then an occupant whose displacement is smaller than the
+// probe's offset has its home after the probed key's, so the key cannot lie
+// further on and the probe stops as at an EMPTY slot.  TOMBSTONEs never stop.
 template <typename K, int G, bool PAIRS, int MODE>
 __device__ __forceinline__ uint64_t lookup_range(const LookupArgs<K>& a, const int sp, const uint64_t ovf_in,
-                                                 const uint64_t len, uint64_t blk, const uint64_t gstride) {
+                                                 const uint64_t len, uint64_t blk, const uint64_t gstride,
+                                                 const bool ordered_in = false) {
    constexpr bool ERASE = MODE == LK_ERASE;
    constexpr bool RESET = MODE == LK_RESET;
    constexpr K kEmpty = empty_key<K>();
@@ -209,6 +215,7 @@ __device__ __forceinline__ uint64_t lookup_range(const LookupArgs<K>& a, const i
    // Offsets past 2^sp only revisit slots already seen, so the bound is clamped there.
    const uint64_t ovf = RESET ? nsl : ovf_in;
    const uint64_t bound = ovf < nsl ? ovf : nsl;
+   const bool ordered = ordered_in && !RESET;
 
    uint64_t erased = 0;
    K my_key = kEmpty, res_val = 0;
@@ -267,7 +274,13 @@ __device__ __forceinline__ uint64_t lookup_range(const LookupArgs<K>& a, const i
          Slot<K>* sl = a.buckets + ((base + pos) & mask);
          const Slot<K> s = *sl;
          const uint64_t bm = group_bits<G>(__ballot(valid && s.key == key), gbase);
-         const uint64_t be = RESET ? 0 : group_bits<G>(__ballot(valid && s.key == kEmpty), gbase);
+         bool stop = valid && s.key == kEmpty;
+         if (ordered && valid && !stop && s.key != key && s.key != kTomb) {
+            const uint64_t slot = (base + pos) & mask;
+            const uint64_t d_occ = (slot - (fib_home(s.key, sp) & mask)) & mask;
+            stop = d_occ < static_cast<uint64_t>(off);
+         }
+         const uint64_t be = RESET ? 0 : group_bits<G>(__ballot(stop), gbase);
          const int fm = bm ? first_bit(bm) : G;
          const int fe = be ? first_bit(be) : G;
          bool done = true;

@@ -69,10 +69,16 @@ class status:
 class Hashmap:
     """Hashinator::Hashmap<K,V> with K = V = uint32_t or uint64_t, accelerated API."""
 
-    def __init__(self, sizePower: int = 5, key_bits: int = 32, device="cuda", tune: hn_tune | None = None):
+    def __init__(self, sizePower: int = 5, key_bits: int = 32, device="cuda", tune: hn_tune | None = None,
+                 auto_reorder: float = 1.0 / 16):
+        """``auto_reorder``: after an insert of at least this fraction of the
+        bucket count, and after clean_tombstones and device_rehash, run the
+        Robin Hood reorder pass (``reorder()``), which makes later misses
+        stop early; 0 turns it off."""
         if key_bits not in (32, 64):
             raise ValueError("key_bits must be 32 or 64")
         self.kb = key_bits
+        self.auto_reorder = auto_reorder
         self.device = torch.device(device)
         self.tune = tune
         self._L = _lib.load()
@@ -187,7 +193,32 @@ class Hashmap:
             rc = self._fn("insert")(_ptr(k), _ptr(v), _ptr(self.buckets), _ptr(self._info), n, _stream(stream),
                                     self._tune())
         check(rc, "insert")
+        self._after_insert(n, stream)
+
+    def _after_insert(self, n, stream):
+        if self.auto_reorder and n >= self.auto_reorder * self.bucket_count():
+            self._reorder(stream)
         self._sync(stream)
+
+    def reorder(self, stream=None) -> bool:
+        """Robin Hood reorder of every cluster (no counterpart in the
+        reference).  Same live set and occupancy; retrieve and erase of absent
+        keys stop early afterwards.  Returns whether the table is now marked
+        ordered (False with tombstones present)."""
+        self._reorder(stream)
+        self._sync(stream)
+        return self.ordered()
+
+    def _reorder(self, stream):
+        sb = self._L.hn_reorder_scratch_bytes(self._sp)
+        scratch = torch.empty(sb, dtype=torch.uint8, device=self.device)
+        check(self._fn("reorder")(_ptr(self.buckets), _ptr(self._info), self._sp, _ptr(scratch), sb,
+                                  _stream(stream)), "reorder")
+        self._scratch_keep = scratch  # freed after the next synchronisation point at the earliest
+
+    def ordered(self) -> bool:
+        """True when the table carries the Robin Hood order flag."""
+        return bool(self.info().reserved1 & _lib.HN_INFO_ORDERED)
 
     def insertIndex(self, keys, targetLF: float = 0.5, stream=None):
         """insertIndex (hashinator.h:1338-1355): value = position in ``keys``."""
@@ -199,7 +230,7 @@ class Hashmap:
         self._grow_for(n, targetLF, stream)
         check(self._fn("insert_index")(_ptr(k), _ptr(self.buckets), _ptr(self._info), n, _stream(stream),
                                        self._tune()), "insertIndex")
-        self._sync(stream)
+        self._after_insert(n, stream)
 
     def retrieve(self, keys, vals=None, hit=None, stream=None):
         """retrieve(keys, vals, len, s) (hashinator.h:1377-1384).  ``vals`` is
@@ -242,6 +273,8 @@ class Hashmap:
         scratch = torch.empty(sb, dtype=torch.uint8, device=self.device)
         check(self._fn("clean_tombstones")(_ptr(self.buckets), _ptr(self._info), self._sp, _ptr(scratch), sb,
                                            _stream(stream)), "clean_tombstones")
+        if self.auto_reorder:
+            self._reorder(stream)  # the reinsertion leaves arrival order
         self._sync(stream)
 
     def device_rehash(self, newSizePower: int, stream=None):
@@ -256,9 +289,11 @@ class Hashmap:
         scratch = torch.empty(self._L.hn_rehash_scratch_bytes(), dtype=torch.uint8, device=self.device)
         check(self._fn("rehash")(_ptr(self.buckets), self._sp, _ptr(nb), newSizePower, _ptr(self._info),
                                  _ptr(scratch), _stream(stream)), "device_rehash")
-        self._sync(stream)
         self.buckets = nb
         self._sp = newSizePower
+        if self.auto_reorder:
+            self._reorder(stream)
+        self._sync(stream)
 
     def resize(self, newSizePower: int, t: int = targets.device, stream=None):
         """resize(p, targets, s) (hashinator.h:552-566).  Every target runs

@@ -68,6 +68,7 @@ class Hashmap {
    mutable bool _view_ok = false;
    hn_tune _tune{0, 0, 0, 0};
    bool _has_tune = false;
+   float _auto_reorder = 1.0f / 16; // reorder after inserts of at least this fraction of the buckets
 
    uint32_t hash(KEY_TYPE in) const { return static_cast<uint32_t>(HashFunction::_hash(in, _info.sizePower)); }
    const hn_tune* tune() const { return _has_tune ? &_tune : nullptr; }
@@ -220,6 +221,7 @@ public:
          _info.sizePower = newSizePower;
          _info.currentMaxBucketOverflow = defaults::BUCKET_OVERFLOW;
          _info.tombstoneCounter = 0;
+         _info.reserved1 = 0; // arrival order, not Robin Hood order
          pushInfo();
          _view.swap(nb);
          _view_ok = true;
@@ -253,6 +255,7 @@ public:
       _buckets = fresh;
       _nbuckets = size_t(1) << newSizePower;
       changed(s);
+      if (_auto_reorder > 0.0f) reorder(s);
    }
 
    template <bool prefetches = true>
@@ -303,7 +306,11 @@ public:
    template <bool warn = true>
    pair_t* expose_bucketdata() noexcept {
       if (warn) printf("Warning, exposing Hashmap internal bucket data!\n");
-      _view_ok = false; // the caller may write through it
+      _view_ok = false; // the caller may write through it, so the table is no longer known to be ordered
+      if (_info.reserved1 & HN_INFO_ORDERED) {
+         _info.reserved1 &= ~HN_INFO_ORDERED;
+         pushInfo();
+      }
       return _buckets;
    }
    MapInfo* device_info() noexcept { return _dinfo; }
@@ -445,6 +452,36 @@ public:
       return extractKeysByPattern<prefetches>(elements, rule, stack, max_size, s);
    }
 
+   /* ------------------------------------------------------------ Robin Hood order (an addition) */
+
+   /* Rewrite every cluster in home-slot order (ties by key); afterwards
+    * retrieve and erase stop a probe early, so absent keys cost about as much
+    * as present ones.  Same live set and occupancy.  No-op while tombstones
+    * are present.  Returns whether the table is now marked ordered. */
+   bool reorder(hipStream_t s = 0) {
+      const size_t sb = hn_reorder_scratch_bytes(_info.sizePower);
+      void* scratch = nullptr;
+      SPLIT_CHECK_ERR(hipMallocAsync(&scratch, sb, s));
+      chk(W64 ? hn_reorder_u64(_buckets, dinfo(), _info.sizePower, scratch, sb, st(s))
+              : hn_reorder_u32(_buckets, dinfo(), _info.sizePower, scratch, sb, st(s)));
+      SPLIT_CHECK_ERR(hipFreeAsync(scratch, s));
+      changed(s);
+      return ordered();
+   }
+   bool ordered() const noexcept { return (_info.reserved1 & HN_INFO_ORDERED) != 0; }
+   /* Fraction of the bucket count an insert must reach to be followed by
+    * reorder() (default 1/16); 0 turns the automatic pass off. */
+   void setAutoReorder(float fraction) noexcept { _auto_reorder = fraction; }
+
+private:
+   void after_insert(size_t len, hipStream_t s) {
+      if (_auto_reorder > 0.0f && static_cast<double>(len) >= _auto_reorder * static_cast<double>(_nbuckets))
+         reorder(s);
+      else
+         changed(s);
+   }
+
+public:
    /* ------------------------------------------------------------ accelerated API (hashinator.h:1261-1409) */
    template <bool prefetches = true>
    void clean_tombstones(hipStream_t s = 0) {
@@ -462,7 +499,10 @@ public:
       chk(W64 ? hn_clean_tombstones_u64(_buckets, dinfo(), sp, scratch, sb, st(s))
               : hn_clean_tombstones_u32(_buckets, dinfo(), sp, scratch, sb, st(s)));
       SPLIT_CHECK_ERR(hipFreeAsync(scratch, s));
-      changed(s);
+      if (_auto_reorder > 0.0f)
+         reorder(s); // the reinsertion leaves arrival order
+      else
+         changed(s);
    }
 
    template <bool prefetches = true>
@@ -474,7 +514,7 @@ public:
       const int64_t neededPowerSize = std::ceil(std::log2((_info.fill + len) * (1.0 / targetLF)));
       if (neededPowerSize > _info.sizePower) resize(static_cast<int>(neededPowerSize), targets::device, s);
       DeviceHasher::insert(keys, vals, _buckets, _dinfo, len, s, tune());
-      changed(s);
+      after_insert(len, s);
    }
    template <bool prefetches = true>
    void insertIndex(KEY_TYPE* keys, size_t len, float targetLF = 0.5, hipStream_t s = 0) {
@@ -485,7 +525,7 @@ public:
       const int64_t neededPowerSize = std::ceil(std::log2((_info.fill + len) * (1.0 / targetLF)));
       if (neededPowerSize > _info.sizePower) resize(static_cast<int>(neededPowerSize), targets::device, s);
       DeviceHasher::insertIndex(keys, _buckets, _dinfo, len, s, tune());
-      changed(s);
+      after_insert(len, s);
    }
    template <bool prefetches = true>
    void insert(pair_t* src, size_t len, float targetLF = 0.5, hipStream_t s = 0) {
@@ -496,7 +536,7 @@ public:
       const int64_t neededPowerSize = std::ceil(std::log2(((_info.fill) + len) * (1.0 / targetLF)));
       if (neededPowerSize > _info.sizePower) resize(static_cast<int>(neededPowerSize), targets::device, s);
       DeviceHasher::insert(src, _buckets, _dinfo, len, s, tune());
-      changed(s);
+      after_insert(len, s);
    }
    /* `hit` (optional, an addition) receives 1/0 per key. */
    template <bool prefetches = true>

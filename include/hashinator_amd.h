@@ -38,8 +38,12 @@ typedef struct ihipStream_t* hn_stream_t; /* == hipStream_t */
 #endif
 
 /* Map metadata; byte-for-byte Hashinator::Info (hashinator/defaults.h:44-54).
- * The two reserved words must be zero between calls (hn_info_init does
- * that); insert kernels use them to finalise `err` without a host round trip. */
+ * reserved0 must be zero between calls (hn_info_init does that); insert
+ * kernels use it to finalise `err` without a host round trip.  reserved1
+ * holds flags: bit 0 is transient (zero between calls), bit 1
+ * (HN_INFO_ORDERED) says the table is in Robin Hood order -- set by
+ * hn_reorder_*, cleared by every insert, clean and rehash; a caller that
+ * writes bucket memory itself must clear it. */
 typedef struct hn_info {
    int32_t size_power;
    int32_t reserved0;
@@ -51,6 +55,7 @@ typedef struct hn_info {
 } hn_info;
 
 enum { HN_STATUS_SUCCESS = 0, HN_STATUS_FAIL = 1, HN_STATUS_INVALID = 2 };
+#define HN_INFO_ORDERED 2
 
 /* BUCKET_OVERFLOW of the wave64 build (defaults.h:28-31): the initial and
  * post-clean value of current_max_bucket_overflow, and the unit it grows in. */
@@ -209,6 +214,25 @@ HN_API int hn_rehash_u32(const void* old_buckets, int old_sp, void* new_buckets,
                   hn_stream_t s);
 HN_API int hn_rehash_u64(const void* old_buckets, int old_sp, void* new_buckets, int new_sp, hn_info* info, void* scratch,
                   hn_stream_t s);
+
+/* Robin Hood reorder (no counterpart in the reference: the reference's
+ * clusters stay in arrival order).  Every cluster (maximal run of non-EMPTY
+ * slots) is rewritten in order of home slot, ties by key -- the layout that
+ * inserting the same keys in sorted-home order gives.  Live set, occupancy,
+ * fill and tombstone_counter are unchanged; current_max_bucket_overflow is
+ * recomputed from the new layout (max(64, nextOverflow(longest probe)); it
+ * never grows).  Then sets HN_INFO_ORDERED in info->reserved1, and retrieve and
+ * erase stop a probe at the first occupant displaced less than the probe
+ * offset (an absent key costs about as much as a present one).  No-op when
+ * tombstone_counter != 0 or the table has no EMPTY slot; a cluster longer
+ * than 8192 - 1024 slots is left as it is, the flag is not set and the
+ * overflow is only raised (never lowered) by the reordered clusters.
+ * `scratch` must hold hn_reorder_scratch_bytes(size_power) bytes. */
+HN_API size_t hn_reorder_scratch_bytes(int size_power);
+HN_API int hn_reorder_u32(void* buckets, hn_info* info, int size_power, void* scratch, size_t scratch_bytes,
+                          hn_stream_t s);
+HN_API int hn_reorder_u64(void* buckets, hn_info* info, int size_power, void* scratch, size_t scratch_bytes,
+                          hn_stream_t s);
 
 /* ---------------------------------------------------------------- compaction and scan */
 

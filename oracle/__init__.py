@@ -132,6 +132,7 @@ def _declare(L):
         h("clean_tombstones").restype, h("clean_tombstones").argtypes = C.c_size_t, [P, IP]
         h("rehash").restype, h("rehash").argtypes = C.c_int, [P, C.c_int, P, C.c_int, IP]
         h("max_probe").restype, h("max_probe").argtypes = C.c_uint64, [P, C.c_int]
+        h("reorder").restype, h("reorder").argtypes = C.c_int, [P, IP]
         for nm in (f"or_copy_if_pairs{w}", f"or_copy_keys_if_pairs{w}"):
             getattr(L, nm).restype = C.c_size_t
             getattr(L, nm).argtypes = [P, C.c_size_t, C.POINTER(Pred), P]
@@ -276,6 +277,9 @@ class GpuModel:
         rc = self._g("rehash")(_p(self.b), self.info.size_power, _p(nb), new_sp, C.byref(self.info))
         self.b = nb
         return rc
+
+    def reorder(self):
+        return self._g("reorder")(_p(self.b), C.byref(self.info))
 
     def max_probe(self):
         return self._g("max_probe")(_p(self.b), self.info.size_power)

@@ -419,6 +419,69 @@ static inline uint32_t or_hash_idx64(uint64_t k, int sp) { return (uint32_t)or_f
       return mx;                                                                                               \
    }                                                                                                           \
                                                                                                                \
+   /* Robin Hood order: every cluster (maximal run of non-EMPTY slots, taken cyclically) sorted by           \
+    * (home - cluster start) mod N, then by key, written back over the cluster's own slots; overflow =        \
+    * max(64, nextOverflow(longest probe of the new layout, 64)).  No-op (returns 0) when the table holds     \
+    * TOMBSTONEs or has no EMPTY slot.  Returns 1 when the table is now ordered.  (An addition of this        \
+    * library: the reference keeps arrival order; kernels_AMD.h insert_kernel.) */                           \
+   int or_gpu##W##_reorder(K* b, or_info* info) {                                                              \
+      const int sp = info->size_power;                                                                         \
+      const size_t n = (size_t)1 << sp, mask = n - 1;                                                          \
+      if (info->tombstones) return 0;                                                                          \
+      size_t e0 = n;                                                                                           \
+      for (size_t i = 0; i < n; ++i)                                                                           \
+         if (b[2 * i] == OR_EMPTY##W) {                                                                        \
+            e0 = i;                                                                                            \
+            break;                                                                                             \
+         }                                                                                                     \
+      if (e0 == n) return 0;                                                                                   \
+      K* tmp = (K*)malloc(2 * n * sizeof(K));                                                                  \
+      uint64_t* rel = (uint64_t*)malloc(n * sizeof(uint64_t));                                                 \
+      uint64_t mx = 0;                                                                                         \
+      size_t u = e0 + 1; /* unwrapped position */                                                              \
+      while (u < e0 + 1 + n) {                                                                                 \
+         if (b[2 * (u & mask)] == OR_EMPTY##W) {                                                               \
+            ++u;                                                                                               \
+            continue;                                                                                          \
+         }                                                                                                     \
+         size_t start = u, len = 0;                                                                            \
+         while (b[2 * ((start + len) & mask)] != OR_EMPTY##W) {                                                \
+            size_t q = (start + len) & mask;                                                                   \
+            tmp[2 * len] = b[2 * q];                                                                           \
+            tmp[2 * len + 1] = b[2 * q + 1];                                                                   \
+            rel[len] = ((size_t)or_hash_idx##W(b[2 * q], sp) - start) & mask;                                  \
+            ++len;                                                                                             \
+         }                                                                                                     \
+         /* insertion sort by (rel, key): clusters are short */                                                \
+         for (size_t i = 1; i < len; ++i) {                                                                    \
+            uint64_t r = rel[i];                                                                               \
+            K k = tmp[2 * i], v = tmp[2 * i + 1];                                                              \
+            size_t j = i;                                                                                      \
+            while (j > 0 && (rel[j - 1] > r || (rel[j - 1] == r && tmp[2 * (j - 1)] > k))) {                   \
+               rel[j] = rel[j - 1];                                                                            \
+               tmp[2 * j] = tmp[2 * (j - 1)];                                                                  \
+               tmp[2 * j + 1] = tmp[2 * (j - 1) + 1];                                                          \
+               --j;                                                                                            \
+            }                                                                                                  \
+            rel[j] = r;                                                                                        \
+            tmp[2 * j] = k;                                                                                    \
+            tmp[2 * j + 1] = v;                                                                                \
+         }                                                                                                     \
+         for (size_t i = 0; i < len; ++i) {                                                                    \
+            size_t q = (start + i) & mask;                                                                     \
+            b[2 * q] = tmp[2 * i];                                                                             \
+            b[2 * q + 1] = tmp[2 * i + 1];                                                                     \
+            if (i - rel[i] + 1 > mx) mx = i - rel[i] + 1;                                                      \
+         }                                                                                                     \
+         u = start + len;                                                                                      \
+      }                                                                                                        \
+      free(tmp);                                                                                               \
+      free(rel);                                                                                               \
+      info->overflow = mx > OR_GPU_BUCKET_OVERFLOW ? or_next_overflow(mx, OR_GPU_BUCKET_OVERFLOW)             \
+                                                   : OR_GPU_BUCKET_OVERFLOW;                                   \
+      return 1;                                                                                                \
+   }                                                                                                           \
+                                                                                                               \
    static int or_pred_ok##W(const or_pred* p, K key, K val) {                                                  \
       K f = p->field == 0 ? key : val;                                                                         \
       switch (p->kind) {                                                                                       \

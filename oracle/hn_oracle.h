@@ -98,6 +98,7 @@ void or_gpu32_reset_all(uint32_t* buckets, or_info* info, size_t len);
 size_t or_gpu32_clean_tombstones(uint32_t* buckets, or_info* info);
 int or_gpu32_rehash(const uint32_t* old_buckets, int old_sp, uint32_t* new_buckets, int new_sp, or_info* info);
 uint64_t or_gpu32_max_probe(const uint32_t* buckets, int size_power);
+int or_gpu32_reorder(uint32_t* buckets, or_info* info);
 
 void or_gpu64_init(uint64_t* buckets, or_info* info, int size_power);
 int or_gpu64_insert(uint64_t* buckets, or_info* info, const uint64_t* keys, const uint64_t* vals, size_t n);
@@ -110,6 +111,7 @@ void or_gpu64_reset_all(uint64_t* buckets, or_info* info, size_t len);
 size_t or_gpu64_clean_tombstones(uint64_t* buckets, or_info* info);
 int or_gpu64_rehash(const uint64_t* old_buckets, int old_sp, uint64_t* new_buckets, int new_sp, or_info* info);
 uint64_t or_gpu64_max_probe(const uint64_t* buckets, int size_power);
+int or_gpu64_reorder(uint64_t* buckets, or_info* info);
 
 /* ---- split::tools semantics ---- */
 size_t or_copy_if_pairs32(const uint32_t* in, size_t n, const or_pred* p, uint32_t* out);

@@ -59,6 +59,7 @@ bool insert_retrieve_find(int power, float lf) {
    hmap.insert(keys.data(), vals.data(), n, 1.0f);
    CHECK(hmap.peek_status() == status::success);
    CHECK(hmap.size() == n);
+   CHECK(hmap.ordered()); // a large insert is followed by the Robin Hood reorder pass
    uint8_t* hit = nullptr;
    SPLIT_CHECK_ERR(hipMallocManaged(&hit, 2 * n));
    for (size_t i = 0; i < 2 * n; ++i) out[i] = K(7);
@@ -81,6 +82,15 @@ bool insert_retrieve_find(int power, float lf) {
    size_t iterated = 0;
    for (auto it = hmap.begin(); it != hmap.end(); ++it) ++iterated;
    CHECK(iterated == n);
+   // a small insert drops the order flag; reorder() restores it; exposing the buckets drops it
+   split::SplitVector<K> one(1, q[n]), oneval(1, K(5));
+   one.optimizeGPU();
+   oneval.optimizeGPU();
+   hmap.insert(one.data(), oneval.data(), 1, 1.0f);
+   CHECK(!hmap.ordered() && hmap.size() == n + 1);
+   CHECK(hmap.reorder());
+   (void)hmap.template expose_bucketdata<false>();
+   CHECK(!hmap.ordered());
    return true;
 }
 

@@ -66,6 +66,27 @@ def map_fixture(bits, sp, seed):
     return rec
 
 
+def reorder_fixture(bits, sp, lf, seed):
+    """The reference's own map fed keys in sorted (home, key) order: the Robin
+    Hood layout.  Seeds are skipped until no cluster wraps past the last slot
+    and the map kept its size, so the image is the canonical one."""
+    R = O.ref()
+    f = getattr(R, f"ref_fibhash{bits}")
+    N = 1 << sp
+    while True:
+        keys = W.unique_keys(int(lf * N), bits, seed)
+        vals = keys ^ NDT[bits](0x9E37)
+        home = np.array([f(int(k), sp) for k in keys], dtype=np.uint64) & np.uint64(N - 1)
+        order = np.lexsort((keys, home))
+        m = O.ref_map(sp, bits)
+        m.insert(keys[order], vals[order])
+        img = m.buckets()
+        if m.info().size_power == sp and img.reshape(-1, 2)[-1, 0] == np.iinfo(NDT[bits]).max:
+            perm = np.random.default_rng(seed).permutation(keys.size)
+            return {"keys": keys[perm], "vals": vals[perm], "sp": np.array([sp]), "img": img}
+        seed += 1
+
+
 def main():
     if not O.ref_available():
         sys.exit("oracle/_ref/libhashref.so is missing: run `make -C oracle ref` first")
@@ -75,6 +96,9 @@ def main():
         for sp in (8, 10, 12):
             rec = map_fixture(bits, sp, seed=1000 * bits + sp)
             np.savez_compressed(os.path.join(HERE, f"cpumap{bits}_sp{sp}.npz"), **rec)
+        for sp in (10, 12):
+            rec = reorder_fixture(bits, sp, 0.85, seed=77 * bits + sp)
+            np.savez_compressed(os.path.join(HERE, f"rhorder{bits}_sp{sp}.npz"), **rec)
     print("fixtures written to", HERE)
 
 

@@ -82,12 +82,12 @@ def test_insert_set_and_occupancy(gpu, bits, width, lf):
     sp = 14
     keys = keyset(int(lf * (1 << sp)), bits, seed=int(lf * 100) + width)
     vals = (keys * NDT[bits](3) + NDT[bits](1)).astype(NDT[bits])
-    hm = Hashmap(sp, bits, tune=hn_tune(width, 256, 0, 0))
+    hm = Hashmap(sp, bits, tune=hn_tune(width, 256, 0, 0), auto_reorder=0)
     hm.insert(keys, vals, targetLF=1.0)
     model = fresh_model(sp, bits)
     model.insert(keys, vals)
     check_insert_invariants(hm, model, bits)
-    # overflow is exactly the rounded longest probe of the device's own image
+    # overflow is exactly the rounded longest probe of the device's own (arrival-order) image
     img = hm.host_buckets().reshape(-1)
     mp = getattr(O.lib(), f"or_gpu{bits}_max_probe")(np.ascontiguousarray(img).ctypes.data, sp)
     assert info_tuple(hm)[2] == max(64, next_overflow(mp))
@@ -193,7 +193,7 @@ def test_clean_tombstones(gpu, bits, budget):
     sp = 14
     keys = keyset(int(0.9 * (1 << sp)), bits, seed=51)
     vals = (keys ^ NDT[bits](77)).astype(NDT[bits])
-    hm = Hashmap(sp, bits)
+    hm = Hashmap(sp, bits, auto_reorder=0)  # the overflow check below is on clean's own image
     hm.insert(keys, vals, targetLF=1.0)
     rng = np.random.default_rng(budget)
     er = rng.choice(keys, keys.size // 4, replace=False)

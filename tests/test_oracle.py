@@ -209,6 +209,145 @@ def test_checks_catch_planted_mistakes():
     assert not np.array_equal(wrong, img)
 
 
+# ------------------------------------------------------------------ Robin Hood reorder
+RH = sorted(glob.glob(os.path.join(GOLD, "rhorder*_sp*.npz")))
+
+
+@pytest.mark.parametrize("path", RH, ids=[os.path.basename(p) for p in RH])
+def test_reorder_golden(path):
+    """The reference's own map, fed the keys in sorted (home, key) order,
+    produces the Robin Hood image; the oracle reaches it from any arrival order."""
+    g = np.load(path)
+    bits = 32 if "rhorder32" in path else 64
+    sp = int(g["sp"][0])
+    m = O.GpuModel(sp, bits)
+    m.insert(g["keys"], g["vals"])
+    assert m.reorder() == 1
+    assert np.array_equal(m.b, g["img"])
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference CPU-only build not present (build container only)")
+@pytest.mark.parametrize("bits", [32, 64])
+def test_reorder_vs_live_reference(bits):
+    hits = 0
+    for sp in (5, 8, 11):
+        for seed in range(12):
+            keys = W.unique_keys(int(0.7 * (1 << sp)), bits, 500 + 16 * seed + sp)
+            m = O.GpuModel(sp, bits)
+            m.insert(keys, keys)
+            assert m.reorder() == 1
+            home = O.fibhash(keys, sp, bits).astype(np.uint64) & np.uint64((1 << sp) - 1)
+            order = np.lexsort((keys, home))
+            r = O.ref_map(sp, bits)
+            r.insert(keys[order], keys[order])
+            if r.info().size_power == sp and m.b.reshape(-1, 2)[-1, 0] == EMPTY[bits]:
+                assert np.array_equal(r.buckets(), m.b)
+                hits += 1
+    assert hits >= 6
+
+
+def rh_sorted(img, bits, sp):
+    """Every cluster in non-decreasing (home offset from cluster start, key) order."""
+    img = np.asarray(img).reshape(-1, 2)
+    n = img.shape[0]
+    occ = img[:, 0] != EMPTY[bits]
+    e0 = int(np.nonzero(~occ)[0][0])
+    prev = None
+    for u in range(e0 + 1, e0 + 1 + n):
+        q = u % n
+        if not occ[q]:
+            prev = None
+            continue
+        if prev is None:
+            start = u
+        h = int(O.fibhash(img[q:q + 1, 0], sp, bits)[0]) & (n - 1)
+        cur = ((h - start) % n, int(img[q, 0]))
+        if prev is not None and cur < prev:
+            return False
+        prev = cur
+    return True
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+@pytest.mark.parametrize("sp,lf", [(4, 0.9), (9, 0.97), (12, 0.85)])
+def test_reorder_invariants(bits, sp, lf):
+    """Same live set and occupancy, clusters sorted, longest probe not longer,
+    idempotent; wrapping clusters included (high load, small tables)."""
+    for seed in range(3):
+        keys = W.unique_keys(int(lf * (1 << sp)), bits, 40 + seed)
+        m = O.GpuModel(sp, bits)
+        m.insert(keys, keys)
+        before = m.b.copy()
+        mp = m.max_probe()
+        assert not rh_sorted(before, bits, sp) or sp == 4
+        assert m.reorder() == 1
+        assert rh_sorted(m.b, bits, sp)
+        assert np.array_equal(live_pairs(m.b, bits), live_pairs(before, bits))
+        assert np.array_equal(occupancy(m.b, bits), occupancy(before, bits))
+        assert reachable_all(m.b, bits, sp)
+        assert m.max_probe() <= mp
+        assert m.info.overflow == max(64, -(-m.max_probe() // 64) * 64)
+        again = m.b.copy()
+        assert m.reorder() == 1 and np.array_equal(m.b, again)
+        # lookups with the Robin Hood early stop (a sequential restatement) agree
+        absent = W.unique_keys(keys.size, bits, 40 + seed, start=keys.size + 5)
+        q = np.concatenate([keys, absent])
+        v, h = m.retrieve(q)
+        assert np.array_equal(h, np.isin(q, keys))
+        assert np.array_equal(rh_lookup(m.b, bits, sp, q), h)
+
+
+def rh_lookup(img, bits, sp, q):
+    """Probe stops at EMPTY or at an occupant displaced less than the offset."""
+    img = np.asarray(img).reshape(-1, 2)
+    n = img.shape[0]
+    out = np.zeros(q.size, dtype=bool)
+    homes = O.fibhash(q, sp, bits).astype(np.int64) & (n - 1)
+    okeys = img[:, 0]
+    ohome = O.fibhash(okeys, sp, bits).astype(np.int64) & (n - 1)
+    for j, (k, h) in enumerate(zip(q, homes)):
+        for off in range(n):
+            s = (h + off) % n
+            if okeys[s] == k:
+                out[j] = True
+                break
+            if okeys[s] == EMPTY[bits]:
+                break
+            if okeys[s] != TOMB[bits] and (s - ohome[s]) % n < off:
+                break
+    return out
+
+
+def test_reorder_skips_tombstoned_and_full_tables():
+    bits, sp = 64, 8
+    keys = W.unique_keys(200, bits, 3)
+    m = O.GpuModel(sp, bits)
+    m.insert(keys, keys)
+    m.erase(keys[:5])
+    b = m.b.copy()
+    assert m.reorder() == 0 and np.array_equal(m.b, b)
+    full = O.GpuModel(4, bits)
+    k = W.unique_keys(16, bits, 4)
+    full.insert(k, k)
+    b = full.b.copy()
+    assert full.reorder() == 0 and np.array_equal(full.b, b)
+
+
+def test_reorder_check_catches_planted_mistake():
+    """A layout sorted by key alone (home ignored) fails the golden image."""
+    g = np.load(RH[0])
+    bits = 32 if "rhorder32" in RH[0] else 64
+    sp = int(g["sp"][0])
+    m = O.GpuModel(sp, bits)
+    m.insert(g["keys"], g["vals"])
+    img = m.b.reshape(-1, 2).copy()
+    occ = img[:, 0] != EMPTY[bits]
+    rows = img[occ]
+    img[occ] = rows[np.argsort(rows[:, 0], kind="stable")]
+    assert not np.array_equal(img.reshape(-1), g["img"])
+    assert not rh_sorted(img, bits, sp)
+
+
 # ------------------------------------------------------------------ split::tools
 @pytest.mark.parametrize("n", [0, 1, 1000, 100_001])
 def test_copy_if_and_scan_brute_force(n):
