@@ -4,12 +4,14 @@
 
 Workload (one step): a fresh Hashmap<uint64_t,uint64_t> of 2^28 slots
 (4 GiB of buckets), insert floor(0.9 * 2^28) = 241,591,910 distinct keys (half
-uniform uint64, half runs of 8..512 consecutive ids), the Robin Hood reorder
-pass (which the library's insert policy runs after a batch this large), then
-retrieve all of them plus as many absent keys (shuffled together).  Ops per
-step = 3 * n.  Inputs are synthetic, generated once, and resident in HBM
-before timing starts.  --no-reorder times the arrival-order table instead
-(the reference's layout; every miss then probes to an EMPTY slot).
+uniform uint64, half runs of 8..512 consecutive ids), then retrieve all of
+them plus as many absent keys (shuffled together).  Ops per step = 3 * n.
+Inputs are synthetic, generated once, and resident in HBM before timing
+starts.  The insert takes the path the library's maps take for a batch this
+large into an empty map: the bulk build (hn_build_u64), which writes the
+Robin Hood ordered table directly.  --no-build times the probing insert
+kernel followed by the reorder pass instead; --no-build --no-reorder the
+reference's own layout (arrival order: every miss probes to an EMPTY slot).
 
 Besides the headline line it reports, from the same process: per-stage
 timings, the rest of the large-map sequence (erase 25%, clean_tombstones,
@@ -122,7 +124,9 @@ def main():
     ap.add_argument("--lf", type=float, default=0.9)
     ap.add_argument("--probe-width", type=int, default=0)
     ap.add_argument("--no-reorder", action="store_true",
-                    help="skip the Robin Hood reorder pass (arrival-order table, as in the reference)")
+                    help="with --no-build: skip the Robin Hood reorder pass (arrival order, as in the reference)")
+    ap.add_argument("--no-build", action="store_true",
+                    help="insert with the probing kernel (hn_insert_u64) instead of the bulk build")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extra", action="store_true", help="skip the erase/clean and load-factor extras")
     ap.add_argument("--no-batched", action="store_true", help="skip the many-maps (batched vs per-map) comparison")
@@ -176,7 +180,10 @@ def main():
     hit = torch.empty(q.numel(), dtype=torch.uint8, device=dev)
     rsb = L.hn_reorder_scratch_bytes(sp)
     rscratch = torch.empty(rsb, dtype=torch.uint8, device=dev)
-    reorder = not args.no_reorder
+    use_build = not args.no_build
+    reorder = not args.no_reorder and not use_build
+    bsb = L.hn_build_scratch_bytes_u64(sp, n) if use_build else 0
+    bscratch = torch.empty(bsb, dtype=torch.uint8, device=dev) if use_build else None
     torch.cuda.synchronize()
     t_gen = time.perf_counter() - t_gen
 
@@ -189,7 +196,10 @@ def main():
         _lib.check(L.hn_table_init_u64(P(buckets), N, S), "table_init")
         _lib.check(L.hn_info_init(P(info), sp, S), "info_init")
         e[1].record(stream)
-        _lib.check(L.hn_insert_u64(P(dk), P(dv), P(buckets), P(info), n, S, tune), "insert")
+        if use_build:
+            _lib.check(L.hn_build_u64(P(dk), P(dv), P(buckets), P(info), n, sp, P(bscratch), bsb, S, None), "build")
+        else:
+            _lib.check(L.hn_insert_u64(P(dk), P(dv), P(buckets), P(info), n, S, tune), "insert")
         e[2].record(stream)
         if reorder:
             _lib.check(L.hn_reorder_u64(P(buckets), P(info), sp, P(rscratch), rsb, S), "reorder")
@@ -223,7 +233,7 @@ def main():
     inf = info.cpu().numpy()
     assert ok and int(inf[1]) == n and (int(inf[4]) & 0xFFFFFFFF) == 0, "result check failed"
     ordered = bool((int(inf[4]) >> 32) & _lib.HN_INFO_ORDERED)
-    assert ordered == reorder, "reorder flag"
+    assert ordered == (reorder or use_build), "order flag"
 
     t_init = np.mean([a[0].elapsed_time(a[1]) for a in recs])
     t_ins = np.mean([a[1].elapsed_time(a[2]) for a in recs])
@@ -251,6 +261,7 @@ def main():
     }
     stages = {
         "table_init_ms": round(float(t_init), 3),
+        "insert_path": "bulk build (hn_build_u64)" if use_build else "probing insert (hn_insert_u64)",
         "insert_ms": round(float(t_ins), 3),
         "insert_Mops": round(n / t_ins / 1e3, 1),
         "insert_GBps_alg": round(ins_bytes / (t_ins * 1e-3) / 1e9, 1),
@@ -271,7 +282,9 @@ def main():
     extra = {}
     if not args.no_extra:
         extra = run_extras(L, torch, dev, S, P, tune, buckets, info, dk, dv, q, out, hit, n, N, sp, make_inputs,
-                           rscratch if reorder else None)
+                           rscratch if (reorder or use_build) else None, bscratch)
+    if bscratch is not None:
+        del bscratch
 
     if not args.no_batched:
         del buckets, out, hit, q, dk, dv
@@ -301,7 +314,8 @@ def main():
             "load_factor": alpha,
             "keys": "50% uniform u64, 50% runs of 8-512 consecutive ids",
             "probe_width": args.probe_width or "default",
-            "robin_hood_reorder": reorder,
+            "insert_path": "build" if use_build else "probe",
+            "robin_hood_order": ordered,
         },
         "roofline": roofline,
         "stages": stages,
@@ -424,9 +438,11 @@ def _batch_ok(bm, n_ins, n_ers):
             and not np.any(inf[:, 4] & 0xFFFFFFFF))
 
 
-def run_extras(L, torch, dev, S, P, tune, buckets, info, dk, dv, q, out, hit, n, N, sp, make_inputs, rscratch):
+def run_extras(L, torch, dev, S, P, tune, buckets, info, dk, dv, q, out, hit, n, N, sp, make_inputs, rscratch,
+               bscratch):
     """Rest of the large-map sequence on the map left by the last step, then
-    the other load factors (one step each)."""
+    the other load factors (one step each), then (with the bulk build as the
+    headline path) the probing insert + reorder on the same keys."""
     from hashinator_amd import _lib
 
     extra = {}
@@ -474,8 +490,13 @@ def run_extras(L, torch, dev, S, P, tune, buckets, info, dk, dv, q, out, hit, n,
             L.hn_info_init(P(info), sp, S)
 
         timed(one)
-        ti = timed(lambda: _lib.check(L.hn_insert_u64(P(dk2), P(dv2), P(buckets), P(info), n2, S, tune), "insert"))
-        ti += timed(ro)
+        if bscratch is not None:
+            ti = timed(lambda: _lib.check(L.hn_build_u64(P(dk2), P(dv2), P(buckets), P(info), n2, sp, P(bscratch),
+                                                         bscratch.numel(), S, None), "build"))
+        else:
+            ti = timed(lambda: _lib.check(L.hn_insert_u64(P(dk2), P(dv2), P(buckets), P(info), n2, S, tune),
+                                          "insert"))
+            ti += timed(ro)
         tr = timed(lambda: _lib.check(L.hn_retrieve_u64(P(q2), P(out2), P(hit2), P(buckets), P(info), q2.numel(),
                                                         S, tune), "retrieve"))
         extra[f"lf{a}"] = {"n": n2, "insert_Mops": round(n2 / ti / 1e3, 1),
@@ -483,6 +504,15 @@ def run_extras(L, torch, dev, S, P, tune, buckets, info, dk, dv, q, out, hit, n,
                            "insert_retrieve_Mops": round(3 * n2 / (ti + tr) / 1e3, 1),
                            "hits_ok": int(hit2.sum().item()) == n2}
         del dk2, dv2, q2, out2, hit2
+    if bscratch is not None:
+        timed(lambda: (L.hn_table_init_u64(P(buckets), N, S), L.hn_info_init(P(info), sp, S)))
+        ti = timed(lambda: _lib.check(L.hn_insert_u64(P(dk), P(dv), P(buckets), P(info), n, S, tune), "insert"))
+        tr = timed(ro)
+        inf = info.cpu().numpy()
+        extra["probing_insert_path"] = {
+            "insert_ms": round(ti, 3), "reorder_ms": round(tr, 3), "insert_Mops": round(n / ti / 1e3, 1),
+            "ok": int(inf[1]) == n and (int(inf[4]) & 0xFFFFFFFF) == 0,
+        }
     return extra
 
 

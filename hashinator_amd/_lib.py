@@ -109,6 +109,12 @@ SIGNATURES = {
     "hn_reorder_scratch_bytes": (S, [I]),
     "hn_reorder_u32": (I, [P, P, I, P, S, P]),
     "hn_reorder_u64": (I, [P, P, I, P, S, P]),
+    "hn_build_scratch_bytes_u32": (S, [I, S]),
+    "hn_build_scratch_bytes_u64": (S, [I, S]),
+    "hn_build_u32": (I, [P, P, P, P, S, I, P, S, P, TUNE]),
+    "hn_build_u64": (I, [P, P, P, P, S, I, P, S, P, TUNE]),
+    "hn_build_pairs_u32": (I, [P, P, P, S, I, P, S, P, TUNE]),
+    "hn_build_pairs_u64": (I, [P, P, P, S, I, P, S, P, TUNE]),
     "hn_compact_scratch_bytes": (S, [S]),
     "hn_copy_if_pairs_u32": (I, [P, S, PRED, P, P, P, S, P]),
     "hn_copy_if_pairs_u64": (I, [P, S, PRED, P, P, P, S, P]),

@@ -1,0 +1,723 @@
+// hn_build.hip -- bulk insert into an empty table, written directly in Robin
+// Hood order.
+//
+// An insert of n keys into an empty linear-probing table (kernels_AMD.h
+// insert_kernel on a fresh map) followed by the reorder pass gives one image
+// that depends on the key set alone: in every cluster the keys stand in order
+// of home slot, ties by key.  That image is also what a max-plus scan gives
+// over the keys sorted by (home, key):  position_j = max(home_j,
+// position_{j-1} + 1), taken cyclically.  This file computes it without
+// probing:
+//
+//   1. stable radix partition of the (key, value) pairs by region
+//      (region = home >> log2 R, R = 2048 slots): per pass a per-tile digit
+//      count, an exclusive scan of the digit-major count matrix, and a stable
+//      multisplit scatter through LDS (digits of at most 9 bits, 1-3 passes);
+//   2. region boundaries in the partitioned array;
+//   3. per region, in LDS: sort by (home, key), drop all but the last
+//      occurrence of a repeated key (the value a serial insert leaves), and
+//      summarise the region as the carry function g(c) = max(a, c + b), where
+//      c is the number of keys spilling into the region from before it;
+//   4. one scan composes the carry functions (g2 after g1 is
+//      (max(a2, a1 + b2), b1 + b2)); the carry into region 0 is the least
+//      fixed point of the whole-table composition, max(A, c + B) with
+//      B = n - N < 0, i.e. c0 = A;
+//   5. per region, a max-scan places its keys and the region writes its slot
+//      range (keys and EMPTY slots) once, coalesced.
+//
+// Traffic per key: the partition passes read and write the 16 (u64) or 8
+// (u32) byte pair twice each, plus one read for the boundaries, one read and
+// write for the region sort, one read for the placement; the table is
+// written once.  No atomics on the table, no probing.
+//
+// The table's memory is the output of the first partition pass when there
+// are two, so the scratch holds one pair array, not two.  A region holding
+// more than kRegionCap keys (only with strongly skewed keys) aborts the
+// build: the table is re-initialised empty, err = INVALID, and the caller
+// inserts with the probing kernel instead.  The same err = INVALID, with the
+// table untouched, answers a table that is not empty (fill or tombstones
+// nonzero).
+#include <algorithm>
+
+#include "hn_device.hpp"
+#include "hn_host.hpp"
+#include "hn_probe.hpp"
+
+namespace hn {
+
+constexpr int kRegionLog = 11;   // 2048 slots per region
+constexpr int kPartBS = 256;     // partition workgroup
+constexpr int kPartIPT = 8;      // pairs per thread and tile
+constexpr int kPartTile = kPartBS * kPartIPT;
+constexpr int kPartSub = 8;      // tiles per partition block
+constexpr int kPartSuper = kPartTile * kPartSub;
+constexpr int kMaxDigitBits = 9; // 512 digits per pass at most
+constexpr int kRegionBS = 256;
+constexpr int kRegionIPT = 10;
+constexpr int kRegionCap = kRegionBS * kRegionIPT; // keys per region held in LDS
+
+struct BuildHeader {
+   uint32_t go;        // the table was empty: the build runs
+   uint32_t fail;      // a region over capacity: the build is abandoned
+   uint32_t max_probe; // longest probe (displacement + 1) of the placed keys
+   uint32_t pad0;
+   int64_t c0;         // carry into region 0
+   uint64_t n_distinct;
+   uint32_t pad[56];
+};
+static_assert(sizeof(BuildHeader) == 256, "header is one 256-byte line");
+
+struct RegionSum {
+   int64_t a, b; // carry out = max(a, carry in + b)
+};
+
+struct BuildPlan {
+   int sp;
+   uint64_t n;          // input pairs
+   uint64_t N;          // slots
+   int rlog;            // log2 slots per region
+   uint64_t R;          // slots per region
+   uint64_t NR;         // regions
+   int rbits;           // log2 NR
+   int passes;          // partition passes
+   int dbits[3];        // digit bits per pass (low digit first)
+   uint32_t nblocks;    // partition tiles
+   size_t off_pairs, off_mat, off_scan, off_start, off_sum, off_carry, total;
+};
+
+inline size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+
+// dcap: widest partition digit (1..9 bits); narrower digits mean more passes.
+template <typename K>
+bool build_plan(int sp, uint64_t n, int dcap, BuildPlan* P) {
+   if (dcap < 1 || dcap > kMaxDigitBits) return false;
+   if (sp < 1 || sp > std::min(max_size_power<K>(), 32)) return false;
+   P->sp = sp;
+   P->n = n;
+   P->N = 1ull << sp;
+   if (n == 0 || n >= P->N) return false;
+   P->rlog = std::min(sp, kRegionLog);
+   P->R = 1ull << P->rlog;
+   P->rbits = sp - P->rlog;
+   P->NR = 1ull << P->rbits;
+   P->passes = std::max(1, (P->rbits + dcap - 1) / dcap);
+   if (P->passes > 3) return false;
+   int left = P->rbits;
+   for (int i = 0; i < P->passes; ++i) {
+      P->dbits[i] = left / (P->passes - i) + ((left % (P->passes - i)) ? 1 : 0);
+      left -= P->dbits[i];
+   }
+   const uint64_t nb = (n + kPartSuper - 1) / kPartSuper;
+   if (nb > 0xFFFFFFFFull) return false;
+   P->nblocks = static_cast<uint32_t>(nb);
+   // Sized for the default digit width, so any narrower one fits the same scratch.
+   const uint64_t mat = (1ull << kMaxDigitBits) * nb;
+   size_t o = sizeof(BuildHeader);
+   P->off_pairs = o;
+   o = align256(o + n * sizeof(Slot<K>));
+   P->off_mat = o;
+   o = align256(o + mat * sizeof(uint32_t));
+   P->off_scan = o;
+   o = align256(o + hn_compact_scratch_bytes(mat));
+   P->off_start = o;
+   o = align256(o + (P->NR + 1) * sizeof(uint64_t));
+   P->off_sum = o;
+   o = align256(o + P->NR * sizeof(RegionSum));
+   P->off_carry = o;
+   o = align256(o + P->NR * sizeof(int64_t));
+   P->total = o;
+   return true;
+}
+
+// Input of a partition pass: the caller's arrays (pass 0) or a pair array.
+template <typename K>
+struct PartIn {
+   const K* keys;        // SRC_SOA / SRC_INDEX
+   const K* vals;        // SRC_SOA
+   const Slot<K>* pairs; // SRC_PAIRS / SRC_TABLE (a pair array)
+};
+
+template <typename K, int SRC>
+__device__ __forceinline__ K part_key(const PartIn<K>& in, uint64_t e) {
+   if constexpr (SRC == SRC_SOA || SRC == SRC_INDEX)
+      return in.keys[e];
+   else
+      return in.pairs[e].key;
+}
+
+template <typename K, int SRC>
+__device__ __forceinline__ Slot<K> part_pair(const PartIn<K>& in, uint64_t e) {
+   Slot<K> s;
+   if constexpr (SRC == SRC_SOA) {
+      s.key = in.keys[e];
+      s.val = in.vals[e];
+   } else if constexpr (SRC == SRC_INDEX) {
+      s.key = in.keys[e];
+      s.val = static_cast<K>(e);
+   } else {
+      s = in.pairs[e];
+   }
+   return s;
+}
+
+template <typename K>
+__device__ __forceinline__ uint32_t region_of(K key, int sp, int rlog) {
+   const uint64_t mask = (1ull << sp) - 1;
+   return static_cast<uint32_t>((fib_home(key, sp) & mask) >> rlog);
+}
+
+__global__ void k_build_begin(const hn_info* info, BuildHeader* h, int sp) {
+   if (threadIdx.x == 0) {
+      h->go = (info->size_power == sp && info->fill == 0 && info->tombstone_counter == 0) ? 1u : 0u;
+      h->fail = 0;
+      h->max_probe = 0;
+      h->c0 = 0;
+      h->n_distinct = 0;
+   }
+}
+
+// Per-block digit counts, written digit-major: mat[d * nblocks + block].  A
+// block covers kPartSub consecutive tiles (kPartSuper pairs), which keeps the
+// count matrix -- and its scan -- small.
+template <typename K, int SRC>
+__global__ __launch_bounds__(kPartBS) void k_part_count(PartIn<K> in, uint64_t n, uint32_t* mat, uint32_t nblocks,
+                                                        int sp, int rlog, int dshift, int dbits,
+                                                        const BuildHeader* h) {
+   __shared__ uint32_t hist[1 << kMaxDigitBits];
+   if (!h->go) return;
+   const uint32_t D = 1u << dbits, dmask = D - 1;
+   for (uint32_t d = threadIdx.x; d < D; d += kPartBS) hist[d] = 0;
+   __syncthreads();
+   const uint64_t t0 = static_cast<uint64_t>(blockIdx.x) * kPartSuper;
+   const uint64_t t1 = std::min<uint64_t>(t0 + kPartSuper, n);
+   for (uint64_t e = t0 + threadIdx.x; e < t1; e += kPartBS)
+      atomicAdd(&hist[(region_of(part_key<K, SRC>(in, e), sp, rlog) >> dshift) & dmask], 1u);
+   __syncthreads();
+   for (uint32_t d = threadIdx.x; d < D; d += kPartBS) mat[static_cast<uint64_t>(d) * nblocks + blockIdx.x] = hist[d];
+}
+
+// Stable multisplit.  A block walks its kPartSub tiles in order, keeping the
+// running output offset of every digit in LDS.  In a tile, wave w ranks the
+// pairs [512w, 512w + 512) in eight rounds of 64: lanes with equal digits are
+// found with one ballot per digit bit, and the wave's own LDS counters carry
+// the earlier rounds (no block barrier per round).  Per-wave counts then give
+// each pair's rank among equal digits in tile order; the tile is staged in
+// LDS in digit order and leaves in runs.
+template <typename K, int SRC>
+__global__ __launch_bounds__(kPartBS) void k_part_scatter(PartIn<K> in, uint64_t n, Slot<K>* out, const uint32_t* offs,
+                                                          uint32_t nblocks, int sp, int rlog, int dshift, int dbits,
+                                                          const BuildHeader* h) {
+   constexpr int NW = kPartBS / kWave;
+   constexpr int ROUNDS = kPartTile / kPartBS;
+   constexpr uint32_t DMAX = 1u << kMaxDigitBits;
+   static_assert(DMAX == 2 * kPartBS, "two digits per thread in the digit loops");
+   __shared__ Slot<K> tile_s[kPartTile];
+   __shared__ uint16_t dig_s[kPartTile];
+   __shared__ uint32_t wcnt_s[NW][DMAX];
+   __shared__ uint32_t lstart_s[DMAX];
+   __shared__ uint32_t goff_s[DMAX];
+   __shared__ uint32_t wsum_s[NW];
+   if (!h->go) return;
+   const uint32_t D = 1u << dbits, dmask = D - 1;
+   const int lane = __lane_id(), wave = threadIdx.x / kWave;
+   const uint64_t lt = (1ull << lane) - 1ull;
+   const uint32_t d0 = threadIdx.x * 2; // this thread's two digits in the digit loops
+   for (uint32_t d = d0; d < d0 + 2 && d < D; ++d) goff_s[d] = offs[static_cast<uint64_t>(d) * nblocks + blockIdx.x];
+   const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kPartSuper;
+   for (int sub = 0; sub < kPartSub; ++sub) {
+      const uint64_t t0 = base + static_cast<uint64_t>(sub) * kPartTile;
+      if (t0 >= n) break;
+      for (uint32_t d = d0; d < d0 + 2 && d < D; ++d) {
+#pragma unroll
+         for (int w = 0; w < NW; ++w) wcnt_s[w][d] = 0;
+      }
+      __syncthreads();
+      Slot<K> el[ROUNDS];
+      uint32_t dg[ROUNDS], rk[ROUNDS];
+      const uint64_t w0 = t0 + static_cast<uint64_t>(wave) * (kPartTile / NW) + lane;
+#pragma unroll
+      for (int r = 0; r < ROUNDS; ++r) {
+         const uint64_t e = w0 + static_cast<uint64_t>(r) * kWave;
+         dg[r] = 0;
+         if (e < n) {
+            el[r] = part_pair<K, SRC>(in, e);
+            dg[r] = (region_of(el[r].key, sp, rlog) >> dshift) & dmask;
+         }
+      }
+      uint32_t* wc = wcnt_s[wave];
+#pragma unroll
+      for (int r = 0; r < ROUNDS; ++r) {
+         const bool valid = w0 + static_cast<uint64_t>(r) * kWave < n;
+         uint64_t m = __ballot(valid);
+         for (int b = 0; b < dbits; ++b) {
+            const bool bit = (dg[r] >> b) & 1u;
+            const uint64_t bal = __ballot(bit);
+            m &= bit ? bal : ~bal;
+         }
+         // All lanes read the counter before its one writer (the group's
+         // first lane) updates it; a wave's LDS operations run in order.
+         if (valid) rk[r] = wc[dg[r]] + static_cast<uint32_t>(__popcll(m & lt));
+         if (valid && first_bit(m) == lane) wc[dg[r]] += static_cast<uint32_t>(__popcll(m));
+      }
+      __syncthreads();
+      // Per digit: waves' counts -> exclusive offsets over waves; tile totals -> exclusive scan.
+      uint32_t tot[2] = {0, 0};
+      for (int i = 0; i < 2; ++i) {
+         const uint32_t d = d0 + i;
+         if (d < D) {
+#pragma unroll
+            for (int w = 0; w < NW; ++w) {
+               const uint32_t c = wcnt_s[w][d];
+               wcnt_s[w][d] = tot[i];
+               tot[i] += c;
+            }
+         }
+      }
+      {
+         const uint32_t acc = tot[0] + tot[1];
+         uint32_t inc = acc;
+#pragma unroll
+         for (int o = 1; o < kWave; o <<= 1) {
+            const uint32_t y = __shfl_up(inc, o, kWave);
+            if (lane >= o) inc += y;
+         }
+         if (lane == kWave - 1) wsum_s[wave] = inc;
+         __syncthreads();
+         uint32_t bse = inc - acc;
+         for (int w = 0; w < wave; ++w) bse += wsum_s[w];
+         if (d0 < D) lstart_s[d0] = bse;
+         if (d0 + 1 < D) lstart_s[d0 + 1] = bse + tot[0];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < ROUNDS; ++r) {
+         if (w0 + static_cast<uint64_t>(r) * kWave < n) {
+            const uint32_t p = lstart_s[dg[r]] + wcnt_s[wave][dg[r]] + rk[r];
+            tile_s[p] = el[r];
+            dig_s[p] = static_cast<uint16_t>(dg[r]);
+         }
+      }
+      __syncthreads();
+      const uint32_t cnt = static_cast<uint32_t>(std::min<uint64_t>(kPartTile, n - t0));
+      for (uint32_t p = threadIdx.x; p < cnt; p += kPartBS) {
+         const uint32_t d = dig_s[p];
+         out[static_cast<uint64_t>(goff_s[d]) + (p - lstart_s[d])] = tile_s[p];
+      }
+      __syncthreads();
+      if (d0 < D) goff_s[d0] += tot[0];
+      if (d0 + 1 < D) goff_s[d0 + 1] += tot[1];
+   }
+}
+
+// start[r] = first index of region r in the partitioned array, start[NR] = n.
+template <typename K>
+__global__ __launch_bounds__(256) void k_region_starts(const Slot<K>* S, uint64_t n, uint64_t* start, uint64_t NR,
+                                                       int sp, int rlog, const BuildHeader* h) {
+   if (!h->go) return;
+   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i <= n; i += stride) {
+      const int64_t prev = i == 0 ? -1 : static_cast<int64_t>(region_of(S[i - 1].key, sp, rlog));
+      const int64_t cur = i == n ? static_cast<int64_t>(NR) : static_cast<int64_t>(region_of(S[i].key, sp, rlog));
+      for (int64_t r = prev + 1; r <= cur; ++r) start[r] = i;
+   }
+}
+
+// Exclusive scan of x[0..m) in LDS by a whole workgroup; returns the total.
+template <int BS>
+__device__ uint32_t lds_exclusive_scan(uint32_t* x, uint32_t m, uint32_t* wsum) {
+   const int lane = __lane_id(), wave = threadIdx.x / kWave;
+   const uint32_t per = (m + BS - 1) / BS;
+   const uint32_t c0 = std::min<uint32_t>(threadIdx.x * per, m), c1 = std::min<uint32_t>(c0 + per, m);
+   uint32_t acc = 0;
+   for (uint32_t i = c0; i < c1; ++i) acc += x[i];
+   uint32_t inc = acc;
+#pragma unroll
+   for (int o = 1; o < kWave; o <<= 1) {
+      const uint32_t y = __shfl_up(inc, o, kWave);
+      if (lane >= o) inc += y;
+   }
+   if (lane == kWave - 1) wsum[wave] = inc;
+   __syncthreads();
+   uint32_t run = inc - acc, total = 0;
+   for (int w = 0; w < BS / kWave; ++w) {
+      if (w < wave) run += wsum[w];
+      total += wsum[w];
+   }
+   for (uint32_t i = c0; i < c1; ++i) {
+      const uint32_t v = x[i];
+      x[i] = run;
+      run += v;
+   }
+   __syncthreads();
+   return total;
+}
+
+// Region r: sort its keys by (home, key), keep the last occurrence of a
+// repeated key, write the survivors back over the region's own range in
+// sorted order, and record the carry function.
+template <typename K>
+__global__ __launch_bounds__(kRegionBS) void k_region_sort(Slot<K>* S, const uint64_t* start, RegionSum* sum, int sp,
+                                                           int rlog, BuildHeader* h) {
+   constexpr int NW = kRegionBS / kWave;
+   constexpr uint32_t RMAX = 1u << kRegionLog;
+   __shared__ K key_s[kRegionCap];
+   __shared__ uint16_t order_s[kRegionCap];
+   __shared__ uint8_t dup_s[kRegionCap];
+   __shared__ uint32_t cnt_s[RMAX + 1];
+   __shared__ uint32_t scnt_s[RMAX + 1];
+   __shared__ uint32_t wsum_s[NW];
+   __shared__ int32_t wmax_s[NW];
+   if (!h->go) return;
+   const uint32_t r = blockIdx.x;
+   const uint64_t lo = start[r], hi = start[r + 1];
+   const uint64_t R = 1ull << rlog;
+   if (hi - lo > static_cast<uint64_t>(kRegionCap)) {
+      if (threadIdx.x == 0) __hip_atomic_store(&h->fail, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return;
+   }
+   const uint32_t m = static_cast<uint32_t>(hi - lo);
+   for (uint32_t i = threadIdx.x; i <= R; i += kRegionBS) {
+      cnt_s[i] = 0;
+      scnt_s[i] = 0;
+   }
+   __syncthreads();
+   K kk[kRegionIPT], vv[kRegionIPT];
+   uint32_t uu[kRegionIPT], sub[kRegionIPT];
+#pragma unroll
+   for (int k = 0; k < kRegionIPT; ++k) {
+      const uint32_t e = threadIdx.x + k * kRegionBS;
+      if (e < m) {
+         const Slot<K> s = S[lo + e];
+         kk[k] = s.key;
+         vv[k] = s.val;
+         key_s[e] = s.key;
+         uu[k] = static_cast<uint32_t>((fib_home(s.key, sp) & ((1ull << sp) - 1)) & (R - 1));
+         sub[k] = atomicAdd(&cnt_s[uu[k]], 1u);
+      }
+   }
+   __syncthreads();
+   lds_exclusive_scan<kRegionBS>(cnt_s, static_cast<uint32_t>(R) + 1, wsum_s);
+#pragma unroll
+   for (int k = 0; k < kRegionIPT; ++k) {
+      const uint32_t e = threadIdx.x + k * kRegionBS;
+      if (e < m) order_s[cnt_s[uu[k]] + sub[k]] = static_cast<uint16_t>(e);
+   }
+   __syncthreads();
+   // A pair is dropped when a later pair (higher index: later in the input)
+   // carries the same key.
+#pragma unroll
+   for (int k = 0; k < kRegionIPT; ++k) {
+      const uint32_t e = threadIdx.x + k * kRegionBS;
+      if (e < m) {
+         uint8_t dup = 0;
+         for (uint32_t j = cnt_s[uu[k]]; j < cnt_s[uu[k] + 1]; ++j) {
+            const uint32_t o = order_s[j];
+            dup |= (key_s[o] == kk[k] && o > e) ? 1 : 0;
+         }
+         dup_s[e] = dup;
+         if (!dup) atomicAdd(&scnt_s[uu[k]], 1u);
+      }
+   }
+   __syncthreads();
+   const uint32_t ms = lds_exclusive_scan<kRegionBS>(scnt_s, static_cast<uint32_t>(R) + 1, wsum_s);
+   int32_t M = INT32_MIN;
+#pragma unroll
+   for (int k = 0; k < kRegionIPT; ++k) {
+      const uint32_t e = threadIdx.x + k * kRegionBS;
+      if (e < m && !dup_s[e]) {
+         uint32_t rank = 0;
+         for (uint32_t j = cnt_s[uu[k]]; j < cnt_s[uu[k] + 1]; ++j) {
+            const uint32_t o = order_s[j];
+            rank += (!dup_s[o] && key_s[o] < kk[k]) ? 1u : 0u;
+         }
+         const uint32_t f = scnt_s[uu[k]] + rank;
+         Slot<K> s;
+         s.key = kk[k];
+         s.val = vv[k];
+         S[lo + f] = s;
+         M = std::max<int32_t>(M, static_cast<int32_t>(uu[k]) - static_cast<int32_t>(f));
+      }
+   }
+   M = wave_max(M);
+   if (__lane_id() == 0) wmax_s[threadIdx.x / kWave] = M;
+   __syncthreads();
+   if (threadIdx.x == 0) {
+      int32_t Mb = INT32_MIN;
+      for (int w = 0; w < NW; ++w) Mb = std::max(Mb, wmax_s[w]);
+      const int64_t mp = static_cast<int64_t>(ms), Ri = static_cast<int64_t>(R);
+      RegionSum g;
+      g.b = mp - Ri;
+      g.a = mp > 0 ? std::max<int64_t>(0, mp - Ri + Mb) : 0;
+      sum[r] = g;
+   }
+}
+
+// Carry into every region.  One workgroup: each thread composes a chunk of
+// regions, the chunk functions are scanned in LDS, then applied from c0.
+constexpr int kCarryBS = 1024;
+__device__ __forceinline__ RegionSum compose(RegionSum first, RegionSum then) {
+   RegionSum g;
+   g.a = std::max(then.a, first.a + then.b);
+   g.b = first.b + then.b;
+   return g;
+}
+
+__global__ __launch_bounds__(kCarryBS) void k_region_carry(const RegionSum* sum, int64_t* carry, uint64_t NR,
+                                                           uint64_t N, BuildHeader* h) {
+   __shared__ int64_t fa[kCarryBS], fb[kCarryBS];
+   if (!h->go || h->fail) return;
+   constexpr int64_t kNeg = INT64_MIN / 4; // identity: max(-inf, c + 0) = c
+   const uint64_t per = (NR + kCarryBS - 1) / kCarryBS;
+   const uint64_t c0 = std::min<uint64_t>(threadIdx.x * per, NR), c1 = std::min<uint64_t>(c0 + per, NR);
+   RegionSum g{kNeg, 0};
+   for (uint64_t r = c0; r < c1; ++r) g = compose(g, sum[r]);
+   fa[threadIdx.x] = g.a;
+   fb[threadIdx.x] = g.b;
+   __syncthreads();
+   for (int o = 1; o < kCarryBS; o <<= 1) {
+      RegionSum mine{fa[threadIdx.x], fb[threadIdx.x]};
+      RegionSum prev{kNeg, 0};
+      if (static_cast<int>(threadIdx.x) >= o) prev = RegionSum{fa[threadIdx.x - o], fb[threadIdx.x - o]};
+      __syncthreads();
+      if (static_cast<int>(threadIdx.x) >= o) {
+         const RegionSum c = compose(prev, mine);
+         fa[threadIdx.x] = c.a;
+         fb[threadIdx.x] = c.b;
+      }
+      __syncthreads();
+   }
+   // Whole table: max(A, c + B) with B = n' - N < 0; its fixed point is A.
+   const int64_t A = std::max<int64_t>(fa[kCarryBS - 1], 0);
+   int64_t c = A;
+   if (threadIdx.x > 0) c = std::max(fa[threadIdx.x - 1], A + fb[threadIdx.x - 1]);
+   for (uint64_t r = c0; r < c1; ++r) {
+      carry[r] = c;
+      c = std::max(sum[r].a, c + sum[r].b);
+   }
+   if (threadIdx.x == 0) {
+      h->c0 = A;
+      h->n_distinct = static_cast<uint64_t>(fb[kCarryBS - 1] + static_cast<int64_t>(N));
+   }
+}
+
+// Region r places its sorted keys after the carry and writes its slot range
+// [rR + carry, max((r+1)R, last position + 1)) once: keys and EMPTY slots.
+template <typename K>
+__global__ __launch_bounds__(kRegionBS) void k_region_place(const Slot<K>* S, const uint64_t* start,
+                                                            const RegionSum* sum, const int64_t* carry, Slot<K>* b,
+                                                            int sp, int rlog, BuildHeader* h) {
+   constexpr int NW = kRegionBS / kWave;
+   constexpr uint32_t SPAN = (1u << kRegionLog) + kRegionCap;
+   __shared__ Slot<K> el_s[kRegionCap];
+   __shared__ int32_t v_s[kRegionCap];
+   __shared__ uint16_t at_s[SPAN];
+   __shared__ int32_t wmax_s[NW];
+   __shared__ uint32_t wprobe_s[NW];
+   if (!h->go || h->fail) return;
+   const uint32_t r = blockIdx.x;
+   const int lane = __lane_id(), wave = threadIdx.x / kWave;
+   const uint64_t N = 1ull << sp, mask = N - 1, R = 1ull << rlog;
+   const uint64_t lo = start[r];
+   const int64_t ms = sum[r].b + static_cast<int64_t>(R);
+   const int64_t c = carry[r];
+   const uint32_t m = static_cast<uint32_t>(ms);
+   for (uint32_t j = threadIdx.x; j < m; j += kRegionBS) {
+      const Slot<K> s = S[lo + j];
+      el_s[j] = s;
+      v_s[j] = static_cast<int32_t>((fib_home(s.key, sp) & mask) & (R - 1)) - static_cast<int32_t>(j);
+   }
+   __syncthreads();
+   // Inclusive max-scan of v over j (chunks per thread, then across lanes and waves).
+   {
+      const uint32_t per = (m + kRegionBS - 1) / kRegionBS;
+      const uint32_t j0 = std::min<uint32_t>(threadIdx.x * per, m), j1 = std::min<uint32_t>(j0 + per, m);
+      int32_t acc = INT32_MIN;
+      for (uint32_t j = j0; j < j1; ++j) acc = std::max(acc, v_s[j]);
+      int32_t inc = acc;
+#pragma unroll
+      for (int o = 1; o < kWave; o <<= 1) {
+         const int32_t y = __shfl_up(inc, o, kWave);
+         if (lane >= o) inc = std::max(inc, y);
+      }
+      if (lane == kWave - 1) wmax_s[wave] = inc;
+      __syncthreads();
+      int32_t run = __shfl_up(inc, 1, kWave);
+      if (lane == 0) run = INT32_MIN;
+      for (int w = 0; w < wave; ++w) run = std::max(run, wmax_s[w]);
+      for (uint32_t j = j0; j < j1; ++j) {
+         run = std::max(run, v_s[j]);
+         v_s[j] = run;
+      }
+   }
+   __syncthreads();
+   const int64_t last = m ? static_cast<int64_t>(m - 1) + std::max<int64_t>(v_s[m - 1], c) : -1;
+   const int64_t end = std::max<int64_t>(static_cast<int64_t>(R), last + 1);
+   const int64_t span = end - c;
+   for (int64_t q = threadIdx.x; q < span; q += kRegionBS) at_s[q] = 0xFFFFu;
+   __syncthreads();
+   uint32_t probe = 0;
+   for (uint32_t j = threadIdx.x; j < m; j += kRegionBS) {
+      const int64_t p = static_cast<int64_t>(j) + std::max<int64_t>(v_s[j], c);
+      at_s[p - c] = static_cast<uint16_t>(j);
+      const int64_t u = static_cast<int64_t>((fib_home(el_s[j].key, sp) & mask) & (R - 1));
+      probe = std::max<uint32_t>(probe, static_cast<uint32_t>(p - u + 1));
+   }
+   __syncthreads();
+   const uint64_t base = static_cast<uint64_t>(r) * R;
+   for (int64_t q = threadIdx.x; q < span; q += kRegionBS) {
+      const uint16_t j = at_s[q];
+      Slot<K> s;
+      if (j != 0xFFFFu) {
+         s = el_s[j];
+      } else {
+         s.key = empty_key<K>();
+         s.val = 0;
+      }
+      b[(base + static_cast<uint64_t>(c + q)) & mask] = s;
+   }
+   probe = wave_max(probe);
+   if (lane == 0) wprobe_s[wave] = probe;
+   __syncthreads();
+   if (threadIdx.x == 0) {
+      uint32_t pb = 0;
+      for (int w = 0; w < NW; ++w) pb = std::max(pb, wprobe_s[w]);
+      if (pb > __hip_atomic_load(&h->max_probe, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+         atomic_max_relaxed(&h->max_probe, pb);
+   }
+}
+
+// Finish: the Info of the built table, or (build abandoned after the table's
+// memory served as scratch) an empty table again with err = INVALID.
+template <typename K>
+__global__ __launch_bounds__(256) void k_build_end(Slot<K>* b, uint64_t N, hn_info* info, const BuildHeader* h) {
+   if (!h->go) {
+      if (blockIdx.x == 0 && threadIdx.x == 0) info->err = HN_STATUS_INVALID;
+      return;
+   }
+   if (h->fail) {
+      const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+      for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < N; i += stride) {
+         Slot<K> s;
+         s.key = empty_key<K>();
+         s.val = 0;
+         b[i] = s;
+      }
+   }
+   if (blockIdx.x == 0 && threadIdx.x == 0) {
+      info->reserved0 = 0;
+      info->tombstone_counter = 0;
+      if (h->fail) {
+         info->fill = 0;
+         info->current_max_bucket_overflow = kBucketOverflow;
+         info->err = HN_STATUS_INVALID;
+         info->reserved1 = 0;
+      } else {
+         info->fill = h->n_distinct;
+         info->current_max_bucket_overflow =
+             std::max<uint64_t>(kBucketOverflow, next_overflow(h->max_probe, kBucketOverflow));
+         info->err = HN_STATUS_SUCCESS;
+         info->reserved1 = kInfoOrdered;
+      }
+   }
+}
+
+template <typename K, int SRC>
+int launch_part(const BuildPlan& P, int pass, PartIn<K> in, Slot<K>* out, uint32_t* mat, uint32_t* scanned,
+                void* scan_scratch, size_t scan_bytes, BuildHeader* h, hipStream_t s) {
+   int dshift = 0;
+   for (int i = 0; i < pass; ++i) dshift += P.dbits[i];
+   const int dbits = P.dbits[pass];
+   const uint64_t D = 1ull << dbits;
+   hipLaunchKernelGGL((k_part_count<K, SRC>), dim3(P.nblocks), dim3(kPartBS), 0, s, in, P.n, mat, P.nblocks, P.sp,
+                      P.rlog, dshift, dbits, h);
+   int rc = hn_exclusive_scan_u32(mat, scanned, D * P.nblocks, scan_scratch, scan_bytes,
+                                  reinterpret_cast<hn_stream_t>(s));
+   if (rc) return rc;
+   hipLaunchKernelGGL((k_part_scatter<K, SRC>), dim3(P.nblocks), dim3(kPartBS), 0, s, in, P.n, out, scanned, P.nblocks,
+                      P.sp, P.rlog, dshift, dbits, h);
+   return static_cast<int>(hipGetLastError());
+}
+
+template <typename K>
+int build(const void* keys, const void* vals, const void* pairs, void* buckets, hn_info* info, size_t len, int sp,
+          void* scratch, size_t bytes, hipStream_t s, const hn_tune* t) {
+   if (!buckets || !info) return HN_EINVAL;
+   if (len == 0) return 0;
+   if (!keys && !pairs) return HN_EINVAL;
+   const int dcap = (t && t->reserved) ? t->reserved : kMaxDigitBits;
+   BuildPlan P;
+   if (!build_plan<K>(sp, len, dcap, &P)) return HN_EINVAL;
+   if (!scratch || bytes < P.total) return HN_ESCRATCH;
+   char* base = static_cast<char*>(scratch);
+   BuildHeader* h = reinterpret_cast<BuildHeader*>(base);
+   Slot<K>* S = reinterpret_cast<Slot<K>*>(base + P.off_pairs);
+   uint32_t* mat = reinterpret_cast<uint32_t*>(base + P.off_mat);
+   void* scan_scratch = base + P.off_scan;
+   const size_t scan_bytes = P.off_start - P.off_scan;
+   uint64_t* start = reinterpret_cast<uint64_t*>(base + P.off_start);
+   RegionSum* sum = reinterpret_cast<RegionSum*>(base + P.off_sum);
+   int64_t* carry = reinterpret_cast<int64_t*>(base + P.off_carry);
+   Slot<K>* T = static_cast<Slot<K>*>(buckets);
+
+   hipLaunchKernelGGL(k_build_begin, dim3(1), dim3(64), 0, s, info, h, sp);
+   // Pass i writes S when (passes - 1 - i) is even, the table's memory otherwise,
+   // so the last pass always lands in S.
+   const Slot<K>* src = nullptr;
+   for (int pass = 0; pass < P.passes; ++pass) {
+      Slot<K>* dst = ((P.passes - 1 - pass) % 2 == 0) ? S : T;
+      int rc;
+      if (pass == 0) {
+         PartIn<K> in{static_cast<const K*>(keys), static_cast<const K*>(vals), static_cast<const Slot<K>*>(pairs)};
+         if (pairs)
+            rc = launch_part<K, SRC_PAIRS>(P, pass, in, dst, mat, mat, scan_scratch, scan_bytes, h, s);
+         else if (vals)
+            rc = launch_part<K, SRC_SOA>(P, pass, in, dst, mat, mat, scan_scratch, scan_bytes, h, s);
+         else
+            rc = launch_part<K, SRC_INDEX>(P, pass, in, dst, mat, mat, scan_scratch, scan_bytes, h, s);
+      } else {
+         PartIn<K> in{nullptr, nullptr, src};
+         rc = launch_part<K, SRC_TABLE>(P, pass, in, dst, mat, mat, scan_scratch, scan_bytes, h, s);
+      }
+      if (rc) return rc;
+      src = dst;
+   }
+   hipLaunchKernelGGL(k_region_starts<K>, dim3(stream_blocks(P.n + 1, 256)), dim3(256), 0, s, S, P.n, start, P.NR,
+                      sp, P.rlog, h);
+   hipLaunchKernelGGL(k_region_sort<K>, dim3(static_cast<uint32_t>(P.NR)), dim3(kRegionBS), 0, s, S, start, sum, sp,
+                      P.rlog, h);
+   hipLaunchKernelGGL(k_region_carry, dim3(1), dim3(kCarryBS), 0, s, sum, carry, P.NR, P.N, h);
+   hipLaunchKernelGGL(k_region_place<K>, dim3(static_cast<uint32_t>(P.NR)), dim3(kRegionBS), 0, s, S, start, sum,
+                      carry, T, sp, P.rlog, h);
+   hipLaunchKernelGGL(k_build_end<K>, dim3(stream_blocks(P.N, 256)), dim3(256), 0, s, T, P.N, info, h);
+   return static_cast<int>(hipGetLastError());
+}
+
+template <typename K>
+size_t build_scratch_bytes(int sp, size_t len) {
+   BuildPlan P;
+   return build_plan<K>(sp, len, kMaxDigitBits, &P) ? P.total : 0;
+}
+
+} // namespace hn
+
+using namespace hn;
+extern "C" {
+size_t hn_build_scratch_bytes_u32(int sp, size_t len) { return build_scratch_bytes<uint32_t>(sp, len); }
+size_t hn_build_scratch_bytes_u64(int sp, size_t len) { return build_scratch_bytes<uint64_t>(sp, len); }
+int hn_build_u32(const void* k, const void* v, void* b, hn_info* i, size_t n, int sp, void* sc, size_t sb,
+                 hn_stream_t s, const hn_tune* t) {
+   return build<uint32_t>(k, v, nullptr, b, i, n, sp, sc, sb, s, t);
+}
+int hn_build_u64(const void* k, const void* v, void* b, hn_info* i, size_t n, int sp, void* sc, size_t sb,
+                 hn_stream_t s, const hn_tune* t) {
+   return build<uint64_t>(k, v, nullptr, b, i, n, sp, sc, sb, s, t);
+}
+int hn_build_pairs_u32(const void* p, void* b, hn_info* i, size_t n, int sp, void* sc, size_t sb, hn_stream_t s,
+                       const hn_tune* t) {
+   return build<uint32_t>(nullptr, nullptr, p, b, i, n, sp, sc, sb, s, t);
+}
+int hn_build_pairs_u64(const void* p, void* b, hn_info* i, size_t n, int sp, void* sc, size_t sb, hn_stream_t s,
+                       const hn_tune* t) {
+   return build<uint64_t>(nullptr, nullptr, p, b, i, n, sp, sc, sb, s, t);
+}
+} // extern "C"

@@ -70,15 +70,19 @@ class Hashmap:
     """Hashinator::Hashmap<K,V> with K = V = uint32_t or uint64_t, accelerated API."""
 
     def __init__(self, sizePower: int = 5, key_bits: int = 32, device="cuda", tune: hn_tune | None = None,
-                 auto_reorder: float = 1.0 / 16):
+                 auto_reorder: float = 1.0 / 16, auto_build: float = 1.0 / 16):
         """``auto_reorder``: after an insert of at least this fraction of the
         bucket count, and after clean_tombstones and device_rehash, run the
         Robin Hood reorder pass (``reorder()``), which makes later misses
-        stop early; 0 turns it off."""
+        stop early; 0 turns it off.  ``auto_build``: an insert of at least
+        this fraction of the bucket count into an empty table takes the bulk
+        path (hn_build_*: partition + per-region sort, no probing), which
+        writes the reordered image directly; 0 turns it off."""
         if key_bits not in (32, 64):
             raise ValueError("key_bits must be 32 or 64")
         self.kb = key_bits
         self.auto_reorder = auto_reorder
+        self.auto_build = auto_build
         self.device = torch.device(device)
         self.tune = tune
         self._L = _lib.load()
@@ -187,6 +191,8 @@ class Hashmap:
             self._info.view(torch.int32)[8] = status.success
             return
         self._grow_for(n, targetLF, stream)
+        if self._try_build(k, None if vals is None else v, n, vals is None, stream):
+            return
         if vals is None:
             rc = self._fn("insert_pairs")(_ptr(k), _ptr(self.buckets), _ptr(self._info), n, _stream(stream), self._tune())
         else:
@@ -194,6 +200,31 @@ class Hashmap:
                                     self._tune())
         check(rc, "insert")
         self._after_insert(n, stream)
+
+    def _try_build(self, k, v, n, pairs, stream) -> bool:
+        """Bulk path for a large insert into an empty table.  False when it
+        does not apply or the device declined it (err = INVALID: the table is
+        then empty and the probing insert runs instead)."""
+        N = self.bucket_count()
+        if not self.auto_build or n < self.auto_build * N or n >= N:
+            return False
+        i = self.info()
+        if i.fill != 0 or i.tombstone_counter != 0:
+            return False
+        sb = getattr(self._L, f"hn_build_scratch_bytes_{self._sfx}")(self._sp, n)
+        if sb == 0:
+            return False
+        scratch = torch.empty(sb, dtype=torch.uint8, device=self.device)
+        if pairs:
+            rc = self._fn("build_pairs")(_ptr(k), _ptr(self.buckets), _ptr(self._info), n, self._sp, _ptr(scratch),
+                                         sb, _stream(stream), self._tune())
+        else:
+            rc = self._fn("build")(_ptr(k), _ptr(v) if v is not None else None, _ptr(self.buckets), _ptr(self._info),
+                                   n, self._sp, _ptr(scratch), sb, _stream(stream), self._tune())
+        check(rc, "build")
+        self._sync(stream)
+        del scratch
+        return self.info().err == status.success
 
     def _after_insert(self, n, stream):
         if self.auto_reorder and n >= self.auto_reorder * self.bucket_count():
@@ -228,6 +259,8 @@ class Hashmap:
             self._info.view(torch.int32)[8] = status.success
             return
         self._grow_for(n, targetLF, stream)
+        if self._try_build(k, None, n, False, stream):
+            return
         check(self._fn("insert_index")(_ptr(k), _ptr(self.buckets), _ptr(self._info), n, _stream(stream),
                                        self._tune()), "insertIndex")
         self._after_insert(n, stream)

@@ -69,6 +69,7 @@ class Hashmap {
    hn_tune _tune{0, 0, 0, 0};
    bool _has_tune = false;
    float _auto_reorder = 1.0f / 16; // reorder after inserts of at least this fraction of the buckets
+   float _auto_build = 1.0f / 16;   // bulk-build inserts of at least this fraction into an empty table
 
    uint32_t hash(KEY_TYPE in) const { return static_cast<uint32_t>(HashFunction::_hash(in, _info.sizePower)); }
    const hn_tune* tune() const { return _has_tune ? &_tune : nullptr; }
@@ -473,7 +474,33 @@ public:
     * reorder() (default 1/16); 0 turns the automatic pass off. */
    void setAutoReorder(float fraction) noexcept { _auto_reorder = fraction; }
 
+   /* Fraction of the bucket count an insert into an empty table must reach
+    * to take the bulk path (hn_build_*: partition + per-region sort, no
+    * probing; writes the reordered image directly); 0 turns it off. */
+   void setAutoBuild(float fraction) noexcept { _auto_build = fraction; }
+
 private:
+   /* The bulk path; false when it does not apply or the device declined it
+    * (then the table is empty and the probing insert runs). */
+   bool try_build(const void* keys, const void* vals, const void* pairs, size_t len, hipStream_t s) {
+      if (_auto_build <= 0.0f || static_cast<double>(len) < _auto_build * static_cast<double>(_nbuckets) ||
+          len >= _nbuckets || _info.fill != 0 || _info.tombstoneCounter != 0)
+         return false;
+      const int sp = _info.sizePower;
+      const size_t sb = W64 ? hn_build_scratch_bytes_u64(sp, len) : hn_build_scratch_bytes_u32(sp, len);
+      if (sb == 0) return false;
+      void* scratch = nullptr;
+      SPLIT_CHECK_ERR(hipMallocAsync(&scratch, sb, s));
+      if (pairs)
+         chk(W64 ? hn_build_pairs_u64(pairs, _buckets, dinfo(), len, sp, scratch, sb, st(s), tune())
+                 : hn_build_pairs_u32(pairs, _buckets, dinfo(), len, sp, scratch, sb, st(s), tune()));
+      else
+         chk(W64 ? hn_build_u64(keys, vals, _buckets, dinfo(), len, sp, scratch, sb, st(s), tune())
+                 : hn_build_u32(keys, vals, _buckets, dinfo(), len, sp, scratch, sb, st(s), tune()));
+      SPLIT_CHECK_ERR(hipFreeAsync(scratch, s));
+      changed(s);
+      return _info.err == status::success;
+   }
    void after_insert(size_t len, hipStream_t s) {
       if (_auto_reorder > 0.0f && static_cast<double>(len) >= _auto_reorder * static_cast<double>(_nbuckets))
          reorder(s);
@@ -513,6 +540,7 @@ public:
       }
       const int64_t neededPowerSize = std::ceil(std::log2((_info.fill + len) * (1.0 / targetLF)));
       if (neededPowerSize > _info.sizePower) resize(static_cast<int>(neededPowerSize), targets::device, s);
+      if (try_build(keys, vals, nullptr, len, s)) return;
       DeviceHasher::insert(keys, vals, _buckets, _dinfo, len, s, tune());
       after_insert(len, s);
    }
@@ -524,6 +552,7 @@ public:
       }
       const int64_t neededPowerSize = std::ceil(std::log2((_info.fill + len) * (1.0 / targetLF)));
       if (neededPowerSize > _info.sizePower) resize(static_cast<int>(neededPowerSize), targets::device, s);
+      if (try_build(keys, nullptr, nullptr, len, s)) return;
       DeviceHasher::insertIndex(keys, _buckets, _dinfo, len, s, tune());
       after_insert(len, s);
    }
@@ -535,6 +564,7 @@ public:
       }
       const int64_t neededPowerSize = std::ceil(std::log2(((_info.fill) + len) * (1.0 / targetLF)));
       if (neededPowerSize > _info.sizePower) resize(static_cast<int>(neededPowerSize), targets::device, s);
+      if (try_build(nullptr, nullptr, src, len, s)) return;
       DeviceHasher::insert(src, _buckets, _dinfo, len, s, tune());
       after_insert(len, s);
    }

@@ -234,6 +234,34 @@ HN_API int hn_reorder_u32(void* buckets, hn_info* info, int size_power, void* sc
 HN_API int hn_reorder_u64(void* buckets, hn_info* info, int size_power, void* scratch, size_t scratch_bytes,
                           hn_stream_t s);
 
+/* Bulk insert into an EMPTY table (fill == 0, tombstone_counter == 0):
+ * Hasher::insert (hashers.h:55-70, kernels_AMD.h insert_kernel) on a fresh
+ * map, producing directly the image hn_insert_* followed by hn_reorder_*
+ * gives -- every cluster in home-slot order, ties by key, HN_INFO_ORDERED set,
+ * overflow = max(64, nextOverflow(longest probe)).  A key repeated in the
+ * batch keeps the value of its last occurrence (what a serial insert leaves).
+ * `vals` NULL stores each key's index in the batch (insertIndex,
+ * hashers.h:72-86); the _pairs variant reads hash_pair elements.  No probing
+ * and no atomics on the table: a stable radix partition by 2048-slot region,
+ * an LDS sort per region and a carry scan (about 10 streaming passes' worth
+ * of the pair array).  Needs len < 2^size_power and `scratch` of
+ * hn_build_scratch_bytes_*(size_power, len) bytes; the table's previous slot
+ * contents are overwritten.  When the table is not empty (table untouched) or
+ * one region receives more than 2560 keys (strongly skewed keys; the table is
+ * left empty) the call sets err = INVALID and the caller inserts with
+ * hn_insert_* instead.  tune->reserved (1..9, 0 = 9) caps the partition digit
+ * width in bits; probe_width and block_size are not used. */
+HN_API size_t hn_build_scratch_bytes_u32(int size_power, size_t len);
+HN_API size_t hn_build_scratch_bytes_u64(int size_power, size_t len);
+HN_API int hn_build_u32(const void* keys, const void* vals, void* buckets, hn_info* info, size_t len, int size_power,
+                        void* scratch, size_t scratch_bytes, hn_stream_t s, const hn_tune* t);
+HN_API int hn_build_u64(const void* keys, const void* vals, void* buckets, hn_info* info, size_t len, int size_power,
+                        void* scratch, size_t scratch_bytes, hn_stream_t s, const hn_tune* t);
+HN_API int hn_build_pairs_u32(const void* pairs, void* buckets, hn_info* info, size_t len, int size_power,
+                              void* scratch, size_t scratch_bytes, hn_stream_t s, const hn_tune* t);
+HN_API int hn_build_pairs_u64(const void* pairs, void* buckets, hn_info* info, size_t len, int size_power,
+                              void* scratch, size_t scratch_bytes, hn_stream_t s, const hn_tune* t);
+
 /* ---------------------------------------------------------------- compaction and scan */
 
 /* Scratch for one compaction or scan of n elements

@@ -1,0 +1,203 @@
+"""Bulk insert into an empty table (hn_build_*): the image equals the oracle's
+serial insert followed by its Robin Hood reorder, bit for bit (the reordered
+image is a function of the key set; the oracle's reorder is pinned to the
+reference's own map in tests/test_oracle.py), and the Info matches
+(fill, overflow recomputed, ordered flag, err)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from hashinator_amd import Hashmap, hn_tune, to_numpy
+from hashinator_amd import _lib
+from tests.helpers import NDT, keyset
+
+pytestmark = pytest.mark.gpu
+
+TDT = {32: torch.int32, 64: torch.int64}
+VDT = {32: np.int32, 64: np.int64}
+
+
+def dev(a, bits):
+    return torch.from_numpy(np.ascontiguousarray(a).view(VDT[bits])).cuda()
+
+
+def run_build(bits, sp, keys, vals=None, pairs=False, digit_bits=0, info=None, table=None):
+    """Direct C-ABI call on a fresh table; returns (image, info words, rc)."""
+    L = _lib.load()
+    sfx = "u32" if bits == 32 else "u64"
+    N = 1 << sp
+    b = table if table is not None else torch.empty((N, 2), dtype=TDT[bits], device="cuda")
+    if table is None:
+        _lib.check(getattr(L, f"hn_table_init_{sfx}")(b.data_ptr(), N, None), "init")
+    inf = info if info is not None else torch.zeros(5, dtype=torch.int64, device="cuda")
+    if info is None:
+        _lib.check(L.hn_info_init(inf.data_ptr(), sp, None), "info")
+    sb = getattr(L, f"hn_build_scratch_bytes_{sfx}")(sp, keys.size)
+    sc = torch.empty(sb, dtype=torch.uint8, device="cuda")
+    tune = C.byref(hn_tune(0, 0, 0, digit_bits)) if digit_bits else None
+    if pairs:
+        pr = np.stack([keys, vals], axis=1)
+        dp = dev(pr, bits)
+        rc = getattr(L, f"hn_build_pairs_{sfx}")(dp.data_ptr(), b.data_ptr(), inf.data_ptr(), keys.size, sp,
+                                                 sc.data_ptr(), sb, None, tune)
+    else:
+        dk = dev(keys, bits)
+        dv = dev(vals, bits) if vals is not None else None
+        rc = getattr(L, f"hn_build_{sfx}")(dk.data_ptr(), dv.data_ptr() if dv is not None else None, b.data_ptr(),
+                                           inf.data_ptr(), keys.size, sp, sc.data_ptr(), sb, None, tune)
+    torch.cuda.synchronize()
+    img = to_numpy(b, bits).reshape(-1)
+    return img, inf.cpu().numpy(), rc
+
+
+def expect(bits, sp, keys, vals):
+    m = O.GpuModel(sp, bits)
+    m.insert(keys, vals)
+    assert m.reorder() == 1
+    return m
+
+
+def check(img, inf, m):
+    assert np.array_equal(img, m.b)
+    assert int(inf[1]) == m.info.fill
+    assert int(inf[2]) == m.info.overflow
+    assert int(inf[3]) == 0
+    assert (int(inf[4]) & 0xFFFFFFFF) == 0  # err = SUCCESS
+    assert (int(inf[4]) >> 32) == _lib.HN_INFO_ORDERED
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+@pytest.mark.parametrize("sp,lf", [(1, 0.5), (3, 0.9), (8, 0.99), (11, 0.9), (12, 0.5), (14, 0.9), (16, 0.95),
+                                   (20, 0.9), (21, 0.9)])
+def test_build_image_exact(gpu, bits, sp, lf):
+    n = max(1, min((1 << sp) - 1, int(lf * (1 << sp))))
+    keys = keyset(n, bits, seed=sp * 7 + bits)
+    vals = (keys ^ NDT[bits](0x1234)).astype(NDT[bits])
+    img, inf, rc = run_build(bits, sp, keys, vals)
+    assert rc == 0
+    check(img, inf, expect(bits, sp, keys, vals))
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+@pytest.mark.parametrize("digit_bits", [1, 2, 4])
+def test_build_multi_pass(gpu, bits, digit_bits):
+    # 2^16 slots = 32 regions (5 region bits): digits of 1/2/4 bits need 3/3/2 passes
+    # (3 passes is the cap; 1-bit digits would need 5 and are refused)
+    sp = 16
+    keys = keyset(int(0.9 * (1 << sp)), bits, seed=digit_bits)
+    vals = np.arange(keys.size).astype(NDT[bits])
+    img, inf, rc = run_build(bits, sp, keys, vals, digit_bits=digit_bits)
+    if digit_bits == 1:
+        assert rc == _lib.HN_EINVAL
+        return
+    assert rc == 0
+    check(img, inf, expect(bits, sp, keys, vals))
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_build_index_and_pairs(gpu, bits):
+    sp = 15
+    keys = keyset(int(0.85 * (1 << sp)), bits, seed=3)
+    img, inf, rc = run_build(bits, sp, keys, None)  # vals NULL: index as value
+    assert rc == 0
+    check(img, inf, expect(bits, sp, keys, np.arange(keys.size).astype(NDT[bits])))
+    vals = (keys * NDT[bits](7)).astype(NDT[bits])
+    img, inf, rc = run_build(bits, sp, keys, vals, pairs=True)
+    assert rc == 0
+    check(img, inf, expect(bits, sp, keys, vals))
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_build_duplicates_last_wins(gpu, bits):
+    sp = 14
+    base = keyset(9000, bits, seed=4)
+    rng = np.random.default_rng(4)
+    rep = base[rng.choice(base.size, 3000, replace=True)]
+    keys = np.concatenate([base, rep])
+    rng.shuffle(keys)
+    vals = np.arange(keys.size).astype(NDT[bits])
+    img, inf, rc = run_build(bits, sp, keys, vals)
+    assert rc == 0
+    m = expect(bits, sp, keys, vals)  # serial insert: the last occurrence's value
+    check(img, inf, m)
+    assert m.info.fill == np.unique(keys).size
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_build_refuses_nonempty_table(gpu, bits):
+    sp = 12
+    keys = keyset(2000, bits, seed=5)
+    hm = Hashmap(sp, bits, auto_build=0, auto_reorder=0)
+    hm.insert(keys[:100], keys[:100], targetLF=1.0)
+    before = hm.host_buckets().copy()
+    img, inf, rc = run_build(bits, sp, keys[100:], keys[100:], info=hm._info, table=hm.buckets)
+    assert rc == 0
+    assert (int(inf[4]) & 0xFFFFFFFF) == _lib.HN_STATUS_INVALID
+    assert int(inf[1]) == 100
+    assert np.array_equal(hm.host_buckets(), before)
+
+
+def concentrated_keys(bits, sp, n, span, seed):
+    out = []
+    rng = np.random.default_rng(seed)
+    while sum(len(o) for o in out) < n:
+        c = rng.integers(1, 2**31 if bits == 32 else 2**62, 1 << 20, dtype=np.uint64).astype(NDT[bits])
+        h = O.fibhash(c, sp, bits).astype(np.uint64) & np.uint64((1 << sp) - 1)
+        out.append(c[h < span])
+    k = np.unique(np.concatenate(out))[:n]
+    return k[np.random.default_rng(seed + 1).permutation(k.size)]
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_build_overfull_region_declines_and_map_falls_back(gpu, bits):
+    # 3000 keys homed in one 2048-slot region: over the 2560-key region capacity
+    sp = 14
+    keys = concentrated_keys(bits, sp, 3000, 2048, seed=6)
+    img, inf, rc = run_build(bits, sp, keys, keys)
+    assert rc == 0
+    assert (int(inf[4]) & 0xFFFFFFFF) == _lib.HN_STATUS_INVALID and int(inf[1]) == 0
+    assert np.all(img.reshape(-1, 2)[:, 0] == np.iinfo(NDT[bits]).max)  # left empty
+    # the map's insert falls back to the probing kernel (then reorders)
+    hm = Hashmap(sp, bits)
+    hm.insert(keys, keys, targetLF=1.0)
+    assert hm.size() == keys.size and hm.info().err == 0
+    _, v, h = hm.retrieve(keys)
+    assert h.cpu().numpy().all() and np.array_equal(to_numpy(v, bits), keys)
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_map_insert_uses_build(gpu, bits):
+    sp = 16
+    keys = keyset(int(0.9 * (1 << sp)), bits, seed=9)
+    vals = (keys + NDT[bits](5)).astype(NDT[bits])
+    hm = Hashmap(sp, bits)
+    hm.insert(keys, vals, targetLF=1.0)
+    m = expect(bits, sp, keys, vals)
+    assert np.array_equal(hm.host_buckets().reshape(-1), m.b)
+    assert hm.peek_status() == 0 and hm.size() == keys.size
+    absent = keyset(keys.size, bits, seed=10, start=1 << 24)
+    absent = absent[~np.isin(absent, keys)]
+    q = np.concatenate([keys, absent])
+    _, v, h = hm.retrieve(q)
+    ev, eh = m.retrieve(q)
+    assert np.array_equal(h.cpu().numpy().astype(bool), eh)
+    assert np.array_equal(to_numpy(v, bits)[eh], ev[eh])
+
+
+def test_build_invalid_args(gpu):
+    L = _lib.load()
+    assert L.hn_build_scratch_bytes_u64(10, 1 << 10) == 0  # len must be < 2^sp
+    assert L.hn_build_scratch_bytes_u64(10, 1000) > 0
+    assert L.hn_build_u64(None, None, None, None, 10, 10, None, 0, None, None) == _lib.HN_EINVAL
+    b = torch.empty((1024, 2), dtype=torch.int64, device="cuda")
+    inf = torch.zeros(5, dtype=torch.int64, device="cuda")
+    k = torch.zeros(10, dtype=torch.int64, device="cuda")
+    sb = L.hn_build_scratch_bytes_u64(10, 10)
+    sc = torch.empty(sb, dtype=torch.uint8, device="cuda")
+    assert L.hn_build_u64(k.data_ptr(), None, b.data_ptr(), inf.data_ptr(), 10, 10, sc.data_ptr(), sb - 1, None,
+                          None) == _lib.HN_ESCRATCH
+    assert L.hn_build_u64(k.data_ptr(), None, b.data_ptr(), inf.data_ptr(), 0, 10, sc.data_ptr(), sb, None,
+                          None) == 0

@@ -55,7 +55,7 @@ def test_retrieve_exact_on_oracle_image(gpu, bits, width):
     vals = rng.integers(0, 2**31, keys.size).astype(NDT[bits])
     model.insert(keys, vals)
     model.erase(keys[::7])  # tombstones on the probe paths
-    hm = Hashmap(sp, bits, tune=hn_tune(width, 256, 0, 0))
+    hm = Hashmap(sp, bits, tune=hn_tune(width, 256, 0, 0), auto_build=0)
     upload(hm, model)
     absent = keyset(keys.size, bits, seed=bits + width, start=keys.size + 2)
     q = np.concatenate([keys, absent])
@@ -82,7 +82,7 @@ def test_insert_set_and_occupancy(gpu, bits, width, lf):
     sp = 14
     keys = keyset(int(lf * (1 << sp)), bits, seed=int(lf * 100) + width)
     vals = (keys * NDT[bits](3) + NDT[bits](1)).astype(NDT[bits])
-    hm = Hashmap(sp, bits, tune=hn_tune(width, 256, 0, 0), auto_reorder=0)
+    hm = Hashmap(sp, bits, tune=hn_tune(width, 256, 0, 0), auto_reorder=0, auto_build=0)
     hm.insert(keys, vals, targetLF=1.0)
     model = fresh_model(sp, bits)
     model.insert(keys, vals)
@@ -101,7 +101,7 @@ def test_insert_duplicates_and_overwrite(gpu, bits):
     dup = np.concatenate([keys, keys[:700], keys[:300]])
     rng.shuffle(dup)
     vals = (dup ^ NDT[bits](0x5A5A)).astype(NDT[bits])  # value a function of the key
-    hm = Hashmap(sp, bits)
+    hm = Hashmap(sp, bits, auto_build=0)
     hm.insert(dup, vals, targetLF=1.0)
     model = fresh_model(sp, bits)
     model.insert(dup, vals)
@@ -118,13 +118,13 @@ def test_insert_duplicates_and_overwrite(gpu, bits):
 def test_insert_index_and_pairs(gpu, bits):
     sp = 11
     keys = keyset(1200, bits, seed=3)
-    hm = Hashmap(sp, bits)
+    hm = Hashmap(sp, bits, auto_build=0)
     hm.insertIndex(keys, targetLF=1.0)
     model = fresh_model(sp, bits)
     model.insert_index(keys)
     check_insert_invariants(hm, model, bits)
     pairs = np.stack([keyset(500, bits, seed=4), np.arange(500, dtype=NDT[bits])], axis=1)
-    hm2 = Hashmap(sp, bits)
+    hm2 = Hashmap(sp, bits, auto_build=0)
     hm2.insert(pairs, None, targetLF=1.0)
     m2 = fresh_model(sp, bits)
     m2.insert_pairs(pairs.reshape(-1))
@@ -142,7 +142,7 @@ def test_insert_into_tombstoned_map(gpu, bits):
     sp = 12
     keys = keyset(3000, bits, seed=21)
     vals = keys.astype(NDT[bits])
-    hm = Hashmap(sp, bits)
+    hm = Hashmap(sp, bits, auto_build=0)
     hm.insert(keys, vals, targetLF=1.0)
     hm.erase(keys[::3])
     img = hm.host_buckets()
@@ -161,7 +161,7 @@ def test_insert_into_tombstoned_map(gpu, bits):
 def test_erase_exact(gpu, bits, width):
     sp = 13
     keys = keyset(int(0.85 * (1 << sp)), bits, seed=31 + width)
-    hm = Hashmap(sp, bits, tune=hn_tune(width, 256, 0, 0))
+    hm = Hashmap(sp, bits, tune=hn_tune(width, 256, 0, 0), auto_build=0)
     hm.insert(keys, keys, targetLF=1.0)
     model = model_from(hm.host_buckets(), info_tuple(hm), bits)
     rng = np.random.default_rng(width)
@@ -177,7 +177,7 @@ def test_erase_exact(gpu, bits, width):
 def test_reset_all_exact(gpu, bits):
     sp = 10
     keys = keyset(700, bits, seed=41)
-    hm = Hashmap(sp, bits)
+    hm = Hashmap(sp, bits, auto_build=0)
     hm.insert(keys, keys, targetLF=1.0)
     hm.erase(keys[:100])
     model = model_from(hm.host_buckets(), info_tuple(hm), bits)
@@ -193,7 +193,7 @@ def test_clean_tombstones(gpu, bits, budget):
     sp = 14
     keys = keyset(int(0.9 * (1 << sp)), bits, seed=51)
     vals = (keys ^ NDT[bits](77)).astype(NDT[bits])
-    hm = Hashmap(sp, bits, auto_reorder=0)  # the overflow check below is on clean's own image
+    hm = Hashmap(sp, bits, auto_reorder=0, auto_build=0)  # the overflow check below is on clean's own image
     hm.insert(keys, vals, targetLF=1.0)
     rng = np.random.default_rng(budget)
     er = rng.choice(keys, keys.size // 4, replace=False)
@@ -218,7 +218,7 @@ def test_clean_tombstones(gpu, bits, budget):
 
 @pytest.mark.parametrize("bits", [32, 64])
 def test_clean_no_tombstones_is_noop(gpu, bits):
-    hm = Hashmap(10, bits)
+    hm = Hashmap(10, bits, auto_build=0)
     keys = keyset(600, bits, seed=61)
     hm.insert(keys, keys, targetLF=1.0)
     before = hm.host_buckets().copy()
@@ -234,7 +234,7 @@ def test_device_rehash(gpu, bits, new_sp):
     sp = 13
     keys = keyset(int(0.7 * (1 << sp)), bits, seed=71)
     vals = (keys * NDT[bits](5)).astype(NDT[bits])
-    hm = Hashmap(sp, bits)
+    hm = Hashmap(sp, bits, auto_build=0)
     hm.insert(keys, vals, targetLF=1.0)
     hm.erase(keys[:500])
     live = live_pairs(hm.host_buckets(), bits)
@@ -254,7 +254,7 @@ def test_device_rehash(gpu, bits, new_sp):
 def test_insert_overflow_sets_fail(gpu, bits):
     sp = 6
     keys = keyset(80, bits, seed=81)  # more keys than slots
-    hm = Hashmap(sp, bits)
+    hm = Hashmap(sp, bits, auto_build=0)
     hm.insert(keys, keys, targetLF=100.0)  # no auto-grow
     sp_, fill, ovf, tomb, err = info_tuple(hm)
     assert fill == 64 and err == 1
@@ -274,7 +274,7 @@ def test_auto_grow_on_insert(gpu, bits):
 @pytest.mark.parametrize("bits", [32, 64])
 def test_tiny_tables(gpu, bits):
     for sp in (1, 2, 3):
-        hm = Hashmap(sp, bits, tune=hn_tune(64, 256, 0, 0))
+        hm = Hashmap(sp, bits, tune=hn_tune(64, 256, 0, 0), auto_build=0)
         keys = keyset((1 << sp) - 1, bits, seed=sp)
         hm.insert(keys, keys, targetLF=100.0)
         _, v, h = hm.retrieve(keys)
@@ -284,7 +284,7 @@ def test_tiny_tables(gpu, bits):
 
 @pytest.mark.parametrize("bits", [32, 64])
 def test_empty_batches(gpu, bits):
-    hm = Hashmap(8, bits)
+    hm = Hashmap(8, bits, auto_build=0)
     hm.insert(np.zeros(0, NDT[bits]), np.zeros(0, NDT[bits]))
     assert hm.peek_status() == 0
     _, v, h = hm.retrieve(np.zeros(0, NDT[bits]))

@@ -20,7 +20,7 @@ pytestmark = pytest.mark.gpu
 
 
 def build(bits, sp, keys, vals, width=0):
-    hm = Hashmap(sp, bits, tune=hn_tune(width, 256, 0, 0) if width else None, auto_reorder=0)
+    hm = Hashmap(sp, bits, tune=hn_tune(width, 256, 0, 0) if width else None, auto_reorder=0, auto_build=0)
     hm.insert(keys, vals, targetLF=1.0)
     model = O.GpuModel(sp, bits)
     model.insert(keys, vals)
@@ -181,7 +181,7 @@ def test_reorder_full_table_is_noop(gpu, bits):
 def test_reorder_invalid_args(gpu):
     L = _lib.load()
     assert L.hn_reorder_u64(None, None, 10, None, 0, None) == _lib.HN_EINVAL
-    hm = Hashmap(10, 64, auto_reorder=0)
+    hm = Hashmap(10, 64, auto_reorder=0, auto_build=0)
     sb = L.hn_reorder_scratch_bytes(10)
     s = torch.empty(sb, dtype=torch.uint8, device="cuda")
     assert L.hn_reorder_u64(hm.buckets.data_ptr(), hm._info.data_ptr(), 10, s.data_ptr(), sb - 1, None) == \
