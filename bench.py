@@ -272,6 +272,29 @@ def main():
         "overflow": int(inf[2]),
     }
 
+    # Random-access ceiling of this device, measured now on the table's own
+    # 4 GiB: independent random reads of 64 B sectors and 128 B lines
+    # (hn_calib_random_gather), 4 GiB of reads each, best of 3.
+    sink = torch.zeros(1, dtype=torch.int64, device=dev)
+    ceil = {}
+    for gran in (64, 128):
+        best = None
+        for rep in range(3):
+            e0, e1 = ev(), ev()
+            e0.record(stream)
+            _lib.check(L.hn_calib_random_gather(P(buckets), N * 16, gran, (N * 16) // gran, 1, 1234 + rep, P(sink), S),
+                       "calib")
+            e1.record(stream)
+            torch.cuda.synchronize()
+            t = e0.elapsed_time(e1)
+            best = t if best is None else min(best, t)
+        ceil[gran] = N * 16 / (best * 1e-3) / 1e9
+    roofline["random_ceiling_GBps"] = {"64B": round(ceil[64], 1), "128B": round(ceil[128], 1)}
+    roofline["frac_of_random_64B_ceiling"] = round(achieved / ceil[64], 4)
+    if traffic:
+        roofline["traffic_GBps"] = round(traffic / (t_ret * 1e-3) / 1e9, 1)
+        roofline["traffic_frac_of_random_128B_ceiling"] = round(traffic / (t_ret * 1e-3) / 1e9 / ceil[128], 4)
+
     if args.calib_copy:
         src = torch.ones(1 << 27, dtype=torch.int64, device=dev)
         dst = torch.empty_like(src)

@@ -18,17 +18,22 @@
 //      occurrence of a repeated key (the value a serial insert leaves), and
 //      summarise the region as the carry function g(c) = max(a, c + b), where
 //      c is the number of keys spilling into the region from before it;
-//   4. one scan composes the carry functions (g2 after g1 is
-//      (max(a2, a1 + b2), b1 + b2)); the carry into region 0 is the least
-//      fixed point of the whole-table composition, max(A, c + B) with
-//      B = n - N < 0, i.e. c0 = A;
-//   5. per region, a max-scan places its keys and the region writes its slot
-//      range (keys and EMPTY slots) once, coalesced.
+//   4. a workgroup walks 16 consecutive regions, handing the carry from one
+//      to the next (carry out = max(a, c + b); the walk starts from the
+//      region before it, taken with no carry in), places each region's keys
+//      with a max-scan and writes its slot range (keys and EMPTY slots)
+//      once, coalesced;
+//   5. one scan composes the carry functions (g2 after g1 is
+//      (max(a2, a1 + b2), b1 + b2)) into the true carries (the carry into
+//      region 0 is the least fixed point of the whole-table composition,
+//      max(A, c + B) with B = n - N < 0, i.e. c0 = A), and the regions whose
+//      carry was wrong, or whose range wraps past the table's end, are
+//      placed again.
 //
 // Traffic per key: the partition passes read and write the 16 (u64) or 8
-// (u32) byte pair twice each, plus one read for the boundaries, one read and
-// write for the region sort, one read for the placement; the table is
-// written once.  No atomics on the table, no probing.
+// (u32) byte pair twice each, plus one read for the boundaries and one for
+// the region step; the table is written once.  No atomics on the table, no
+// probing.
 //
 // The table's memory is the output of the first partition pass when there
 // are two, so the scratch holds one pair array, not two.  A region holding
@@ -63,7 +68,8 @@ struct BuildHeader {
    uint32_t pad0;
    int64_t c0;         // carry into region 0
    uint64_t n_distinct;
-   uint32_t pad[56];
+   uint32_t n_fix;     // regions placed again after the carries are known
+   uint32_t pad[55];
 };
 static_assert(sizeof(BuildHeader) == 256, "header is one 256-byte line");
 
@@ -82,7 +88,7 @@ struct BuildPlan {
    int passes;          // partition passes
    int dbits[3];        // digit bits per pass (low digit first)
    uint32_t nblocks;    // partition tiles
-   size_t off_pairs, off_mat, off_scan, off_start, off_sum, off_carry, total;
+   size_t off_pairs, off_mat, off_scan, off_start, off_word, off_spec, off_carry, off_mark, off_list, off_blk, total;
 };
 
 inline size_t align256(size_t x) { return (x + 255) / 256 * 256; }
@@ -121,10 +127,18 @@ bool build_plan(int sp, uint64_t n, int dcap, BuildPlan* P) {
    o = align256(o + hn_compact_scratch_bytes(mat));
    P->off_start = o;
    o = align256(o + (P->NR + 1) * sizeof(uint64_t));
-   P->off_sum = o;
-   o = align256(o + P->NR * sizeof(RegionSum));
+   P->off_word = o;
+   o = align256(o + P->NR * sizeof(uint64_t));
+   P->off_spec = o;
+   o = align256(o + P->NR * sizeof(int64_t));
    P->off_carry = o;
    o = align256(o + P->NR * sizeof(int64_t));
+   P->off_mark = o;
+   o = align256(o + P->NR * sizeof(uint32_t));
+   P->off_list = o;
+   o = align256(o + P->NR * sizeof(uint32_t));
+   P->off_blk = o;
+   o = align256(o + ((P->NR + 1023) / 1024) * sizeof(RegionSum));
    P->total = o;
    return true;
 }
@@ -352,238 +366,454 @@ __device__ uint32_t lds_exclusive_scan(uint32_t* x, uint32_t m, uint32_t* wsum) 
    return total;
 }
 
-// Region r: sort its keys by (home, key), keep the last occurrence of a
-// repeated key, write the survivors back over the region's own range in
-// sorted order, and record the carry function.
-template <typename K>
-__global__ __launch_bounds__(kRegionBS) void k_region_sort(Slot<K>* S, const uint64_t* start, RegionSum* sum, int sp,
-                                                           int rlog, BuildHeader* h) {
-   constexpr int NW = kRegionBS / kWave;
-   constexpr uint32_t RMAX = 1u << kRegionLog;
-   __shared__ K key_s[kRegionCap];
-   __shared__ uint16_t order_s[kRegionCap];
-   __shared__ uint8_t dup_s[kRegionCap];
-   __shared__ uint32_t cnt_s[RMAX + 1];
-   __shared__ uint32_t scnt_s[RMAX + 1];
-   __shared__ uint32_t wsum_s[NW];
-   __shared__ int32_t wmax_s[NW];
-   if (!h->go) return;
-   const uint32_t r = blockIdx.x;
-   const uint64_t lo = start[r], hi = start[r + 1];
-   const uint64_t R = 1ull << rlog;
-   if (hi - lo > static_cast<uint64_t>(kRegionCap)) {
-      if (threadIdx.x == 0) __hip_atomic_store(&h->fail, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      return;
+// Carry functions g(c) = max(a, c + b): `first` applied, then `then`.
+constexpr int64_t kNegInf = INT64_MIN / 4; // identity: max(-inf, c + 0) = c
+__host__ __device__ __forceinline__ RegionSum compose(RegionSum first, RegionSum then) {
+   RegionSum g;
+   g.a = first.a + then.b > then.a ? first.a + then.b : then.a;
+   g.b = first.b + then.b;
+   return g;
+}
+
+// Carry bookkeeping over regions: each region's carry function, the carry
+// it was placed with (spec), the true carry, and the regions to place again.
+struct RegionLB {
+   uint64_t* word;  // [NR] the region's carry function, packed
+   int64_t* spec;   // [NR] carry the region was placed with
+   int64_t* carry;  // [NR] true carry
+   uint32_t* mark;  // [NR] 1: place again
+   uint32_t* list;  // [NR] regions to place again (count in the header)
+};
+
+// A region's carry function in one 64-bit word: a <= keys in the region
+// < 2^24; b + R in [0, 2^24).
+__device__ __forceinline__ uint64_t pack_sum(RegionSum g, int64_t R) {
+   return (1ull << 63) | (static_cast<uint64_t>(g.b + R) << 24) | static_cast<uint64_t>(g.a);
+}
+__device__ __forceinline__ RegionSum unpack_sum(uint64_t w, int64_t R) {
+   RegionSum g;
+   g.a = static_cast<int64_t>(w & 0xFFFFFFull);
+   g.b = static_cast<int64_t>((w >> 24) & 0xFFFFFFull) - R;
+   return g;
+}
+__device__ __forceinline__ RegionSum load_sum(const RegionLB& lb, uint64_t r, int64_t R) {
+   return unpack_sum(lb.word[r], R);
+}
+
+__global__ __launch_bounds__(256) void k_region_lb_init(RegionLB lb, uint64_t NR, BuildHeader* h) {
+   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < NR; i += stride) {
+      lb.word[i] = 0;
+      lb.mark[i] = 0;
    }
-   const uint32_t m = static_cast<uint32_t>(hi - lo);
+   if (blockIdx.x == 0 && threadIdx.x == 0) {
+      h->n_fix = 0;
+   }
+}
+
+constexpr int kRegionNW = kRegionBS / kWave;
+constexpr uint32_t kRegionMax = 1u << kRegionLog;
+constexpr uint32_t kSpanMax = kRegionMax + kRegionCap; // slots one region writes at most
+constexpr uint32_t kSpanWords = (kSpanMax + 31) / 32;
+
+// LDS of one region: the sort tables are dead once each key knows its sorted
+// index, so the placement arrays reuse their space.
+template <typename K>
+struct RegionLds {
+   K key[kRegionCap];
+   uint16_t order[kRegionCap];
+   union {
+      uint8_t dup[kRegionCap];
+      uint32_t occ[kSpanWords]; // slots of the write span holding a key
+   } u1;
+   union {
+      struct {
+         uint32_t cnt[kRegionMax + 1];  // keys per home offset, then their exclusive starts
+         uint32_t scnt[kRegionMax + 1]; // the same over the surviving keys
+      } c;
+      int32_t v[kRegionCap]; // by sorted index f: home offset - f, then its running max
+   } u2;
+   uint32_t wsum[kRegionNW];
+   int32_t wmax[kRegionNW];
+   uint32_t r;
+   int64_t carry;
+};
+static_assert(sizeof(uint32_t) * kSpanWords <= kRegionCap, "occupancy bits fit the dup table");
+static_assert(sizeof(int32_t) * kRegionCap <= 2 * sizeof(uint32_t) * (kRegionMax + 1), "v fits the count tables");
+
+// Sort region keys S[lo, lo + m) in LDS by (home, key), keeping the last
+// occurrence of a repeated key.  Thread t owns keys t + k*kRegionBS; ff[k] =
+// its sorted index among the survivors, or -1.  Leaves v[f] = home offset - f.
+// Returns the number of survivors.  Called by the whole workgroup.
+template <typename K>
+__device__ uint32_t region_sort(const Slot<K>* S, uint64_t lo, uint32_t m, int sp, uint64_t R, RegionLds<K>& L,
+                                K* kk, K* vv, uint32_t* uu, int32_t* ff) {
+   __syncthreads(); // the previous user of L is done
    for (uint32_t i = threadIdx.x; i <= R; i += kRegionBS) {
-      cnt_s[i] = 0;
-      scnt_s[i] = 0;
+      L.u2.c.cnt[i] = 0;
+      L.u2.c.scnt[i] = 0;
    }
    __syncthreads();
-   K kk[kRegionIPT], vv[kRegionIPT];
-   uint32_t uu[kRegionIPT], sub[kRegionIPT];
+   uint32_t sub[kRegionIPT];
 #pragma unroll
    for (int k = 0; k < kRegionIPT; ++k) {
       const uint32_t e = threadIdx.x + k * kRegionBS;
+      ff[k] = -1;
       if (e < m) {
          const Slot<K> s = S[lo + e];
          kk[k] = s.key;
          vv[k] = s.val;
-         key_s[e] = s.key;
+         L.key[e] = s.key;
          uu[k] = static_cast<uint32_t>((fib_home(s.key, sp) & ((1ull << sp) - 1)) & (R - 1));
-         sub[k] = atomicAdd(&cnt_s[uu[k]], 1u);
+         sub[k] = atomicAdd(&L.u2.c.cnt[uu[k]], 1u);
       }
    }
    __syncthreads();
-   lds_exclusive_scan<kRegionBS>(cnt_s, static_cast<uint32_t>(R) + 1, wsum_s);
+   lds_exclusive_scan<kRegionBS>(L.u2.c.cnt, static_cast<uint32_t>(R) + 1, L.wsum);
 #pragma unroll
    for (int k = 0; k < kRegionIPT; ++k) {
       const uint32_t e = threadIdx.x + k * kRegionBS;
-      if (e < m) order_s[cnt_s[uu[k]] + sub[k]] = static_cast<uint16_t>(e);
+      if (e < m) L.order[L.u2.c.cnt[uu[k]] + sub[k]] = static_cast<uint16_t>(e);
    }
    __syncthreads();
-   // A pair is dropped when a later pair (higher index: later in the input)
-   // carries the same key.
+   // Dropped: a later key of the batch (higher index) is the same key.
 #pragma unroll
    for (int k = 0; k < kRegionIPT; ++k) {
       const uint32_t e = threadIdx.x + k * kRegionBS;
       if (e < m) {
          uint8_t dup = 0;
-         for (uint32_t j = cnt_s[uu[k]]; j < cnt_s[uu[k] + 1]; ++j) {
-            const uint32_t o = order_s[j];
-            dup |= (key_s[o] == kk[k] && o > e) ? 1 : 0;
+         for (uint32_t j = L.u2.c.cnt[uu[k]]; j < L.u2.c.cnt[uu[k] + 1]; ++j) {
+            const uint32_t o = L.order[j];
+            dup |= (L.key[o] == kk[k] && o > e) ? 1 : 0;
          }
-         dup_s[e] = dup;
-         if (!dup) atomicAdd(&scnt_s[uu[k]], 1u);
+         L.u1.dup[e] = dup;
+         if (!dup) atomicAdd(&L.u2.c.scnt[uu[k]], 1u);
       }
    }
    __syncthreads();
-   const uint32_t ms = lds_exclusive_scan<kRegionBS>(scnt_s, static_cast<uint32_t>(R) + 1, wsum_s);
-   int32_t M = INT32_MIN;
+   const uint32_t ms = lds_exclusive_scan<kRegionBS>(L.u2.c.scnt, static_cast<uint32_t>(R) + 1, L.wsum);
 #pragma unroll
    for (int k = 0; k < kRegionIPT; ++k) {
       const uint32_t e = threadIdx.x + k * kRegionBS;
-      if (e < m && !dup_s[e]) {
+      if (e < m && !L.u1.dup[e]) {
          uint32_t rank = 0;
-         for (uint32_t j = cnt_s[uu[k]]; j < cnt_s[uu[k] + 1]; ++j) {
-            const uint32_t o = order_s[j];
-            rank += (!dup_s[o] && key_s[o] < kk[k]) ? 1u : 0u;
+         for (uint32_t j = L.u2.c.cnt[uu[k]]; j < L.u2.c.cnt[uu[k] + 1]; ++j) {
+            const uint32_t o = L.order[j];
+            rank += (!L.u1.dup[o] && L.key[o] < kk[k]) ? 1u : 0u;
          }
-         const uint32_t f = scnt_s[uu[k]] + rank;
-         Slot<K> s;
-         s.key = kk[k];
-         s.val = vv[k];
-         S[lo + f] = s;
-         M = std::max<int32_t>(M, static_cast<int32_t>(uu[k]) - static_cast<int32_t>(f));
+         ff[k] = static_cast<int32_t>(L.u2.c.scnt[uu[k]] + rank);
       }
    }
-   M = wave_max(M);
-   if (__lane_id() == 0) wmax_s[threadIdx.x / kWave] = M;
+   __syncthreads(); // count tables dead from here; v takes their place
+#pragma unroll
+   for (int k = 0; k < kRegionIPT; ++k)
+      if (ff[k] >= 0) L.u2.v[ff[k]] = static_cast<int32_t>(uu[k]) - ff[k];
    __syncthreads();
-   if (threadIdx.x == 0) {
-      int32_t Mb = INT32_MIN;
-      for (int w = 0; w < NW; ++w) Mb = std::max(Mb, wmax_s[w]);
-      const int64_t mp = static_cast<int64_t>(ms), Ri = static_cast<int64_t>(R);
-      RegionSum g;
-      g.b = mp - Ri;
-      g.a = mp > 0 ? std::max<int64_t>(0, mp - Ri + Mb) : 0;
-      sum[r] = g;
-   }
+   return ms;
 }
 
-// Carry into every region.  One workgroup: each thread composes a chunk of
-// regions, the chunk functions are scanned in LDS, then applied from c0.
-constexpr int kCarryBS = 1024;
-__device__ __forceinline__ RegionSum compose(RegionSum first, RegionSum then) {
+// Carry function of a sorted region: carry out = max(0, ms - R + max(M, c)),
+// M = max_f (home offset - f).
+template <typename K>
+__device__ RegionSum region_summary(RegionLds<K>& L, uint32_t ms, uint64_t R, const int32_t* ff) {
+   int32_t M = INT32_MIN;
+#pragma unroll
+   for (int k = 0; k < kRegionIPT; ++k)
+      if (ff[k] >= 0) M = std::max(M, L.u2.v[ff[k]]);
+   M = wave_max(M);
+   if (__lane_id() == 0) L.wmax[threadIdx.x / kWave] = M;
+   __syncthreads();
+   int32_t Mb = INT32_MIN;
+   for (int w = 0; w < kRegionNW; ++w) Mb = std::max(Mb, L.wmax[w]);
+   __syncthreads();
    RegionSum g;
-   g.a = std::max(then.a, first.a + then.b);
-   g.b = first.b + then.b;
+   const int64_t mp = static_cast<int64_t>(ms), Ri = static_cast<int64_t>(R);
+   g.b = mp - Ri;
+   g.a = mp > 0 ? std::max<int64_t>(0, mp - Ri + Mb) : 0;
    return g;
 }
 
-__global__ __launch_bounds__(kCarryBS) void k_region_carry(const RegionSum* sum, int64_t* carry, uint64_t NR,
-                                                           uint64_t N, BuildHeader* h) {
-   __shared__ int64_t fa[kCarryBS], fb[kCarryBS];
-   if (!h->go || h->fail) return;
-   constexpr int64_t kNeg = INT64_MIN / 4; // identity: max(-inf, c + 0) = c
-   const uint64_t per = (NR + kCarryBS - 1) / kCarryBS;
-   const uint64_t c0 = std::min<uint64_t>(threadIdx.x * per, NR), c1 = std::min<uint64_t>(c0 + per, NR);
-   RegionSum g{kNeg, 0};
-   for (uint64_t r = c0; r < c1; ++r) g = compose(g, sum[r]);
-   fa[threadIdx.x] = g.a;
-   fb[threadIdx.x] = g.b;
-   __syncthreads();
-   for (int o = 1; o < kCarryBS; o <<= 1) {
-      RegionSum mine{fa[threadIdx.x], fb[threadIdx.x]};
-      RegionSum prev{kNeg, 0};
-      if (static_cast<int>(threadIdx.x) >= o) prev = RegionSum{fa[threadIdx.x - o], fb[threadIdx.x - o]};
-      __syncthreads();
-      if (static_cast<int>(threadIdx.x) >= o) {
-         const RegionSum c = compose(prev, mine);
-         fa[threadIdx.x] = c.a;
-         fb[threadIdx.x] = c.b;
-      }
-      __syncthreads();
-   }
-   // Whole table: max(A, c + B) with B = n' - N < 0; its fixed point is A.
-   const int64_t A = std::max<int64_t>(fa[kCarryBS - 1], 0);
-   int64_t c = A;
-   if (threadIdx.x > 0) c = std::max(fa[threadIdx.x - 1], A + fb[threadIdx.x - 1]);
-   for (uint64_t r = c0; r < c1; ++r) {
-      carry[r] = c;
-      c = std::max(sum[r].a, c + sum[r].b);
-   }
-   if (threadIdx.x == 0) {
-      h->c0 = A;
-      h->n_distinct = static_cast<uint64_t>(fb[kCarryBS - 1] + static_cast<int64_t>(N));
-   }
-}
-
-// Region r places its sorted keys after the carry and writes its slot range
-// [rR + carry, max((r+1)R, last position + 1)) once: keys and EMPTY slots.
+// Place a sorted region after carry c: key f at p = f + max(running max of v, c),
+// and write the span [c, max(R, last p + 1)) of slots (keys and EMPTY) at
+// base + offset.  `keep` picks by unwrapped position: 0 = below N only, 1 =
+// all, 2 = N and above only.  Returns this thread's longest probe.
 template <typename K>
-__global__ __launch_bounds__(kRegionBS) void k_region_place(const Slot<K>* S, const uint64_t* start,
-                                                            const RegionSum* sum, const int64_t* carry, Slot<K>* b,
-                                                            int sp, int rlog, BuildHeader* h) {
-   constexpr int NW = kRegionBS / kWave;
-   constexpr uint32_t SPAN = (1u << kRegionLog) + kRegionCap;
-   __shared__ Slot<K> el_s[kRegionCap];
-   __shared__ int32_t v_s[kRegionCap];
-   __shared__ uint16_t at_s[SPAN];
-   __shared__ int32_t wmax_s[NW];
-   __shared__ uint32_t wprobe_s[NW];
-   if (!h->go || h->fail) return;
-   const uint32_t r = blockIdx.x;
+__device__ uint32_t region_place(RegionLds<K>& L, uint32_t ms, int64_t c, uint64_t base, int sp, uint64_t R,
+                                 int keep, const K* kk, const K* vv, const uint32_t* uu, const int32_t* ff,
+                                 Slot<K>* b) {
    const int lane = __lane_id(), wave = threadIdx.x / kWave;
-   const uint64_t N = 1ull << sp, mask = N - 1, R = 1ull << rlog;
-   const uint64_t lo = start[r];
-   const int64_t ms = sum[r].b + static_cast<int64_t>(R);
-   const int64_t c = carry[r];
-   const uint32_t m = static_cast<uint32_t>(ms);
-   for (uint32_t j = threadIdx.x; j < m; j += kRegionBS) {
-      const Slot<K> s = S[lo + j];
-      el_s[j] = s;
-      v_s[j] = static_cast<int32_t>((fib_home(s.key, sp) & mask) & (R - 1)) - static_cast<int32_t>(j);
-   }
-   __syncthreads();
-   // Inclusive max-scan of v over j (chunks per thread, then across lanes and waves).
+   const uint64_t N = 1ull << sp, mask = N - 1;
    {
-      const uint32_t per = (m + kRegionBS - 1) / kRegionBS;
-      const uint32_t j0 = std::min<uint32_t>(threadIdx.x * per, m), j1 = std::min<uint32_t>(j0 + per, m);
+      const uint32_t per = (ms + kRegionBS - 1) / kRegionBS;
+      const uint32_t j0 = std::min<uint32_t>(threadIdx.x * per, ms), j1 = std::min<uint32_t>(j0 + per, ms);
       int32_t acc = INT32_MIN;
-      for (uint32_t j = j0; j < j1; ++j) acc = std::max(acc, v_s[j]);
+      for (uint32_t j = j0; j < j1; ++j) acc = std::max(acc, L.u2.v[j]);
       int32_t inc = acc;
 #pragma unroll
       for (int o = 1; o < kWave; o <<= 1) {
          const int32_t y = __shfl_up(inc, o, kWave);
          if (lane >= o) inc = std::max(inc, y);
       }
-      if (lane == kWave - 1) wmax_s[wave] = inc;
+      if (lane == kWave - 1) L.wmax[wave] = inc;
       __syncthreads();
       int32_t run = __shfl_up(inc, 1, kWave);
       if (lane == 0) run = INT32_MIN;
-      for (int w = 0; w < wave; ++w) run = std::max(run, wmax_s[w]);
+      for (int w = 0; w < wave; ++w) run = std::max(run, L.wmax[w]);
       for (uint32_t j = j0; j < j1; ++j) {
-         run = std::max(run, v_s[j]);
-         v_s[j] = run;
+         run = std::max(run, L.u2.v[j]);
+         L.u2.v[j] = run;
       }
    }
    __syncthreads();
-   const int64_t last = m ? static_cast<int64_t>(m - 1) + std::max<int64_t>(v_s[m - 1], c) : -1;
+   const int64_t last = ms ? static_cast<int64_t>(ms - 1) + std::max<int64_t>(L.u2.v[ms - 1], c) : -1;
    const int64_t end = std::max<int64_t>(static_cast<int64_t>(R), last + 1);
    const int64_t span = end - c;
-   for (int64_t q = threadIdx.x; q < span; q += kRegionBS) at_s[q] = 0xFFFFu;
+   const int64_t words = span > 0 ? (span + 31) / 32 : 0;
+   for (int64_t w = threadIdx.x; w < words; w += kRegionBS) L.u1.occ[w] = 0;
    __syncthreads();
+   auto kept = [&](uint64_t pos) { return keep == 1 || (keep == 0 ? pos < N : pos >= N); };
    uint32_t probe = 0;
-   for (uint32_t j = threadIdx.x; j < m; j += kRegionBS) {
-      const int64_t p = static_cast<int64_t>(j) + std::max<int64_t>(v_s[j], c);
-      at_s[p - c] = static_cast<uint16_t>(j);
-      const int64_t u = static_cast<int64_t>((fib_home(el_s[j].key, sp) & mask) & (R - 1));
-      probe = std::max<uint32_t>(probe, static_cast<uint32_t>(p - u + 1));
+#pragma unroll
+   for (int k = 0; k < kRegionIPT; ++k) {
+      if (ff[k] >= 0) {
+         const int64_t p = ff[k] + std::max<int64_t>(L.u2.v[ff[k]], c);
+         const int64_t q = p - c;
+         atomicOr(&L.u1.occ[q >> 5], 1u << (q & 31));
+         const uint64_t pos = base + static_cast<uint64_t>(p);
+         if (kept(pos)) {
+            Slot<K> s;
+            s.key = kk[k];
+            s.val = vv[k];
+            b[pos & mask] = s;
+         }
+         probe = std::max<uint32_t>(probe, static_cast<uint32_t>(p - static_cast<int64_t>(uu[k]) + 1));
+      }
    }
    __syncthreads();
-   const uint64_t base = static_cast<uint64_t>(r) * R;
    for (int64_t q = threadIdx.x; q < span; q += kRegionBS) {
-      const uint16_t j = at_s[q];
-      Slot<K> s;
-      if (j != 0xFFFFu) {
-         s = el_s[j];
-      } else {
-         s.key = empty_key<K>();
-         s.val = 0;
+      if (!((L.u1.occ[q >> 5] >> (q & 31)) & 1u)) {
+         const uint64_t pos = base + static_cast<uint64_t>(c + q);
+         if (kept(pos)) {
+            Slot<K> s;
+            s.key = empty_key<K>();
+            s.val = 0;
+            b[pos & mask] = s;
+         }
       }
-      b[(base + static_cast<uint64_t>(c + q)) & mask] = s;
    }
+   return probe;
+}
+
+template <typename K>
+__device__ void probe_to_header(RegionLds<K>& L, uint32_t probe, BuildHeader* h) {
    probe = wave_max(probe);
-   if (lane == 0) wprobe_s[wave] = probe;
+   __syncthreads();
+   if (__lane_id() == 0) L.wmax[threadIdx.x / kWave] = static_cast<int32_t>(probe);
    __syncthreads();
    if (threadIdx.x == 0) {
       uint32_t pb = 0;
-      for (int w = 0; w < NW; ++w) pb = std::max(pb, wprobe_s[w]);
+      for (int w = 0; w < kRegionNW; ++w) pb = std::max(pb, static_cast<uint32_t>(L.wmax[w]));
       if (pb > __hip_atomic_load(&h->max_probe, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
          atomic_max_relaxed(&h->max_probe, pb);
    }
+}
+
+// One workgroup walks kRegionChunk consecutive regions in order: sort a
+// region, record its carry function, place it with the carry handed on from
+// the region before, and hand on max(a, c + b).  The walk starts from the
+// carry out of the region before the chunk taken with no carry in (that
+// region is sorted once more for its summary).  Slots at unwrapped positions
+// >= N are skipped.  Nothing waits on another workgroup; the carry kernels
+// finds the regions whose carry was wrong (a spill crossing a whole region
+// into a chunk, or the wrap past the table's end) and k_region_fix places
+// them again.
+constexpr uint32_t kRegionChunk = 16;
+
+template <typename K>
+__global__ __launch_bounds__(kRegionBS) void k_region_build(const Slot<K>* S, const uint64_t* start, RegionLB lb,
+                                                            Slot<K>* b, int sp, int rlog, uint64_t NR,
+                                                            BuildHeader* h) {
+   __shared__ RegionLds<K> L;
+   if (!h->go) return;
+   const uint64_t R = 1ull << rlog;
+   const int64_t Ri = static_cast<int64_t>(R);
+   const uint64_t r0 = static_cast<uint64_t>(blockIdx.x) * kRegionChunk;
+   const uint64_t r1 = std::min<uint64_t>(r0 + kRegionChunk, NR);
+   K kk[kRegionIPT], vv[kRegionIPT];
+   uint32_t uu[kRegionIPT];
+   int32_t ff[kRegionIPT];
+   uint32_t probe = 0;
+   int64_t c = 0;
+   if (r0 > 0) {
+      // Seed the walk with the carry out of the region before the chunk, that
+      // region taken with no carry in: exact unless a spill crosses it whole.
+      const uint64_t lo = start[r0 - 1], m = start[r0] - lo;
+      if (m <= static_cast<uint64_t>(kRegionCap)) {
+         const uint32_t ms = region_sort<K>(S, lo, static_cast<uint32_t>(m), sp, R, L, kk, vv, uu, ff);
+         const RegionSum g = region_summary<K>(L, ms, R, ff);
+         c = std::max(g.a, g.b);
+      }
+   }
+   for (uint64_t r = r0; r < r1; ++r) {
+      const uint64_t lo = start[r], m = start[r + 1] - lo;
+      if (m > static_cast<uint64_t>(kRegionCap)) { // abandon the build
+         if (threadIdx.x == 0) __hip_atomic_store(&h->fail, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+         return;
+      }
+      const uint32_t ms = region_sort<K>(S, lo, static_cast<uint32_t>(m), sp, R, L, kk, vv, uu, ff);
+      const RegionSum g = region_summary<K>(L, ms, R, ff);
+      if (threadIdx.x == 0) {
+         lb.word[r] = pack_sum(g, Ri);
+         lb.spec[r] = c;
+      }
+      probe = std::max(probe, region_place<K>(L, ms, c, r * R, sp, R, 0, kk, vv, uu, ff, b));
+      c = std::max(g.a, c + g.b);
+   }
+   probe_to_header<K>(L, probe, h);
+}
+
+// True carries.  The carry into region 0 is the least fixed point of the
+// whole-table composition max(A, c + B), B = n - N < 0: c0 = A; region r's
+// carry is the composition over regions 0..r-1 applied to c0.  Three small
+// kernels: per block of kCarryTile regions the composition (k_carry_local),
+// the blocks' prefixes and c0 (k_carry_top, one workgroup), each region's
+// carry (k_carry_apply); then k_carry_mark lists the regions to place again.
+constexpr int kCarryBS = 256;
+constexpr int kCarryPer = 4;
+constexpr uint64_t kCarryTile = kCarryBS * kCarryPer;
+
+// Block-wide inclusive scan (composition) of one RegionSum per thread, in LDS.
+__device__ RegionSum carry_block_scan(RegionSum mine, int64_t* fa, int64_t* fb) {
+   fa[threadIdx.x] = mine.a;
+   fb[threadIdx.x] = mine.b;
+   __syncthreads();
+   for (int o = 1; o < static_cast<int>(blockDim.x); o <<= 1) {
+      const RegionSum cur{fa[threadIdx.x], fb[threadIdx.x]};
+      RegionSum prev{kNegInf, 0};
+      if (static_cast<int>(threadIdx.x) >= o) prev = RegionSum{fa[threadIdx.x - o], fb[threadIdx.x - o]};
+      __syncthreads();
+      if (static_cast<int>(threadIdx.x) >= o) {
+         const RegionSum c = compose(prev, cur);
+         fa[threadIdx.x] = c.a;
+         fb[threadIdx.x] = c.b;
+      }
+      __syncthreads();
+   }
+   return RegionSum{fa[threadIdx.x], fb[threadIdx.x]};
+}
+
+__global__ __launch_bounds__(kCarryBS) void k_carry_local(RegionLB lb, RegionSum* blk, uint64_t NR, int64_t R,
+                                                           const BuildHeader* h) {
+   __shared__ int64_t fa[kCarryBS], fb[kCarryBS];
+   if (!h->go || h->fail) return;
+   const uint64_t r0 = blockIdx.x * kCarryTile + threadIdx.x * kCarryPer;
+   RegionSum g{kNegInf, 0};
+   for (uint64_t r = r0; r < r0 + kCarryPer && r < NR; ++r) g = compose(g, load_sum(lb, r, R));
+   const RegionSum inc = carry_block_scan(g, fa, fb);
+   if (threadIdx.x == kCarryBS - 1) blk[blockIdx.x] = inc;
+}
+
+// One workgroup: blk[i] becomes the carry into block i (as RegionSum{c, 0}).
+__global__ __launch_bounds__(1024) void k_carry_top(RegionSum* blk, uint64_t nblk, uint64_t N, BuildHeader* h) {
+   __shared__ int64_t fa[1024], fb[1024];
+   if (!h->go || h->fail) return;
+   const uint64_t per = (nblk + 1023) / 1024;
+   const uint64_t b0 = std::min<uint64_t>(threadIdx.x * per, nblk), b1 = std::min<uint64_t>(b0 + per, nblk);
+   RegionSum g{kNegInf, 0};
+   for (uint64_t i = b0; i < b1; ++i) g = compose(g, blk[i]);
+   const RegionSum inc = carry_block_scan(g, fa, fb);
+   const RegionSum T{fa[1023], fb[1023]};
+   const int64_t c0 = std::max<int64_t>(T.a, 0);
+   RegionSum P{kNegInf, 0}; // composition over the blocks before b0
+   if (threadIdx.x > 0) P = RegionSum{fa[threadIdx.x - 1], fb[threadIdx.x - 1]};
+   (void)inc;
+   __syncthreads();
+   int64_t c = std::max(P.a, c0 + P.b);
+   for (uint64_t i = b0; i < b1; ++i) {
+      const RegionSum gi = blk[i];
+      blk[i] = RegionSum{c, 0};
+      c = std::max(gi.a, c + gi.b);
+   }
+   if (threadIdx.x == 0) {
+      h->c0 = c0;
+      h->n_distinct = static_cast<uint64_t>(T.b + static_cast<int64_t>(N));
+   }
+}
+
+__global__ __launch_bounds__(kCarryBS) void k_carry_apply(RegionLB lb, const RegionSum* blk, uint64_t NR, int64_t R,
+                                                           const BuildHeader* h) {
+   __shared__ int64_t fa[kCarryBS], fb[kCarryBS];
+   if (!h->go || h->fail) return;
+   const uint64_t r0 = blockIdx.x * kCarryTile + threadIdx.x * kCarryPer;
+   RegionSum gs[kCarryPer];
+   RegionSum g{kNegInf, 0};
+#pragma unroll
+   for (int k = 0; k < kCarryPer; ++k) {
+      gs[k] = r0 + k < NR ? load_sum(lb, r0 + k, R) : RegionSum{kNegInf, 0};
+      g = compose(g, gs[k]);
+   }
+   carry_block_scan(g, fa, fb);
+   RegionSum P{kNegInf, 0};
+   if (threadIdx.x > 0) P = RegionSum{fa[threadIdx.x - 1], fb[threadIdx.x - 1]};
+   const int64_t cin = blk[blockIdx.x].a;
+   int64_t c = std::max(P.a, cin + P.b);
+#pragma unroll
+   for (int k = 0; k < kCarryPer; ++k) {
+      if (r0 + k < NR) {
+         lb.carry[r0 + k] = c;
+         c = std::max(gs[k].a, c + gs[k].b);
+      }
+   }
+}
+
+// Placed again: every region whose spec carry differs, the regions whose
+// true slot ranges cover the slots such a region wrote wrongly
+// ([rR + spec, rR + carry)), and the regions whose range reaches past N
+// (those slots were skipped).
+__global__ __launch_bounds__(256) void k_carry_mark(RegionLB lb, uint64_t NR, int64_t R, int64_t N, BuildHeader* h) {
+   if (!h->go || h->fail) return;
+   const int64_t c0 = h->c0;
+   auto start_of = [&](uint64_t r) { return static_cast<int64_t>(r) * R + lb.carry[r]; };
+   auto mark = [&](uint64_t r) {
+      if (atomicExch(&lb.mark[r], 1u) == 0) lb.list[atomicAdd(&h->n_fix, 1u)] = static_cast<uint32_t>(r);
+   };
+   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+   for (uint64_t r = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; r < NR; r += stride) {
+      const int64_t cr = lb.carry[r], sr = lb.spec[r];
+      const int64_t next = r + 1 < NR ? start_of(r + 1) : N + c0; // end of r's true range
+      if (next > N) mark(r);                                      // reaches past N
+      if (cr != sr) {
+         mark(r);
+         const int64_t lo = static_cast<int64_t>(r) * R + sr;
+         for (int64_t q = static_cast<int64_t>(r) - 1; q >= 0; --q) {
+            mark(static_cast<uint64_t>(q));
+            if (start_of(static_cast<uint64_t>(q)) <= lo) break;
+         }
+      }
+   }
+}
+
+// Place the listed regions again with their true carries, all slots of
+// their ranges (wrapping past N).
+template <typename K>
+__global__ __launch_bounds__(kRegionBS) void k_region_fix(const Slot<K>* S, const uint64_t* start, RegionLB lb,
+                                                         Slot<K>* b, int sp, int rlog, BuildHeader* h) {
+   __shared__ RegionLds<K> L;
+   if (!h->go || h->fail) return;
+   const uint64_t R = 1ull << rlog;
+   K kk[kRegionIPT], vv[kRegionIPT];
+   uint32_t uu[kRegionIPT];
+   int32_t ff[kRegionIPT];
+   uint32_t probe = 0;
+   const uint32_t nfix = h->n_fix;
+   for (uint32_t i = blockIdx.x; i < nfix; i += gridDim.x) {
+      const uint32_t r = lb.list[i];
+      const uint64_t lo = start[r];
+      const uint32_t ms = region_sort<K>(S, lo, static_cast<uint32_t>(start[r + 1] - lo), sp, R, L, kk, vv, uu, ff);
+      probe = std::max(probe, region_place<K>(L, ms, lb.carry[r], static_cast<uint64_t>(r) * R, sp, R, 1, kk, vv,
+                                              uu, ff, b));
+   }
+   probe_to_header<K>(L, probe, h);
 }
 
 // Finish: the Info of the built table, or (build abandoned after the table's
@@ -655,8 +885,12 @@ int build(const void* keys, const void* vals, const void* pairs, void* buckets, 
    void* scan_scratch = base + P.off_scan;
    const size_t scan_bytes = P.off_start - P.off_scan;
    uint64_t* start = reinterpret_cast<uint64_t*>(base + P.off_start);
-   RegionSum* sum = reinterpret_cast<RegionSum*>(base + P.off_sum);
-   int64_t* carry = reinterpret_cast<int64_t*>(base + P.off_carry);
+   RegionLB lb;
+   lb.word = reinterpret_cast<uint64_t*>(base + P.off_word);
+   lb.spec = reinterpret_cast<int64_t*>(base + P.off_spec);
+   lb.carry = reinterpret_cast<int64_t*>(base + P.off_carry);
+   lb.mark = reinterpret_cast<uint32_t*>(base + P.off_mark);
+   lb.list = reinterpret_cast<uint32_t*>(base + P.off_list);
    Slot<K>* T = static_cast<Slot<K>*>(buckets);
 
    hipLaunchKernelGGL(k_build_begin, dim3(1), dim3(64), 0, s, info, h, sp);
@@ -683,11 +917,18 @@ int build(const void* keys, const void* vals, const void* pairs, void* buckets, 
    }
    hipLaunchKernelGGL(k_region_starts<K>, dim3(stream_blocks(P.n + 1, 256)), dim3(256), 0, s, S, P.n, start, P.NR,
                       sp, P.rlog, h);
-   hipLaunchKernelGGL(k_region_sort<K>, dim3(static_cast<uint32_t>(P.NR)), dim3(kRegionBS), 0, s, S, start, sum, sp,
-                      P.rlog, h);
-   hipLaunchKernelGGL(k_region_carry, dim3(1), dim3(kCarryBS), 0, s, sum, carry, P.NR, P.N, h);
-   hipLaunchKernelGGL(k_region_place<K>, dim3(static_cast<uint32_t>(P.NR)), dim3(kRegionBS), 0, s, S, start, sum,
-                      carry, T, sp, P.rlog, h);
+   hipLaunchKernelGGL(k_region_lb_init, dim3(stream_blocks(P.NR, 256)), dim3(256), 0, s, lb, P.NR, h);
+   hipLaunchKernelGGL(k_region_build<K>, dim3(static_cast<uint32_t>((P.NR + kRegionChunk - 1) / kRegionChunk)),
+                      dim3(kRegionBS), 0, s, S, start, lb, T, sp, P.rlog, P.NR, h);
+   RegionSum* blk = reinterpret_cast<RegionSum*>(base + P.off_blk);
+   const uint64_t nblk = (P.NR + kCarryTile - 1) / kCarryTile;
+   const int64_t Ri = static_cast<int64_t>(P.R), Ni = static_cast<int64_t>(P.N);
+   hipLaunchKernelGGL(k_carry_local, dim3(static_cast<uint32_t>(nblk)), dim3(kCarryBS), 0, s, lb, blk, P.NR, Ri, h);
+   hipLaunchKernelGGL(k_carry_top, dim3(1), dim3(1024), 0, s, blk, nblk, P.N, h);
+   hipLaunchKernelGGL(k_carry_apply, dim3(static_cast<uint32_t>(nblk)), dim3(kCarryBS), 0, s, lb, blk, P.NR, Ri, h);
+   hipLaunchKernelGGL(k_carry_mark, dim3(stream_blocks(P.NR, 256)), dim3(256), 0, s, lb, P.NR, Ri, Ni, h);
+   const uint32_t gfix = static_cast<uint32_t>(std::min<uint64_t>(P.NR, static_cast<uint64_t>(cu_count()) * 2));
+   hipLaunchKernelGGL(k_region_fix<K>, dim3(gfix), dim3(kRegionBS), 0, s, S, start, lb, T, sp, P.rlog, h);
    hipLaunchKernelGGL(k_build_end<K>, dim3(stream_blocks(P.N, 256)), dim3(256), 0, s, T, P.N, info, h);
    return static_cast<int>(hipGetLastError());
 }
