@@ -131,6 +131,7 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the erase/clean and load-factor extras")
     ap.add_argument("--no-batched", action="store_true", help="skip the many-maps (batched vs per-map) comparison")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop measurement")
+    ap.add_argument("--no-compaction", action="store_true", help="skip the 2^28-element compaction/scan measurement")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one plain 1 GiB device copy (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -309,6 +310,8 @@ def main():
     if bscratch is not None:
         del bscratch
 
+    if not args.no_compaction:
+        extra["compaction"] = run_compaction(torch, L, dev, S, P, buckets, int(info.cpu().numpy()[1]))
     if not args.no_batched:
         del buckets, out, hit, q, dk, dv
         torch.cuda.empty_cache()
@@ -348,6 +351,93 @@ def main():
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)
+
+
+def run_compaction(torch, L, dev, S, P, buckets, fill):
+    """split::tools on 2^28 hash_pair<u64,u64> {i, random with bit 0 = keep}:
+    stable copy_if with the rule (second & 1) == 1, keep drawn Bernoulli(0.3)
+    and in alternating runs of Geometric(1/64) lengths; exclusive scan of the
+    2^28 uint32 keep flags; extractPattern (valid keys) over the large map's
+    table.  Results are checked against torch on the same device; times are
+    HIP events, best of 3."""
+    import ctypes as C
+
+    from hashinator_amd import _lib
+
+    n = 1 << 28
+    stream = torch.cuda.current_stream()
+    g = torch.Generator(device=dev).manual_seed(0x48415348494E4154 & 0x7FFFFFFF)
+    pairs = torch.empty((n, 2), dtype=torch.int64, device=dev)
+    pairs[:, 0] = torch.arange(n, dtype=torch.int64, device=dev)
+    out = torch.empty_like(pairs)
+    cnt = torch.zeros(1, dtype=torch.int64, device=dev)
+    sb = L.hn_compact_scratch_bytes(n)
+    sc = torch.empty(sb, dtype=torch.uint8, device=dev)
+    pred = _lib.hn_pred(_lib.HN_PRED_MASKEQ, 1, 1, 1)
+
+    def best(fn):
+        t = None
+        for _ in range(3):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            fn()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            dt = e0.elapsed_time(e1)
+            t = dt if t is None else min(t, dt)
+        return t
+
+    def copy_if():
+        _lib.check(L.hn_copy_if_pairs_u64(P(pairs), n, C.byref(pred), P(out), P(cnt), P(sc), sb, S), "copy_if")
+
+    res = {}
+    rng = np.random.default_rng(64)
+    for mode in ("bernoulli", "clustered"):
+        if mode == "bernoulli":
+            keep = torch.rand(n, generator=g, device=dev) < 0.3
+        else:
+            runs = rng.geometric(1 / 64, size=(n // 32) + 16)
+            runs = runs[: np.searchsorted(np.cumsum(runs), n) + 1]
+            bit = (np.arange(runs.size) & 1).astype(np.uint8)
+            keep = torch.from_numpy(np.repeat(bit, runs)[:n].astype(bool)).to(dev)
+        rnd = torch.randint(-(2**62), 2**62, (n,), generator=g, device=dev)
+        pairs[:, 1] = (rnd & ~1) | keep.to(torch.int64)
+        t = best(copy_if)
+        k = int(keep.sum().item())
+        ok = int(cnt.item()) == k
+        # chunked check (torch's own boolean indexing is not reliable on a 4 GiB tensor)
+        base = 0
+        for c0 in range(0, n, 1 << 24):
+            kc = keep[c0:c0 + (1 << 24)]
+            m = int(kc.sum().item())
+            ok = ok and torch.equal(out[base:base + m], pairs[c0:c0 + (1 << 24)][kc])
+            base += m
+        alg = n * 16 + k * 16
+        res[f"copy_if_{mode}"] = {"ms": round(t, 3), "kept": k, "GBps_alg": round(alg / (t * 1e-3) / 1e9, 1),
+                                  "frac_of_8TBps": round(alg / (t * 1e-3) / 8e12, 4), "ok": bool(ok)}
+    flags = keep.to(torch.int32)
+    sout = torch.empty_like(flags)
+    t = best(lambda: _lib.check(L.hn_exclusive_scan_u32(P(flags), P(sout), n, P(sc), sb, S), "scan"))
+    ok = True
+    run = 0
+    for c0 in range(0, n, 1 << 24):
+        fc = flags[c0:c0 + (1 << 24)].to(torch.int64)
+        ref = torch.cumsum(fc, 0) - fc + run
+        ok = ok and torch.equal(sout[c0:c0 + (1 << 24)].to(torch.int64), ref)
+        run += int(fc.sum().item())
+    res["exclusive_scan_u32"] = {"ms": round(t, 3), "GBps_alg": round(8 * n / (t * 1e-3) / 1e9, 1),
+                                 "frac_of_8TBps": round(8 * n / (t * 1e-3) / 8e12, 4), "ok": bool(ok)}
+    del flags, sout, keep, pairs
+    N = buckets.shape[0]
+    vpred = _lib.hn_pred(_lib.HN_PRED_VALID, 0, 0, 0)
+    t = best(lambda: _lib.check(L.hn_copy_if_pairs_u64(P(buckets), N, C.byref(vpred), P(out), P(cnt), P(sc), sb, S),
+                                "extractPattern"))
+    alg = N * 16 + fill * 16
+    res["extractPattern_valid"] = {"ms": round(t, 3), "slots": N, "extracted": int(cnt.item()),
+                                   "GBps_alg": round(alg / (t * 1e-3) / 1e9, 1), "ok": int(cnt.item()) == fill}
+    del out, sc
+    torch.cuda.empty_cache()
+    return res
 
 
 def run_batched(torch):
@@ -482,7 +572,7 @@ def run_extras(L, torch, dev, S, P, tune, buckets, info, dk, dv, q, out, hit, n,
     g = torch.Generator(device=dev).manual_seed(12345)
     er = dk[torch.randperm(n, device=dev, generator=g)[: n // 4]].contiguous()
     t_er = timed(lambda: _lib.check(L.hn_erase_u64(P(er), P(buckets), P(info), er.numel(), S, tune), "erase"))
-    sb = L.hn_clean_tombstones_scratch_bytes_u64(sp, 1 << 30)
+    sb = L.hn_clean_tombstones_scratch_bytes_u64(sp, 0)
     scratch = torch.empty(sb, dtype=torch.uint8, device=dev)
     t_cl = timed(lambda: _lib.check(L.hn_clean_tombstones_u64(P(buckets), P(info), sp, P(scratch), sb, S), "clean"))
 

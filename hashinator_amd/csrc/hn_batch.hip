@@ -7,7 +7,8 @@
 // clean_tombstones (hashinator.h:1261-1315).
 //
 // One workgroup owns one map at a time and runs the requested phases on it
-// back to back, separated by workgroup barriers with agent-scope fences;
+// back to back, separated by workgroup barriers with workgroup-scope fences
+// (no other workgroup touches the map during the launch);
 // maps are handed out by an atomic ticket in the order the caller lists them
 // (largest first balances the grid).  The probe loops are the single-map
 // ones (hn_probe.hpp), so a map's results are those of the plain calls.
@@ -37,9 +38,9 @@ struct BatchArgs {
 };
 
 __device__ __forceinline__ void phase_barrier() {
-   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
    __syncthreads();
-   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
 template <typename K, int G>

@@ -69,7 +69,10 @@ struct BuildHeader {
    int64_t c0;         // carry into region 0
    uint64_t n_distinct;
    uint32_t n_fix;     // regions placed again after the carries are known
-   uint32_t pad[55];
+   int32_t fb_fail;    // the fallback insert found no slot for a key
+   uint64_t n;         // pairs after the first partition pass (the live slots when the input is a table)
+   uint64_t fb_len;    // pairs the fallback insert takes from the partitioned array (0: none)
+   uint32_t pad[50];
 };
 static_assert(sizeof(BuildHeader) == 256, "header is one 256-byte line");
 
@@ -94,19 +97,22 @@ struct BuildPlan {
 inline size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 
 // dcap: widest partition digit (1..9 bits); narrower digits mean more passes.
+// rebuild: the input is the table itself (n = 2^sp slots, live ones kept);
+// the pass count is made odd so that the first pass writes the scratch.
 template <typename K>
-bool build_plan(int sp, uint64_t n, int dcap, BuildPlan* P) {
+bool build_plan(int sp, uint64_t n, int dcap, bool rebuild, BuildPlan* P) {
    if (dcap < 1 || dcap > kMaxDigitBits) return false;
    if (sp < 1 || sp > std::min(max_size_power<K>(), 32)) return false;
    P->sp = sp;
    P->n = n;
    P->N = 1ull << sp;
-   if (n == 0 || n >= P->N) return false;
+   if (n == 0 || n > P->N || (!rebuild && n == P->N)) return false;
    P->rlog = std::min(sp, kRegionLog);
    P->R = 1ull << P->rlog;
    P->rbits = sp - P->rlog;
    P->NR = 1ull << P->rbits;
    P->passes = std::max(1, (P->rbits + dcap - 1) / dcap);
+   if (rebuild && P->passes == 2) P->passes = 3;
    if (P->passes > 3) return false;
    int left = P->rbits;
    for (int i = 0; i < P->passes; ++i) {
@@ -180,34 +186,63 @@ __device__ __forceinline__ uint32_t region_of(K key, int sp, int rlog) {
    return static_cast<uint32_t>((fib_home(key, sp) & mask) >> rlog);
 }
 
-__global__ void k_build_begin(const hn_info* info, BuildHeader* h, int sp) {
+// Build from caller arrays into an empty table, or (rebuild) from the
+// table's own live slots, which is clean_tombstones.
+enum BuildMode { BUILD_EMPTY = 0, BUILD_REBUILD = 1 };
+
+__global__ void k_build_begin(const hn_info* info, BuildHeader* h, int sp, uint64_t len, int mode) {
    if (threadIdx.x == 0) {
-      h->go = (info->size_power == sp && info->fill == 0 && info->tombstone_counter == 0) ? 1u : 0u;
+      const bool sp_ok = info->size_power == sp;
+      if (mode == BUILD_EMPTY)
+         h->go = (sp_ok && info->fill == 0 && info->tombstone_counter == 0) ? 1u : 0u;
+      else
+         h->go = (sp_ok && info->tombstone_counter != 0) ? 1u : 0u;
       h->fail = 0;
       h->max_probe = 0;
       h->c0 = 0;
       h->n_distinct = 0;
+      h->fb_fail = 0;
+      h->fb_len = 0;
+      h->n = mode == BUILD_EMPTY ? len : 0; // a rebuild counts the live slots in its first pass
    }
+}
+
+template <typename K>
+__device__ __forceinline__ bool live_key(K k) {
+   return k != empty_key<K>() && k != tomb_key<K>();
 }
 
 // Per-block digit counts, written digit-major: mat[d * nblocks + block].  A
 // block covers kPartSub consecutive tiles (kPartSuper pairs), which keeps the
 // count matrix -- and its scan -- small.
-template <typename K, int SRC>
-__global__ __launch_bounds__(kPartBS) void k_part_count(PartIn<K> in, uint64_t n, uint32_t* mat, uint32_t nblocks,
-                                                        int sp, int rlog, int dshift, int dbits,
-                                                        const BuildHeader* h) {
+template <typename K, int SRC, bool FILTER>
+__global__ __launch_bounds__(kPartBS) void k_part_count(PartIn<K> in, uint64_t n_in, const uint64_t* d_n, uint32_t* mat,
+                                                        uint32_t nblocks, int sp, int rlog, int dshift, int dbits,
+                                                        BuildHeader* h) {
    __shared__ uint32_t hist[1 << kMaxDigitBits];
+   __shared__ uint32_t live_s;
    if (!h->go) return;
+   const uint64_t n = d_n ? *d_n : n_in;
    const uint32_t D = 1u << dbits, dmask = D - 1;
    for (uint32_t d = threadIdx.x; d < D; d += kPartBS) hist[d] = 0;
+   if (threadIdx.x == 0) live_s = 0;
    __syncthreads();
    const uint64_t t0 = static_cast<uint64_t>(blockIdx.x) * kPartSuper;
    const uint64_t t1 = std::min<uint64_t>(t0 + kPartSuper, n);
-   for (uint64_t e = t0 + threadIdx.x; e < t1; e += kPartBS)
-      atomicAdd(&hist[(region_of(part_key<K, SRC>(in, e), sp, rlog) >> dshift) & dmask], 1u);
+   uint32_t live = 0;
+   for (uint64_t e = t0 + threadIdx.x; e < t1; e += kPartBS) {
+      const K key = part_key<K, SRC>(in, e);
+      if (FILTER && !live_key(key)) continue;
+      atomicAdd(&hist[(region_of(key, sp, rlog) >> dshift) & dmask], 1u);
+      ++live;
+   }
+   if (FILTER) {
+      live = wave_sum(live);
+      if (__lane_id() == 0 && live) atomicAdd(&live_s, live);
+   }
    __syncthreads();
    for (uint32_t d = threadIdx.x; d < D; d += kPartBS) mat[static_cast<uint64_t>(d) * nblocks + blockIdx.x] = hist[d];
+   if (FILTER && threadIdx.x == 0 && live_s) atomic_add_relaxed(&h->n, static_cast<uint64_t>(live_s));
 }
 
 // Stable multisplit.  A block walks its kPartSub tiles in order, keeping the
@@ -217,10 +252,10 @@ __global__ __launch_bounds__(kPartBS) void k_part_count(PartIn<K> in, uint64_t n
 // the earlier rounds (no block barrier per round).  Per-wave counts then give
 // each pair's rank among equal digits in tile order; the tile is staged in
 // LDS in digit order and leaves in runs.
-template <typename K, int SRC>
-__global__ __launch_bounds__(kPartBS) void k_part_scatter(PartIn<K> in, uint64_t n, Slot<K>* out, const uint32_t* offs,
-                                                          uint32_t nblocks, int sp, int rlog, int dshift, int dbits,
-                                                          const BuildHeader* h) {
+template <typename K, int SRC, bool FILTER>
+__global__ __launch_bounds__(kPartBS) void k_part_scatter(PartIn<K> in, uint64_t n_in, const uint64_t* d_n,
+                                                          Slot<K>* out, const uint32_t* offs, uint32_t nblocks, int sp,
+                                                          int rlog, int dshift, int dbits, const BuildHeader* h) {
    constexpr int NW = kPartBS / kWave;
    constexpr int ROUNDS = kPartTile / kPartBS;
    constexpr uint32_t DMAX = 1u << kMaxDigitBits;
@@ -232,6 +267,7 @@ __global__ __launch_bounds__(kPartBS) void k_part_scatter(PartIn<K> in, uint64_t
    __shared__ uint32_t goff_s[DMAX];
    __shared__ uint32_t wsum_s[NW];
    if (!h->go) return;
+   const uint64_t n = d_n ? *d_n : n_in;
    const uint32_t D = 1u << dbits, dmask = D - 1;
    const int lane = __lane_id(), wave = threadIdx.x / kWave;
    const uint64_t lt = (1ull << lane) - 1ull;
@@ -248,20 +284,23 @@ __global__ __launch_bounds__(kPartBS) void k_part_scatter(PartIn<K> in, uint64_t
       __syncthreads();
       Slot<K> el[ROUNDS];
       uint32_t dg[ROUNDS], rk[ROUNDS];
+      bool ok[ROUNDS];
       const uint64_t w0 = t0 + static_cast<uint64_t>(wave) * (kPartTile / NW) + lane;
 #pragma unroll
       for (int r = 0; r < ROUNDS; ++r) {
          const uint64_t e = w0 + static_cast<uint64_t>(r) * kWave;
          dg[r] = 0;
+         ok[r] = false;
          if (e < n) {
             el[r] = part_pair<K, SRC>(in, e);
+            ok[r] = !FILTER || live_key(el[r].key);
             dg[r] = (region_of(el[r].key, sp, rlog) >> dshift) & dmask;
          }
       }
       uint32_t* wc = wcnt_s[wave];
 #pragma unroll
       for (int r = 0; r < ROUNDS; ++r) {
-         const bool valid = w0 + static_cast<uint64_t>(r) * kWave < n;
+         const bool valid = ok[r];
          uint64_t m = __ballot(valid);
          for (int b = 0; b < dbits; ++b) {
             const bool bit = (dg[r] >> b) & 1u;
@@ -305,14 +344,15 @@ __global__ __launch_bounds__(kPartBS) void k_part_scatter(PartIn<K> in, uint64_t
       __syncthreads();
 #pragma unroll
       for (int r = 0; r < ROUNDS; ++r) {
-         if (w0 + static_cast<uint64_t>(r) * kWave < n) {
+         if (ok[r]) {
             const uint32_t p = lstart_s[dg[r]] + wcnt_s[wave][dg[r]] + rk[r];
             tile_s[p] = el[r];
             dig_s[p] = static_cast<uint16_t>(dg[r]);
          }
       }
       __syncthreads();
-      const uint32_t cnt = static_cast<uint32_t>(std::min<uint64_t>(kPartTile, n - t0));
+      uint32_t cnt = 0; // pairs of this tile that are written: the digit scan's total
+      for (int w = 0; w < NW; ++w) cnt += wsum_s[w];
       for (uint32_t p = threadIdx.x; p < cnt; p += kPartBS) {
          const uint32_t d = dig_s[p];
          out[static_cast<uint64_t>(goff_s[d]) + (p - lstart_s[d])] = tile_s[p];
@@ -325,9 +365,10 @@ __global__ __launch_bounds__(kPartBS) void k_part_scatter(PartIn<K> in, uint64_t
 
 // start[r] = first index of region r in the partitioned array, start[NR] = n.
 template <typename K>
-__global__ __launch_bounds__(256) void k_region_starts(const Slot<K>* S, uint64_t n, uint64_t* start, uint64_t NR,
-                                                       int sp, int rlog, const BuildHeader* h) {
+__global__ __launch_bounds__(256) void k_region_starts(const Slot<K>* S, uint64_t* start, uint64_t NR, int sp, int rlog,
+                                                       const BuildHeader* h) {
    if (!h->go) return;
+   const uint64_t n = h->n;
    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i <= n; i += stride) {
       const int64_t prev = i == 0 ? -1 : static_cast<int64_t>(region_of(S[i - 1].key, sp, rlog));
@@ -816,68 +857,80 @@ __global__ __launch_bounds__(kRegionBS) void k_region_fix(const Slot<K>* S, cons
    probe_to_header<K>(L, probe, h);
 }
 
-// Finish: the Info of the built table, or (build abandoned after the table's
-// memory served as scratch) an empty table again with err = INVALID.
+// A region over capacity abandons the placement: the table is emptied and
+// the probing insert takes the partitioned pairs (all of them, in S).
 template <typename K>
-__global__ __launch_bounds__(256) void k_build_end(Slot<K>* b, uint64_t N, hn_info* info, const BuildHeader* h) {
-   if (!h->go) {
-      if (blockIdx.x == 0 && threadIdx.x == 0) info->err = HN_STATUS_INVALID;
-      return;
-   }
-   if (h->fail) {
-      const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-      for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < N; i += stride) {
-         Slot<K> s;
-         s.key = empty_key<K>();
-         s.val = 0;
-         b[i] = s;
-      }
-   }
+__global__ __launch_bounds__(256) void k_build_fallback_prep(Slot<K>* b, uint64_t N, hn_info* info, BuildHeader* h) {
+   const bool fb = h->go && h->fail;
    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      info->reserved0 = 0;
-      info->tombstone_counter = 0;
-      if (h->fail) {
+      h->fb_len = fb ? h->n : 0;
+      if (fb) {
          info->fill = 0;
          info->current_max_bucket_overflow = kBucketOverflow;
-         info->err = HN_STATUS_INVALID;
-         info->reserved1 = 0;
-      } else {
-         info->fill = h->n_distinct;
-         info->current_max_bucket_overflow =
-             std::max<uint64_t>(kBucketOverflow, next_overflow(h->max_probe, kBucketOverflow));
-         info->err = HN_STATUS_SUCCESS;
-         info->reserved1 = kInfoOrdered;
       }
+   }
+   if (!fb) return;
+   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < N; i += stride) {
+      Slot<K> s;
+      s.key = empty_key<K>();
+      s.val = 0;
+      b[i] = s;
    }
 }
 
-template <typename K, int SRC>
-int launch_part(const BuildPlan& P, int pass, PartIn<K> in, Slot<K>* out, uint32_t* mat, uint32_t* scanned,
-                void* scan_scratch, size_t scan_bytes, BuildHeader* h, hipStream_t s) {
+// Finish the Info.  Not run (go = 0): an empty-table build reports
+// err = INVALID with the table untouched; a rebuild (no tombstones) is a
+// no-op.  Placed: fill, overflow of the layout, ordered.  Fallback: the
+// insert has set fill and overflow.
+__global__ void k_build_end(hn_info* info, const BuildHeader* h, int mode) {
+   if (threadIdx.x != 0) return;
+   if (!h->go) {
+      if (mode == BUILD_EMPTY) info->err = HN_STATUS_INVALID;
+      return;
+   }
+   info->reserved0 = 0;
+   info->tombstone_counter = 0;
+   if (h->fail) {
+      info->err = h->fb_fail ? HN_STATUS_FAIL : HN_STATUS_SUCCESS;
+      info->reserved1 = 0;
+   } else {
+      info->fill = h->n_distinct;
+      info->current_max_bucket_overflow =
+          std::max<uint64_t>(kBucketOverflow, next_overflow(h->max_probe, kBucketOverflow));
+      info->err = HN_STATUS_SUCCESS;
+      info->reserved1 = kInfoOrdered;
+   }
+}
+
+template <typename K, int SRC, bool FILTER>
+int launch_part(const BuildPlan& P, int pass, PartIn<K> in, uint64_t n_in, const uint64_t* d_n, Slot<K>* out,
+                uint32_t* mat, void* scan_scratch, size_t scan_bytes, BuildHeader* h, hipStream_t s) {
    int dshift = 0;
    for (int i = 0; i < pass; ++i) dshift += P.dbits[i];
    const int dbits = P.dbits[pass];
    const uint64_t D = 1ull << dbits;
-   hipLaunchKernelGGL((k_part_count<K, SRC>), dim3(P.nblocks), dim3(kPartBS), 0, s, in, P.n, mat, P.nblocks, P.sp,
-                      P.rlog, dshift, dbits, h);
-   int rc = hn_exclusive_scan_u32(mat, scanned, D * P.nblocks, scan_scratch, scan_bytes,
-                                  reinterpret_cast<hn_stream_t>(s));
+   hipLaunchKernelGGL((k_part_count<K, SRC, FILTER>), dim3(P.nblocks), dim3(kPartBS), 0, s, in, n_in, d_n, mat,
+                      P.nblocks, P.sp, P.rlog, dshift, dbits, h);
+   int rc = hn_exclusive_scan_u32(mat, mat, D * P.nblocks, scan_scratch, scan_bytes, reinterpret_cast<hn_stream_t>(s));
    if (rc) return rc;
-   hipLaunchKernelGGL((k_part_scatter<K, SRC>), dim3(P.nblocks), dim3(kPartBS), 0, s, in, P.n, out, scanned, P.nblocks,
-                      P.sp, P.rlog, dshift, dbits, h);
+   hipLaunchKernelGGL((k_part_scatter<K, SRC, FILTER>), dim3(P.nblocks), dim3(kPartBS), 0, s, in, n_in, d_n, out, mat,
+                      P.nblocks, P.sp, P.rlog, dshift, dbits, h);
    return static_cast<int>(hipGetLastError());
 }
 
+// Defined with the insert kernels (hn_map.hip): the probing insert of
+// pairs[0, *d_len), failures OR-ed into *fail_sink, Info left to the caller.
 template <typename K>
-int build(const void* keys, const void* vals, const void* pairs, void* buckets, hn_info* info, size_t len, int sp,
-          void* scratch, size_t bytes, hipStream_t s, const hn_tune* t) {
-   if (!buckets || !info) return HN_EINVAL;
-   if (len == 0) return 0;
-   if (!keys && !pairs) return HN_EINVAL;
-   const int dcap = (t && t->reserved) ? t->reserved : kMaxDigitBits;
-   BuildPlan P;
-   if (!build_plan<K>(sp, len, dcap, &P)) return HN_EINVAL;
-   if (!scratch || bytes < P.total) return HN_ESCRATCH;
+int launch_insert_pairs_dlen(const Slot<K>* pairs, Slot<K>* buckets, hn_info* info, uint64_t max_len,
+                             const uint64_t* d_len, int32_t* fail_sink, hipStream_t s);
+
+// Everything after the first partition pass's input is chosen.  src0 picks
+// the first pass's input kind; n_in is its element count (the slot count for
+// a rebuild, whose first pass keeps live slots only).
+template <typename K>
+int build_core(const BuildPlan& P, int mode, int src0, PartIn<K> in0, uint64_t n_in, void* buckets, hn_info* info,
+               void* scratch, hipStream_t s) {
    char* base = static_cast<char*>(scratch);
    BuildHeader* h = reinterpret_cast<BuildHeader*>(base);
    Slot<K>* S = reinterpret_cast<Slot<K>*>(base + P.off_pairs);
@@ -893,33 +946,41 @@ int build(const void* keys, const void* vals, const void* pairs, void* buckets, 
    lb.list = reinterpret_cast<uint32_t*>(base + P.off_list);
    Slot<K>* T = static_cast<Slot<K>*>(buckets);
 
-   hipLaunchKernelGGL(k_build_begin, dim3(1), dim3(64), 0, s, info, h, sp);
-   // Pass i writes S when (passes - 1 - i) is even, the table's memory otherwise,
-   // so the last pass always lands in S.
+   hipLaunchKernelGGL(k_build_begin, dim3(1), dim3(64), 0, s, info, h, P.sp, static_cast<uint64_t>(P.n), mode);
+   // Pass i writes S when (passes - 1 - i) is even, the table's memory
+   // otherwise, so the last pass lands in S; a rebuild has an odd number of
+   // passes, so its first pass (which reads the table) writes S.
    const Slot<K>* src = nullptr;
    for (int pass = 0; pass < P.passes; ++pass) {
       Slot<K>* dst = ((P.passes - 1 - pass) % 2 == 0) ? S : T;
       int rc;
       if (pass == 0) {
-         PartIn<K> in{static_cast<const K*>(keys), static_cast<const K*>(vals), static_cast<const Slot<K>*>(pairs)};
-         if (pairs)
-            rc = launch_part<K, SRC_PAIRS>(P, pass, in, dst, mat, mat, scan_scratch, scan_bytes, h, s);
-         else if (vals)
-            rc = launch_part<K, SRC_SOA>(P, pass, in, dst, mat, mat, scan_scratch, scan_bytes, h, s);
-         else
-            rc = launch_part<K, SRC_INDEX>(P, pass, in, dst, mat, mat, scan_scratch, scan_bytes, h, s);
+         switch (src0) {
+         case SRC_PAIRS:
+            rc = launch_part<K, SRC_PAIRS, false>(P, 0, in0, n_in, nullptr, dst, mat, scan_scratch, scan_bytes, h, s);
+            break;
+         case SRC_SOA:
+            rc = launch_part<K, SRC_SOA, false>(P, 0, in0, n_in, nullptr, dst, mat, scan_scratch, scan_bytes, h, s);
+            break;
+         case SRC_INDEX:
+            rc = launch_part<K, SRC_INDEX, false>(P, 0, in0, n_in, nullptr, dst, mat, scan_scratch, scan_bytes, h, s);
+            break;
+         default:
+            rc = launch_part<K, SRC_TABLE, true>(P, 0, in0, n_in, nullptr, dst, mat, scan_scratch, scan_bytes, h, s);
+            break;
+         }
       } else {
          PartIn<K> in{nullptr, nullptr, src};
-         rc = launch_part<K, SRC_TABLE>(P, pass, in, dst, mat, mat, scan_scratch, scan_bytes, h, s);
+         rc = launch_part<K, SRC_TABLE, false>(P, pass, in, 0, &h->n, dst, mat, scan_scratch, scan_bytes, h, s);
       }
       if (rc) return rc;
       src = dst;
    }
-   hipLaunchKernelGGL(k_region_starts<K>, dim3(stream_blocks(P.n + 1, 256)), dim3(256), 0, s, S, P.n, start, P.NR,
-                      sp, P.rlog, h);
+   hipLaunchKernelGGL(k_region_starts<K>, dim3(stream_blocks(P.n + 1, 256)), dim3(256), 0, s, S, start, P.NR, P.sp,
+                      P.rlog, h);
    hipLaunchKernelGGL(k_region_lb_init, dim3(stream_blocks(P.NR, 256)), dim3(256), 0, s, lb, P.NR, h);
    hipLaunchKernelGGL(k_region_build<K>, dim3(static_cast<uint32_t>((P.NR + kRegionChunk - 1) / kRegionChunk)),
-                      dim3(kRegionBS), 0, s, S, start, lb, T, sp, P.rlog, P.NR, h);
+                      dim3(kRegionBS), 0, s, S, start, lb, T, P.sp, P.rlog, P.NR, h);
    RegionSum* blk = reinterpret_cast<RegionSum*>(base + P.off_blk);
    const uint64_t nblk = (P.NR + kCarryTile - 1) / kCarryTile;
    const int64_t Ri = static_cast<int64_t>(P.R), Ni = static_cast<int64_t>(P.N);
@@ -928,16 +989,54 @@ int build(const void* keys, const void* vals, const void* pairs, void* buckets, 
    hipLaunchKernelGGL(k_carry_apply, dim3(static_cast<uint32_t>(nblk)), dim3(kCarryBS), 0, s, lb, blk, P.NR, Ri, h);
    hipLaunchKernelGGL(k_carry_mark, dim3(stream_blocks(P.NR, 256)), dim3(256), 0, s, lb, P.NR, Ri, Ni, h);
    const uint32_t gfix = static_cast<uint32_t>(std::min<uint64_t>(P.NR, static_cast<uint64_t>(cu_count()) * 2));
-   hipLaunchKernelGGL(k_region_fix<K>, dim3(gfix), dim3(kRegionBS), 0, s, S, start, lb, T, sp, P.rlog, h);
-   hipLaunchKernelGGL(k_build_end<K>, dim3(stream_blocks(P.N, 256)), dim3(256), 0, s, T, P.N, info, h);
+   hipLaunchKernelGGL(k_region_fix<K>, dim3(gfix), dim3(kRegionBS), 0, s, S, start, lb, T, P.sp, P.rlog, h);
+   hipLaunchKernelGGL(k_build_fallback_prep<K>, dim3(stream_blocks(P.N, 256)), dim3(256), 0, s, T, P.N, info, h);
+   int rc = launch_insert_pairs_dlen<K>(S, T, info, P.n, &h->fb_len, &h->fb_fail, s);
+   if (rc) return rc;
+   hipLaunchKernelGGL(k_build_end, dim3(1), dim3(64), 0, s, info, h, mode);
    return static_cast<int>(hipGetLastError());
+}
+
+template <typename K>
+int build(const void* keys, const void* vals, const void* pairs, void* buckets, hn_info* info, size_t len, int sp,
+          void* scratch, size_t bytes, hipStream_t s, const hn_tune* t) {
+   if (!buckets || !info) return HN_EINVAL;
+   if (len == 0) return 0;
+   if (!keys && !pairs) return HN_EINVAL;
+   const int dcap = (t && t->reserved) ? t->reserved : kMaxDigitBits;
+   BuildPlan P;
+   if (!build_plan<K>(sp, len, dcap, false, &P)) return HN_EINVAL;
+   if (!scratch || bytes < P.total) return HN_ESCRATCH;
+   PartIn<K> in{static_cast<const K*>(keys), static_cast<const K*>(vals), static_cast<const Slot<K>*>(pairs)};
+   const int src0 = pairs ? SRC_PAIRS : (vals ? SRC_SOA : SRC_INDEX);
+   return build_core<K>(P, BUILD_EMPTY, src0, in, len, buckets, info, scratch, s);
 }
 
 template <typename K>
 size_t build_scratch_bytes(int sp, size_t len) {
    BuildPlan P;
-   return build_plan<K>(sp, len, kMaxDigitBits, &P) ? P.total : 0;
+   return build_plan<K>(sp, len, kMaxDigitBits, false, &P) ? P.total : 0;
 }
+
+// clean_tombstones as a rebuild of the live slots (see hn_map.hip).
+template <typename K>
+size_t rebuild_scratch_bytes(int sp) {
+   BuildPlan P;
+   return (sp >= 1 && sp <= 32 && build_plan<K>(sp, 1ull << sp, kMaxDigitBits, true, &P)) ? P.total : 0;
+}
+
+template <typename K>
+int rebuild(void* buckets, hn_info* info, int sp, void* scratch, size_t bytes, hipStream_t s) {
+   BuildPlan P;
+   if (!build_plan<K>(sp, 1ull << sp, kMaxDigitBits, true, &P)) return HN_EINVAL;
+   if (!scratch || bytes < P.total) return HN_ESCRATCH;
+   PartIn<K> in{nullptr, nullptr, static_cast<const Slot<K>*>(buckets)};
+   return build_core<K>(P, BUILD_REBUILD, SRC_TABLE, in, P.N, buckets, info, scratch, s);
+}
+template size_t rebuild_scratch_bytes<uint32_t>(int);
+template size_t rebuild_scratch_bytes<uint64_t>(int);
+template int rebuild<uint32_t>(void*, hn_info*, int, void*, size_t, hipStream_t);
+template int rebuild<uint64_t>(void*, hn_info*, int, void*, size_t, hipStream_t);
 
 } // namespace hn
 

@@ -24,62 +24,72 @@ constexpr int kBS = 256;         // threads per tile
 constexpr int kWaves = kBS / kWave;
 constexpr int kCompactIPT = 8;   // elements per thread in a compaction tile
 constexpr int kCompactTile = kBS * kCompactIPT;
-constexpr uint32_t kFlagAgg = 1, kFlagInc = 2;
+constexpr uint64_t kFlagAgg = 1, kFlagInc = 2;
 
+// Per-tile look-back state: the 64-bit value in two words, each carrying
+// its own state (0 none, 1 aggregate, 2 inclusive) in the top bits and one
+// 32-bit half below.  Words are written and read with relaxed device-scope
+// atomics; a reader takes a tile's value only when both halves carry the
+// same state, so no release/acquire fence is needed (on a multi-XCD device
+// those write back or invalidate the whole L2).
 struct Lookback {
    uint32_t* ticket;
-   uint32_t* flags;    // [tiles]
-   uint64_t* agg;      // [tiles]
-   uint64_t* inc;      // [tiles]
+   uint64_t* lo; // [tiles]
+   uint64_t* hi; // [tiles]
 };
 
-inline size_t lookback_bytes(uint64_t tiles) { return 256 + tiles * (4 + 8 + 8) + 64; }
+inline size_t lookback_bytes(uint64_t tiles) { return 256 + tiles * 16 + 64; }
 
 inline Lookback lookback_layout(void* scratch, uint64_t tiles) {
    char* p = static_cast<char*>(scratch);
    Lookback L;
    L.ticket = reinterpret_cast<uint32_t*>(p);
-   L.agg = reinterpret_cast<uint64_t*>(p + 256);
-   L.inc = L.agg + tiles;
-   L.flags = reinterpret_cast<uint32_t*>(L.inc + tiles);
+   L.lo = reinterpret_cast<uint64_t*>(p + 256);
+   L.hi = L.lo + tiles;
    return L;
 }
 
 __global__ __launch_bounds__(256) void k_lookback_init(Lookback L, uint64_t tiles) {
    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < tiles; i += stride)
-      L.flags[i] = 0;
+   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < tiles; i += stride) {
+      L.lo[i] = 0;
+      L.hi[i] = 0;
+   }
    if (blockIdx.x == 0 && threadIdx.x == 0) *L.ticket = 0;
 }
 
 __global__ void k_store_u64(uint64_t* p, uint64_t v) { *p = v; }
+
+__device__ __forceinline__ void lb_publish(const Lookback& L, uint64_t tile, uint64_t flag, uint64_t v) {
+   __hip_atomic_store(&L.hi[tile], (flag << 62) | (v >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+   __hip_atomic_store(&L.lo[tile], (flag << 62) | (v & 0xFFFFFFFFull), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// State of tile p (0 while its two halves disagree) and its value.
+__device__ __forceinline__ uint64_t lb_read(const Lookback& L, uint64_t p, uint64_t* v) {
+   const uint64_t a = __hip_atomic_load(&L.lo[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+   const uint64_t b = __hip_atomic_load(&L.hi[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+   if ((a >> 62) != (b >> 62)) return 0;
+   *v = ((b & 0xFFFFFFFFull) << 32) | (a & 0xFFFFFFFFull);
+   return a >> 62;
+}
 
 // Exclusive prefix of tile `tile` whose own aggregate is `mine`; called by
 // one full wave.  Publishes the tile's aggregate and inclusive prefix.
 __device__ uint64_t lookback_prefix(const Lookback& L, uint32_t tile, uint64_t mine) {
    const int lane = __lane_id();
    if (tile == 0) {
-      if (lane == 0) {
-         L.inc[0] = mine;
-         __hip_atomic_store(&L.flags[0], kFlagInc, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-      }
+      if (lane == 0) lb_publish(L, 0, kFlagInc, mine);
       return 0;
    }
-   if (lane == 0) {
-      L.agg[tile] = mine;
-      __hip_atomic_store(&L.flags[tile], kFlagAgg, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-   }
+   if (lane == 0) lb_publish(L, tile, kFlagAgg, mine);
    uint64_t prefix = 0;
    int64_t hi = static_cast<int64_t>(tile) - 1; // newest predecessor not yet folded in
    for (;;) {
       const int64_t p = hi - lane;
-      uint32_t f = kFlagInc; // lanes past tile 0 behave as an inclusive zero
+      uint64_t f = kFlagInc; // lanes past tile 0 behave as an inclusive zero
       uint64_t v = 0;
-      if (p >= 0) {
-         f = __hip_atomic_load(&L.flags[p], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-         if (f == kFlagInc) v = L.inc[p];
-         else if (f == kFlagAgg) v = L.agg[p];
-      }
+      if (p >= 0) f = lb_read(L, static_cast<uint64_t>(p), &v);
       const uint64_t not_ready = __ballot(f == 0);
       const uint64_t inc_mask = __ballot(f == kFlagInc);
       // Lanes up to and including the first inclusive one, if none before it is still pending.
@@ -94,10 +104,7 @@ __device__ uint64_t lookback_prefix(const Lookback& L, uint32_t tile, uint64_t m
       if (first_inc < 64) break;
       hi -= 64;
    }
-   if (lane == 0) {
-      L.inc[tile] = prefix + mine;
-      __hip_atomic_store(&L.flags[tile], kFlagInc, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-   }
+   if (lane == 0) lb_publish(L, tile, kFlagInc, prefix + mine);
    return prefix;
 }
 

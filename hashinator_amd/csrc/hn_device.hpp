@@ -15,8 +15,9 @@ namespace hn {
 
 constexpr int kWave = 64;
 // hn_info::reserved1 bits: 1 = an insert kernel saw a failure (transient,
-// cleared by the kernel's last wave); 2 = the table is in Robin Hood order
-// (set by the reorder pass, cleared by every insert).
+// cleared when the insert's err is published); 2 = the table is in Robin
+// Hood order (set by the reorder pass and the bulk build, cleared by every
+// insert).
 constexpr int32_t kInfoFail = 1;
 constexpr int32_t kInfoOrdered = 2;
 constexpr uint64_t kBucketOverflow = HN_BUCKET_OVERFLOW;
@@ -106,29 +107,18 @@ __device__ __forceinline__ T atomic_cas(T* p, T expected, T desired) {
 }
 
 // Per-wave epilogue of the insert kernels: fold the wave's counters into the
-// map's Info, and let the last wave of the grid publish `err`.  Must be
-// reached by every wave of the grid exactly once, with all lanes active.
-__device__ __forceinline__ void insert_epilogue(hn_info* info, uint64_t placed, uint64_t max_probe, int failed,
-                                                const uint64_t* expected_fill) {
+// map's Info with relaxed device-scope atomics (a failure sets bit 0 of
+// reserved1).  `err` is published by a one-thread kernel launched after the
+// insert (k_insert_finish), so no wave needs a device-scope fence.  All
+// lanes active.
+__device__ __forceinline__ void insert_fold(hn_info* info, uint64_t placed, uint64_t max_probe, int failed) {
    placed = wave_sum(placed);
    max_probe = wave_max(max_probe);
    failed = wave_max(failed);
    if (__lane_id() == 0) {
       if (placed) atomic_add_relaxed(&info->fill, placed);
       if (max_probe) atomic_max_relaxed(&info->current_max_bucket_overflow, next_overflow(max_probe, kBucketOverflow));
-      if (failed) __hip_atomic_fetch_or(&info->reserved1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      const int32_t waves = static_cast<int32_t>(gridDim.x * (blockDim.x / kWave));
-      const int32_t prev = __hip_atomic_fetch_add(&info->reserved0, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-      if (prev == waves - 1) {
-         int32_t f = __hip_atomic_load(&info->reserved1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) & kInfoFail;
-         if (expected_fill) {
-            uint64_t now = __hip_atomic_load(&info->fill, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-            if (now != *expected_fill) f = 1;
-         }
-         info->err = f ? HN_STATUS_FAIL : HN_STATUS_SUCCESS;
-         info->reserved1 = 0;
-         __hip_atomic_store(&info->reserved0, 0, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-      }
+      if (failed) __hip_atomic_fetch_or(&info->reserved1, kInfoFail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
    }
 }
 

@@ -20,6 +20,13 @@
 
 namespace hn {
 
+// The rebuild path of clean_tombstones (hn_build.hip): every live slot goes
+// through the bulk build into the same table, in Robin Hood order.
+template <typename K>
+size_t rebuild_scratch_bytes(int sp);
+template <typename K>
+int rebuild(void* buckets, hn_info* info, int sp, void* scratch, size_t bytes, hipStream_t s);
+
 // insert_kernel / insert_index_kernel (kernels_AMD.h:136-405, 507-641)
 template <typename K, int G, int SRC>
 __global__ __launch_bounds__(1024) void k_insert(InsertArgs<K> a) {
@@ -42,8 +49,19 @@ __global__ __launch_bounds__(1024) void k_insert(InsertArgs<K> a) {
          if (failed) __hip_atomic_fetch_or(a.fail_sink, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
    } else {
-      insert_epilogue(a.info, c.placed, c.max_probe, c.failed, a.expected_fill);
+      insert_fold(a.info, c.placed, c.max_probe, c.failed);
    }
+}
+
+// After an insert kernel: err = FAIL if a key found no slot (or the fill is
+// not the expected one), else SUCCESS; reserved words cleared (an insert does
+// not keep Robin Hood order).
+__global__ void k_insert_finish(hn_info* info, const uint64_t* expected_fill) {
+   int32_t f = info->reserved1 & kInfoFail;
+   if (expected_fill && info->fill != *expected_fill) f = 1;
+   info->err = f ? HN_STATUS_FAIL : HN_STATUS_SUCCESS;
+   info->reserved0 = 0;
+   info->reserved1 = 0;
 }
 
 // retrieve_kernel (kernels_AMD.h:647-752), delete_kernel (412-505), reset_to_empty (65-134)
@@ -297,6 +315,7 @@ int launch_insert(InsertArgs<K> a, uint64_t grid_len, hipStream_t s, const hn_tu
    InsertKernel<K, SRC> k = pick_insert<K, SRC>(L.probe_width);
    if (!k) return HN_EINVAL;
    hipLaunchKernelGGL(k, dim3(L.blocks), dim3(L.block_size), 0, s, a);
+   if (!a.fail_sink) hipLaunchKernelGGL(k_insert_finish, dim3(1), dim3(1), 0, s, a.info, a.expected_fill);
    return static_cast<int>(hipGetLastError());
 }
 
@@ -419,7 +438,10 @@ size_t clean_scratch_bytes(int sp, size_t budget) {
    if (sp < 1 || sp > max_size_power<K>()) return 0;
    const uint64_t n = 1ull << sp;
    const size_t full = clean_header_bytes(1) + n * sizeof(Slot<K>);
-   if (budget == 0 || budget >= full) return full;
+   const size_t rb = sp <= 32 ? rebuild_scratch_bytes<K>(sp) : 0;
+   if (budget == 0) return std::max(full, rb);
+   if (rb && budget >= rb) return rb;
+   if (budget >= full) return full;
    uint64_t rs, nr;
    size_t hb;
    for (size_t b = budget; b >= hn_clean_tombstones_min_scratch(); b /= 2)
@@ -427,10 +449,25 @@ size_t clean_scratch_bytes(int sp, size_t budget) {
    return hn_clean_tombstones_min_scratch();
 }
 
+// Probing insert of pairs[0, *d_len) with failures OR-ed into *fail_sink
+// (used by the bulk build's fallback, hn_build.hip).
+template <typename K>
+int launch_insert_pairs_dlen(const Slot<K>* pairs, Slot<K>* buckets, hn_info* info, uint64_t max_len,
+                             const uint64_t* d_len, int32_t* fail_sink, hipStream_t s) {
+   InsertArgs<K> a{nullptr, nullptr, pairs, buckets, info, max_len, d_len, nullptr, fail_sink};
+   return launch_insert<K, SRC_PAIRS>(a, max_len, s, nullptr, OP_REINSERT);
+}
+template int launch_insert_pairs_dlen<uint32_t>(const Slot<uint32_t>*, Slot<uint32_t>*, hn_info*, uint64_t,
+                                                const uint64_t*, int32_t*, hipStream_t);
+template int launch_insert_pairs_dlen<uint64_t>(const Slot<uint64_t>*, Slot<uint64_t>*, hn_info*, uint64_t,
+                                                const uint64_t*, int32_t*, hipStream_t);
+
 template <typename K>
 int clean_tombstones(void* buckets, hn_info* info, int sp, void* scratch, size_t bytes, hipStream_t s) {
    if (!buckets || !info || !scratch) return HN_EINVAL;
    if (sp < 1 || sp > max_size_power<K>()) return HN_EINVAL;
+   const size_t rb = sp <= 32 ? rebuild_scratch_bytes<K>(sp) : 0;
+   if (rb && bytes >= rb) return rebuild<K>(buckets, info, sp, scratch, bytes, s);
    const uint64_t n = 1ull << sp;
    uint64_t rs = 0, nr = 0;
    size_t hb = 0;

@@ -306,9 +306,10 @@ class Hashmap:
         scratch = torch.empty(sb, dtype=torch.uint8, device=self.device)
         check(self._fn("clean_tombstones")(_ptr(self.buckets), _ptr(self._info), self._sp, _ptr(scratch), sb,
                                            _stream(stream)), "clean_tombstones")
-        if self.auto_reorder:
-            self._reorder(stream)  # the reinsertion leaves arrival order
         self._sync(stream)
+        if self.auto_reorder and not self.ordered():
+            self._reorder(stream)  # the bounded-scratch path reinserts in arrival order
+            self._sync(stream)
 
     def device_rehash(self, newSizePower: int, stream=None):
         """device_rehash(p, s) (hashinator.h:284-336)."""

@@ -514,22 +514,17 @@ public:
    void clean_tombstones(hipStream_t s = 0) {
       if (_info.tombstoneCounter == 0) return;
       const int sp = _info.sizePower;
-      const size_t full =
+      // Whole-table scratch: the rebuild path (every live slot through the
+      // bulk build, leaving the table in Robin Hood order).
+      const size_t sb =
           W64 ? hn_clean_tombstones_scratch_bytes_u64(sp, 0) : hn_clean_tombstones_scratch_bytes_u32(sp, 0);
-      // A table with no EMPTY slot is cleaned in one region; otherwise an eighth of it is enough.
-      const bool no_empty = _info.fill + _info.tombstoneCounter >= _nbuckets;
-      const size_t budget = no_empty ? 0 : std::max<size_t>(full / 8, size_t(64) << 20);
-      const size_t sb = W64 ? hn_clean_tombstones_scratch_bytes_u64(sp, budget)
-                            : hn_clean_tombstones_scratch_bytes_u32(sp, budget);
       void* scratch = nullptr;
       SPLIT_CHECK_ERR(hipMallocAsync(&scratch, sb, s));
       chk(W64 ? hn_clean_tombstones_u64(_buckets, dinfo(), sp, scratch, sb, st(s))
               : hn_clean_tombstones_u32(_buckets, dinfo(), sp, scratch, sb, st(s)));
       SPLIT_CHECK_ERR(hipFreeAsync(scratch, s));
-      if (_auto_reorder > 0.0f)
-         reorder(s); // the reinsertion leaves arrival order
-      else
-         changed(s);
+      changed(s);
+      if (_auto_reorder > 0.0f && !ordered()) reorder(s);
    }
 
    template <bool prefetches = true>

@@ -38,8 +38,7 @@ typedef struct ihipStream_t* hn_stream_t; /* == hipStream_t */
 #endif
 
 /* Map metadata; byte-for-byte Hashinator::Info (hashinator/defaults.h:44-54).
- * reserved0 must be zero between calls (hn_info_init does that); insert
- * kernels use it to finalise `err` without a host round trip.  reserved1
+ * reserved0 is zero between calls (hn_info_init sets it).  reserved1
  * holds flags: bit 0 is transient (zero between calls), bit 1
  * (HN_INFO_ORDERED) says the table is in Robin Hood order -- set by
  * hn_reorder_*, cleared by every insert, clean and rehash; a caller that
@@ -193,12 +192,17 @@ HN_API size_t hn_clean_tombstones_min_scratch(void);
 
 /* Hashmap::clean_tombstones (hashinator.h:1261-1315).  No-op when
  * tombstone_counter == 0 (checked on the device).  Otherwise TOMBSTONEs
- * become EMPTY, every live element not in its home slot is lifted out and
- * reinserted, tombstone_counter = 0, overflow restarts at
- * HN_BUCKET_OVERFLOW, fill is unchanged.  The table is processed region by
- * region starting after an EMPTY slot, so scratch is bounded by the region
- * size, not the table.  A table with no EMPTY slot at all needs scratch for
- * the whole table; with less, err = FAIL and the table is left unchanged. */
+ * become EMPTY and the live elements are placed again: tombstone_counter = 0,
+ * fill is unchanged, overflow = max(64, nextOverflow(longest probe)), and the
+ * occupied-slot set is the one a fresh insert of the live set gives.
+ * With scratch of hn_clean_tombstones_scratch_bytes_*(size_power, 0) bytes
+ * (about the table's size) the live slots are rebuilt in one pass through
+ * the bulk build (hn_build_*), leaving the table in Robin Hood order
+ * (HN_INFO_ORDERED).  With less, the table is processed region by region
+ * starting after an EMPTY slot: every live element not in its home slot is
+ * lifted out and reinserted, so scratch is bounded by the region size; a
+ * table with no EMPTY slot at all then needs scratch for the whole table,
+ * and with less, err = FAIL and the table is left unchanged. */
 HN_API int hn_clean_tombstones_u32(void* buckets, hn_info* info, int size_power, void* scratch, size_t scratch_bytes,
                             hn_stream_t s);
 HN_API int hn_clean_tombstones_u64(void* buckets, hn_info* info, int size_power, void* scratch, size_t scratch_bytes,
@@ -246,11 +250,13 @@ HN_API int hn_reorder_u64(void* buckets, hn_info* info, int size_power, void* sc
  * an LDS sort per region and a carry scan (about 10 streaming passes' worth
  * of the pair array).  Needs len < 2^size_power and `scratch` of
  * hn_build_scratch_bytes_*(size_power, len) bytes; the table's previous slot
- * contents are overwritten.  When the table is not empty (table untouched) or
- * one region receives more than 2560 keys (strongly skewed keys; the table is
- * left empty) the call sets err = INVALID and the caller inserts with
- * hn_insert_* instead.  tune->reserved (1..9, 0 = 9) caps the partition digit
- * width in bits; probe_width and block_size are not used. */
+ * contents are overwritten.  When the table is not empty the call sets
+ * err = INVALID and leaves it untouched (insert with hn_insert_* instead).
+ * When one 2048-slot region receives more than 2560 keys (strongly skewed
+ * keys) the table is filled by the probing insert instead, on the device:
+ * same live set, arrival order, HN_INFO_ORDERED not set.  tune->reserved
+ * (1..9, 0 = 9) caps the partition digit width in bits; probe_width and
+ * block_size are not used. */
 HN_API size_t hn_build_scratch_bytes_u32(int size_power, size_t len);
 HN_API size_t hn_build_scratch_bytes_u64(int size_power, size_t len);
 HN_API int hn_build_u32(const void* keys, const void* vals, void* buckets, hn_info* info, size_t len, int size_power,

@@ -12,7 +12,7 @@ import torch
 import oracle as O
 from hashinator_amd import Hashmap, hn_tune, to_numpy
 from hashinator_amd import _lib
-from tests.helpers import NDT, keyset
+from tests.helpers import NDT, keyset, live_pairs, occupancy, reachable_all
 
 pytestmark = pytest.mark.gpu
 
@@ -152,20 +152,58 @@ def concentrated_keys(bits, sp, n, span, seed):
 
 
 @pytest.mark.parametrize("bits", [32, 64])
-def test_build_overfull_region_declines_and_map_falls_back(gpu, bits):
-    # 3000 keys homed in one 2048-slot region: over the 2560-key region capacity
+def test_build_overfull_region_falls_back_on_device(gpu, bits):
+    # 3000 keys homed in one 2048-slot region: over the 2560-key region
+    # capacity, so the table is filled by the probing insert instead
     sp = 14
     keys = concentrated_keys(bits, sp, 3000, 2048, seed=6)
-    img, inf, rc = run_build(bits, sp, keys, keys)
+    vals = (keys ^ NDT[bits](3)).astype(NDT[bits])
+    img, inf, rc = run_build(bits, sp, keys, vals)
     assert rc == 0
-    assert (int(inf[4]) & 0xFFFFFFFF) == _lib.HN_STATUS_INVALID and int(inf[1]) == 0
-    assert np.all(img.reshape(-1, 2)[:, 0] == np.iinfo(NDT[bits]).max)  # left empty
-    # the map's insert falls back to the probing kernel (then reorders)
+    assert (int(inf[4]) & 0xFFFFFFFF) == 0 and int(inf[1]) == keys.size and int(inf[3]) == 0
+    assert (int(inf[4]) >> 32) == 0  # arrival order: not marked ordered
+    m = O.GpuModel(sp, bits)
+    m.insert(keys, vals)
+    assert np.array_equal(live_pairs(img, bits), live_pairs(m.b, bits))
+    assert np.array_equal(occupancy(img, bits), occupancy(m.b, bits))
+    assert reachable_all(img, bits, sp)
+    mp = getattr(O.lib(), f"or_gpu{bits}_max_probe")(np.ascontiguousarray(img).ctypes.data, sp)
+    assert int(inf[2]) >= mp
     hm = Hashmap(sp, bits)
-    hm.insert(keys, keys, targetLF=1.0)
-    assert hm.size() == keys.size and hm.info().err == 0
+    hm.insert(keys, vals, targetLF=1.0)
     _, v, h = hm.retrieve(keys)
-    assert h.cpu().numpy().all() and np.array_equal(to_numpy(v, bits), keys)
+    assert h.cpu().numpy().all() and np.array_equal(to_numpy(v, bits), vals)
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+@pytest.mark.parametrize("sp,lf", [(10, 0.9), (14, 0.9), (20, 0.95), (21, 0.85)])
+def test_clean_rebuild_image_exact(gpu, bits, sp, lf):
+    """clean_tombstones with whole-table scratch rebuilds the live slots: the
+    image is the oracle's clean followed by its reorder, bit for bit."""
+    keys = keyset(int(lf * (1 << sp)), bits, seed=sp + bits)
+    vals = (keys * NDT[bits](9)).astype(NDT[bits])
+    hm = Hashmap(sp, bits, auto_build=0, auto_reorder=0)
+    hm.insert(keys, vals, targetLF=1.0)
+    er = keys[np.random.default_rng(sp).choice(keys.size, keys.size // 4, replace=False)]
+    hm.erase(er)
+    m = O.GpuModel(sp, bits, buckets=hm.host_buckets().reshape(-1).copy(), info=O.Info(
+        sp, 0, hm.info().fill, hm.info().current_max_bucket_overflow, hm.info().tombstone_counter, 0, 0))
+    hm.clean_tombstones()  # default: whole-table scratch -> rebuild
+    m.clean_tombstones()
+    assert m.reorder() == 1
+    # every key slot exact; values exact in live slots (an EMPTY slot's value is unspecified:
+    # the oracle keeps the erased value under the EMPTY key, the rebuild writes 0)
+    img, ref = hm.host_buckets(), m.b.reshape(-1, 2)
+    assert np.array_equal(img[:, 0], ref[:, 0])
+    live = ref[:, 0] != np.iinfo(NDT[bits]).max
+    assert np.array_equal(img[live, 1], ref[live, 1])
+    i = hm.info()
+    assert (i.fill, i.tombstone_counter, i.current_max_bucket_overflow) == (m.info.fill, 0, m.info.overflow)
+    assert i.err == 0 and (i.reserved1 & _lib.HN_INFO_ORDERED)
+    # no tombstones: a no-op, Info untouched
+    before = hm.host_buckets().copy()
+    hm.clean_tombstones()
+    assert np.array_equal(hm.host_buckets(), before)
 
 
 @pytest.mark.parametrize("bits", [32, 64])

@@ -133,7 +133,7 @@ def test_ordered_retrieve_and_erase_exact(gpu, bits, width):
 
 @pytest.mark.parametrize("bits", [32, 64])
 def test_flag_cleared_by_insert_clean_rehash(gpu, bits):
-    sp = 12
+    sp = 16  # large enough that 64 KiB of clean scratch is below the rebuild's need
     keys = keyset(int(0.8 * (1 << sp)), bits, seed=5)
     hm, model = build(bits, sp, keys[:-100], keys[:-100])
     assert hm.reorder() and ordered_flag(hm)
@@ -147,13 +147,16 @@ def test_flag_cleared_by_insert_clean_rehash(gpu, bits):
     before = hm.host_buckets()
     assert hm.reorder()  # tombstones present: no-op; the flag set before stays valid
     assert np.array_equal(hm.host_buckets(), before)
-    hm.clean_tombstones()
+    hm.clean_tombstones(scratch_budget=1 << 16)  # bounded scratch: region-by-region reinsertion
     assert not ordered_flag(hm)
     assert hm.reorder()
+    hm.erase(keys[50:60])
+    hm.clean_tombstones()  # whole-table scratch: rebuilt in Robin Hood order
+    assert ordered_flag(hm)
     hm.device_rehash(sp + 1)
     assert not ordered_flag(hm)
     _, v, h = hm.retrieve(keys)
-    assert np.array_equal(h.cpu().numpy().astype(bool), ~np.isin(keys, keys[:50]))
+    assert np.array_equal(h.cpu().numpy().astype(bool), ~np.isin(keys, keys[:60]))
 
 
 @pytest.mark.parametrize("bits", [32, 64])
