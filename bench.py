@@ -132,6 +132,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the many-maps (batched vs per-map) comparison")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop measurement")
     ap.add_argument("--no-compaction", action="store_true", help="skip the 2^28-element compaction/scan measurement")
+    ap.add_argument("--calib-sweep", action="store_true",
+                    help="only measure the random-gather ceiling (16..256 B reads, 1/4/16 dependent reads) and exit")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one plain 1 GiB device copy (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -153,6 +155,10 @@ def main():
 
     def P(t):
         return C.c_void_p(t.data_ptr())
+
+    if args.calib_sweep:
+        print(json.dumps(calib_sweep(torch, L, dev, S, P)), flush=True)
+        return
 
     tune = C.byref(hn_tune(args.probe_width, 0, 0, 0)) if args.probe_width else None
     sp, alpha = args.sp, args.lf
@@ -351,6 +357,34 @@ def main():
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)
+
+
+def calib_sweep(torch, L, dev, S, P):
+    """hn_calib_random_gather over a 4 GiB buffer: 16 GiB of reads per point,
+    best of 3, for read sizes 16..256 B and 1, 4 or 16 dependent reads per
+    lane group.  GB/s and G reads/s."""
+    from hashinator_amd import _lib
+
+    buf = torch.ones(1 << 29, dtype=torch.int64, device=dev)  # 4 GiB
+    sink = torch.zeros(1, dtype=torch.int64, device=dev)
+    stream = torch.cuda.current_stream()
+    out = {"buffer_bytes": 1 << 32, "read_bytes_per_point": 1 << 34, "points": []}
+    for gran in (16, 32, 64, 128, 256):
+        for chain in (1, 4, 16):
+            n_access = (1 << 34) // gran
+            best = None
+            for rep in range(3):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                _lib.check(L.hn_calib_random_gather(P(buf), 1 << 32, gran, n_access, chain, 99 + rep, P(sink), S),
+                           "calib")
+                e1.record(stream)
+                torch.cuda.synchronize()
+                t = e0.elapsed_time(e1)
+                best = t if best is None else min(best, t)
+            out["points"].append({"gran": gran, "chain": chain, "GBps": round((1 << 34) / (best * 1e-3) / 1e9, 1),
+                                  "Greads_per_s": round(n_access / (best * 1e-3) / 1e9, 2)})
+    return out
 
 
 def run_compaction(torch, L, dev, S, P, buckets, fill):

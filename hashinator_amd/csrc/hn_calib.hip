@@ -18,22 +18,41 @@ __device__ __forceinline__ uint64_t calib_mix(uint64_t x) {
    return x;
 }
 
-// L lanes (gran / 16 bytes) read one aligned chunk together.
+// L lanes (gran / 16 bytes) read one aligned chunk together.  A lane group
+// runs kStreams independent address streams at once (their reads are in
+// flight together); within a stream, `chain` reads follow each other, the
+// next address depending on the data read.  With a power-of-two chunk count
+// the address is a mask, not a 64-bit modulo, so address arithmetic does not
+// bound the read rate.
+constexpr int kStreams = 4;
+
 template <int L>
 __global__ __launch_bounds__(256) void k_random_gather(const uint4* buf, uint64_t chunks, uint64_t n_access, int chain,
                                                        uint64_t seed, uint64_t* sink) {
    const int gl = __lane_id() & (L - 1);
    const uint64_t groups = static_cast<uint64_t>(gridDim.x) * (blockDim.x / L);
-   uint64_t g = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / L;
+   const uint64_t g = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / L;
+   const bool pow2 = (chunks & (chunks - 1)) == 0;
+   const uint64_t mask = chunks - 1;
+   const uint64_t per_iter = static_cast<uint64_t>(kStreams) * chain; // reads per group per iteration
    uint64_t acc = 0;
-   for (uint64_t a = g; a * chain < n_access; a += groups) {
-      uint64_t h = calib_mix(a ^ seed);
+   for (uint64_t a = g; a * per_iter < n_access; a += groups) {
+      uint64_t h[kStreams];
+#pragma unroll
+      for (int t = 0; t < kStreams; ++t) h[t] = calib_mix((a * kStreams + t) ^ seed);
       for (int c = 0; c < chain; ++c) {
-         const uint64_t chunk = h % chunks;
-         const uint4 v = buf[chunk * L + gl];
-         acc += v.x ^ v.w;
-         // the next offset depends on what was read (a dependent chain)
-         h = calib_mix(h + (v.y & 1u));
+         uint4 v[kStreams];
+#pragma unroll
+         for (int t = 0; t < kStreams; ++t) {
+            const uint64_t chunk = pow2 ? (h[t] & mask) : (h[t] % chunks);
+            v[t] = buf[chunk * L + gl];
+         }
+#pragma unroll
+         for (int t = 0; t < kStreams; ++t) {
+            acc += v[t].x ^ v[t].w;
+            // the next offset depends on what was read (a dependent chain)
+            h[t] = calib_mix(h[t] + (v[t].y & 1u));
+         }
       }
    }
    acc = wave_sum(acc);

@@ -41,9 +41,11 @@ template <typename K>
 __host__ __device__ constexpr K tomb_key() {
    return static_cast<K>(~static_cast<K>(0) - 1);
 }
+// Hashmap::hash returns uint32_t (hashinator.h:77-81), so no table has more
+// than 2^32 addressable slots; rehash refuses beyond (hashinator.h:288-290).
 template <typename K>
 __host__ __device__ constexpr int max_size_power() {
-   return sizeof(K) == 4 ? 32 : 63;
+   return 32;
 }
 
 // Fibonacci hash (hashfunctions.h:39-43): 32-bit key, 64-bit product, shift,
@@ -56,6 +58,35 @@ __device__ __forceinline__ uint64_t fib_home(uint32_t key, int sp) {
 __device__ __forceinline__ uint64_t fib_home(uint64_t key, int sp) {
    key ^= key >> (64 - sp);
    return static_cast<uint32_t>((key * 11400714819323198485ull) >> (64 - sp));
+}
+
+// The same hashes in 32-bit words for sp in [1, 32] (the probe loops' hot
+// path).  64-bit key: key ^= key >> (64 - sp) only touches the low word, and
+// bits [64 - sp, 64) of key * phi come from the high word of the product.
+__device__ __forceinline__ uint32_t fib_home32(uint64_t key, int sp) {
+   uint32_t lo = static_cast<uint32_t>(key);
+   const uint32_t hi = static_cast<uint32_t>(key >> 32);
+   lo ^= hi >> (32 - sp);
+   const uint32_t ph = __umulhi(lo, 0x7F4A7C15u) + lo * 0x9E3779B9u + hi * 0x7F4A7C15u;
+   return ph >> (32 - sp);
+}
+__device__ __forceinline__ uint32_t fib_home32(uint32_t key, int sp) {
+   key ^= key >> (32 - sp);
+   const uint32_t plo = key * 2654435769u, phi = __umulhi(key, 2654435769u);
+   return __builtin_amdgcn_alignbit(phi, plo, static_cast<uint32_t>(32 - sp));
+}
+__device__ __forceinline__ uint32_t slot_mask32(int sp) { return sp >= 32 ? 0xFFFFFFFFu : ((1u << sp) - 1u); }
+
+// The G-lane slice of a wave-wide ballot in 32-bit arithmetic (G <= 32).
+template <int G>
+__device__ __forceinline__ uint32_t group_bits32(uint64_t ballot, int gbase) {
+   static_assert(G <= 32, "32-bit slice");
+   const uint32_t half = gbase < 32 ? static_cast<uint32_t>(ballot) : static_cast<uint32_t>(ballot >> 32);
+   if constexpr (G == 32) {
+      return half;
+   } else {
+      return (half >> (gbase & 31)) & ((1u << G) - 1u);
+   }
 }
 
 __host__ __device__ constexpr uint64_t next_overflow(uint64_t cur, uint64_t w) {

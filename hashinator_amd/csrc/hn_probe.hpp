@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "hn_device.hpp"
 
@@ -198,20 +199,43 @@ enum LookupMode { LK_RETRIEVE = 0, LK_ERASE = 1, LK_RESET = 2 };
 // see hn_reorder_*).  Then an occupant whose displacement is smaller than the
 // probe's offset has its home after the probed key's, so the key cannot lie
 // further on and the probe stops as at an EMPTY slot.  TOMBSTONEs never stop.
+//
+// Index arithmetic is 32-bit where it can be (slot indices, masks, ballot
+// slices, the occupant's home): the loop is issue-bound, not memory-bound.
+template <int G>
+using GroupMask = std::conditional_t<(G <= 32), uint32_t, uint64_t>;
+template <int G>
+__device__ __forceinline__ GroupMask<G> group_slice(uint64_t ballot, int gbase) {
+   if constexpr (G <= 32)
+      return group_bits32<G>(ballot, gbase);
+   else
+      return ballot;
+}
+template <typename M>
+__device__ __forceinline__ int first_set(M m) {
+   if constexpr (sizeof(M) == 4)
+      return __ffs(m) - 1;
+   else
+      return __ffsll(static_cast<unsigned long long>(m)) - 1;
+}
+
 template <typename K, int G, bool PAIRS, int MODE>
 __device__ __forceinline__ uint64_t lookup_range(const LookupArgs<K>& a, const int sp, const uint64_t ovf_in,
                                                  const uint64_t len, uint64_t blk, const uint64_t gstride,
                                                  const bool ordered_in = false) {
+   using M = GroupMask<G>;
    constexpr bool ERASE = MODE == LK_ERASE;
    constexpr bool RESET = MODE == LK_RESET;
    constexpr K kEmpty = empty_key<K>();
    constexpr K kTomb = tomb_key<K>();
+   constexpr int GLOG = G == 1 ? 0 : G == 2 ? 1 : G == 4 ? 2 : G == 8 ? 3 : G == 16 ? 4 : G == 32 ? 5 : 6;
    const int lane = __lane_id();
    const int gl = lane & (G - 1);
    const int gbase = lane & ~(G - 1);
-   const uint64_t nsl = 1ull << sp;
-   const uint64_t mask = nsl - 1;
-   const uint64_t W = nsl < static_cast<uint64_t>(G) ? nsl : static_cast<uint64_t>(G);
+   const uint32_t mask = slot_mask32(sp);
+   const uint64_t nsl = static_cast<uint64_t>(mask) + 1;
+   const int wlog = GLOG < sp ? GLOG : sp; // window of W = min(G, 2^sp) slots
+   const uint32_t W = 1u << wlog;
    // Offsets past 2^sp only revisit slots already seen, so the bound is clamped there.
    const uint64_t ovf = RESET ? nsl : ovf_in;
    const uint64_t bound = ovf < nsl ? ovf : nsl;
@@ -221,8 +245,10 @@ __device__ __forceinline__ uint64_t lookup_range(const LookupArgs<K>& a, const i
    K my_key = kEmpty, res_val = 0;
    bool res_hit = false;
    K key = kEmpty;
-   uint64_t pending = 0, base = 0, kw = 0;
-   int64_t d0 = 0;
+   M pending = 0;
+   uint32_t base = 0;
+   uint64_t kw = 0;
+   uint32_t d0 = 0;
    bool active = false;
 
    auto load_block = [&]() {
@@ -238,7 +264,7 @@ __device__ __forceinline__ uint64_t lookup_range(const LookupArgs<K>& a, const i
          }
       }
       res_hit = false;
-      pending = group_bits<G>(__ballot(ok), gbase);
+      pending = group_slice<G>(__ballot(ok), gbase);
       active = pending != 0;
    };
    auto flush_block = [&]() {
@@ -257,10 +283,10 @@ __device__ __forceinline__ uint64_t lookup_range(const LookupArgs<K>& a, const i
       }
    };
    auto start_entry = [&]() {
-      key = __shfl(my_key, gbase + first_bit(pending), kWave);
-      const uint64_t home = fib_home(key, sp) & mask;
-      base = home & ~(W - 1);
-      d0 = static_cast<int64_t>(home - base);
+      key = __shfl(my_key, gbase + first_set(pending), kWave);
+      const uint32_t home = fib_home32(key, sp) & mask;
+      base = home & ~(W - 1u);
+      d0 = home - base;
       kw = 0;
    };
 
@@ -268,21 +294,22 @@ __device__ __forceinline__ uint64_t lookup_range(const LookupArgs<K>& a, const i
    if (active) start_entry();
    while (__ballot(active)) {
       if (active) {
-         const uint64_t pos = kw * W + static_cast<uint64_t>(gl);
-         const int64_t off = static_cast<int64_t>(pos) - d0;
-         const bool valid = static_cast<uint64_t>(gl) < W && off >= 0 && static_cast<uint64_t>(off) < bound;
-         Slot<K>* sl = a.buckets + ((base + pos) & mask);
+         const uint64_t pos = (kw << wlog) + static_cast<uint64_t>(gl);
+         const bool ahead = pos >= d0; // at or after the home slot
+         const uint64_t off = pos - d0;
+         const bool valid = static_cast<uint32_t>(gl) < W && ahead && off < bound;
+         const uint32_t slot = (base + static_cast<uint32_t>(pos)) & mask;
+         Slot<K>* sl = a.buckets + slot;
          const Slot<K> s = *sl;
-         const uint64_t bm = group_bits<G>(__ballot(valid && s.key == key), gbase);
+         const M bm = group_slice<G>(__ballot(valid && s.key == key), gbase);
          bool stop = valid && s.key == kEmpty;
          if (ordered && valid && !stop && s.key != key && s.key != kTomb) {
-            const uint64_t slot = (base + pos) & mask;
-            const uint64_t d_occ = (slot - (fib_home(s.key, sp) & mask)) & mask;
-            stop = d_occ < static_cast<uint64_t>(off);
+            const uint32_t d_occ = (slot - fib_home32(s.key, sp)) & mask;
+            stop = d_occ < off;
          }
-         const uint64_t be = RESET ? 0 : group_bits<G>(__ballot(stop), gbase);
-         const int fm = bm ? first_bit(bm) : G;
-         const int fe = be ? first_bit(be) : G;
+         const M be = RESET ? M(0) : group_slice<G>(__ballot(stop), gbase);
+         const int fm = bm ? first_set(bm) : G;
+         const int fe = be ? first_set(be) : G;
          bool done = true;
          if (fm < fe) {
             if constexpr (ERASE) {
@@ -291,13 +318,13 @@ __device__ __forceinline__ uint64_t lookup_range(const LookupArgs<K>& a, const i
                if (gl == fm) atomic_cas(&sl->key, key, kEmpty);
             } else {
                const K v = __shfl(s.val, gbase + fm, kWave);
-               if (gl == first_bit(pending)) {
+               if (gl == first_set(pending)) {
                   res_val = v;
                   res_hit = true;
                }
             }
          } else {
-            const uint64_t covered = (kw + 1) * W - static_cast<uint64_t>(d0);
+            const uint64_t covered = ((kw + 1) << wlog) - d0;
             if (!be && covered < bound) {
                ++kw;
                done = false;

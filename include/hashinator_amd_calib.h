@@ -16,9 +16,10 @@ extern "C" {
 
 /* Random gather: n_access reads of `gran` bytes (16, 32, 64, 128 or 256) at
  * uniformly random gran-aligned offsets of buf[0, bytes), each read issued by
- * gran/16 lanes as one coalesced request.  `chain` > 1 makes every lane group
- * issue `chain` dependent reads (the next offset depends on the data read),
- * which models a probe sequence.  The reads are folded into *sink (device). */
+ * gran/16 lanes as one coalesced request.  Every lane group keeps 4
+ * independent address streams in flight; `chain` > 1 makes each stream issue
+ * `chain` dependent reads (the next offset depends on the data read), which
+ * models a probe sequence.  The reads are folded into *sink (device). */
 HN_API int hn_calib_random_gather(const void* buf, size_t bytes, int gran, size_t n_access, int chain, uint64_t seed,
                                   uint64_t* sink, hn_stream_t s);
 
