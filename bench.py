@@ -253,11 +253,14 @@ def main():
     ret_bytes = alg_bytes_retrieve(n, n, n / N, ordered=ordered)
     ins_bytes = alg_bytes_insert(n, n / N)
     achieved = ret_bytes / (t_ret * 1e-3) / 1e9
-    traffic = read_traffic("retrieve_u64_sp28_lf09_rh_traffic.md" if ordered else
-                           "retrieve_u64_sp28_lf09_traffic.md")
+    # measured for the default (lane) kernel only
+    traffic = None if args.probe_width else read_traffic("retrieve_u64_sp28_lf09_rh_traffic.md" if ordered else
+                                                         "retrieve_u64_sp28_lf09_traffic.md")
+    kname = ("k_lookup_lane<u64> (retrieve, one key per lane" if not args.probe_width else
+             f"k_lookup<u64,{args.probe_width}> (retrieve, {args.probe_width} lanes per key")
     roofline = {
         "bound": "hbm",
-        "kernel": "k_lookup<u64> (retrieve, " + ("Robin Hood ordered table)" if ordered else "arrival order)"),
+        "kernel": kname + (", Robin Hood ordered table)" if ordered else ", arrival order)"),
         "achieved": round(achieved, 1),
         "peak": PEAK_HBM_GBS,
         "unit": "GB/s",
@@ -297,7 +300,6 @@ def main():
             best = t if best is None else min(best, t)
         ceil[gran] = N * 16 / (best * 1e-3) / 1e9
     roofline["random_ceiling_GBps"] = {"64B": round(ceil[64], 1), "128B": round(ceil[128], 1)}
-    roofline["frac_of_random_64B_ceiling"] = round(achieved / ceil[64], 4)
     if traffic:
         roofline["traffic_GBps"] = round(traffic / (t_ret * 1e-3) / 1e9, 1)
         roofline["traffic_frac_of_random_128B_ceiling"] = round(traffic / (t_ret * 1e-3) / 1e9 / ceil[128], 4)
@@ -362,7 +364,8 @@ def main():
 def calib_sweep(torch, L, dev, S, P):
     """hn_calib_random_gather over a 4 GiB buffer: 16 GiB of reads per point,
     best of 3, for read sizes 16..256 B and 1, 4 or 16 dependent reads per
-    lane group.  GB/s and G reads/s."""
+    lane group; then hn_calib_random_gather_lane (one lane per read) for
+    16..128 B.  GB/s and G reads/s."""
     from hashinator_amd import _lib
 
     buf = torch.ones(1 << 29, dtype=torch.int64, device=dev)  # 4 GiB
@@ -384,6 +387,20 @@ def calib_sweep(torch, L, dev, S, P):
                 best = t if best is None else min(best, t)
             out["points"].append({"gran": gran, "chain": chain, "GBps": round((1 << 34) / (best * 1e-3) / 1e9, 1),
                                   "Greads_per_s": round(n_access / (best * 1e-3) / 1e9, 2)})
+    # one lane reads the whole chunk with gran/16 consecutive 16-byte loads
+    for gran in (16, 32, 64, 128):
+        n_access = (1 << 34) // gran
+        best = None
+        for rep in range(3):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            _lib.check(L.hn_calib_random_gather_lane(P(buf), 1 << 32, gran, n_access, 77 + rep, P(sink), S), "calib")
+            e1.record(stream)
+            torch.cuda.synchronize()
+            t = e0.elapsed_time(e1)
+            best = t if best is None else min(best, t)
+        out["points"].append({"gran": gran, "per_lane": True, "GBps": round((1 << 34) / (best * 1e-3) / 1e9, 1),
+                              "Greads_per_s": round(n_access / (best * 1e-3) / 1e9, 2)})
     return out
 
 

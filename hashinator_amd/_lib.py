@@ -136,6 +136,7 @@ SIGNATURES = {
     "hn_version": (C.c_char_p, []),
     # include/hashinator_amd_calib.h
     "hn_calib_random_gather": (I, [P, S, I, S, I, C.c_uint64, P, P]),
+    "hn_calib_random_gather_lane": (I, [P, S, I, S, C.c_uint64, P, P]),
 }
 
 _LIB = None

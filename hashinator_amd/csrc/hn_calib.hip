@@ -59,9 +59,57 @@ __global__ __launch_bounds__(256) void k_random_gather(const uint4* buf, uint64_
    if (__lane_id() == 0 && acc == 0x123456789ull) *sink = acc; // practically never; keeps the loads
 }
 
+// One lane reads a whole chunk on its own: NV consecutive 16-byte loads,
+// issued back to back, for each of kLaneStreams independent chunks.  This is
+// the access of a probe that keeps one key per lane and scans a window of
+// slots in registers; the rate tells whether the L1 merges the NV loads of
+// one line into one request.
+constexpr int kLaneStreams = 2;
+
+template <int NV>
+__global__ __launch_bounds__(256) void k_random_gather_lane(const uint4* buf, uint64_t chunks, uint64_t n_access,
+                                                            uint64_t seed, uint64_t* sink) {
+   const uint64_t lanes = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+   const uint64_t g = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+   const uint64_t mask = chunks - 1; // chunks is a power of two
+   uint64_t acc = 0;
+   for (uint64_t a = g; a * kLaneStreams < n_access; a += lanes) {
+      uint4 v[kLaneStreams][NV];
+#pragma unroll
+      for (int t = 0; t < kLaneStreams; ++t) {
+         const uint64_t chunk = calib_mix((a * kLaneStreams + t) ^ seed) & mask;
+#pragma unroll
+         for (int j = 0; j < NV; ++j) v[t][j] = buf[chunk * NV + j];
+      }
+#pragma unroll
+      for (int t = 0; t < kLaneStreams; ++t)
+#pragma unroll
+         for (int j = 0; j < NV; ++j) acc += v[t][j].x ^ v[t][j].w;
+   }
+   acc = wave_sum(acc);
+   if (__lane_id() == 0 && acc == 0x123456789ull) *sink = acc;
+}
+
 } // namespace hn
 
 using namespace hn;
+
+extern "C" int hn_calib_random_gather_lane(const void* buf, size_t bytes, int gran, size_t n_access, uint64_t seed,
+                                           uint64_t* sink, hn_stream_t s) {
+   if (!buf || !sink) return HN_EINVAL;
+   const uint64_t chunks = bytes / static_cast<uint64_t>(gran);
+   if (chunks == 0 || (chunks & (chunks - 1)) != 0) return HN_EINVAL;
+   const uint32_t blocks = static_cast<uint32_t>(cu_count()) * 8;
+   const uint4* b = static_cast<const uint4*>(buf);
+   switch (gran) {
+   case 16: hipLaunchKernelGGL(k_random_gather_lane<1>, dim3(blocks), dim3(256), 0, s, b, chunks, n_access, seed, sink); break;
+   case 32: hipLaunchKernelGGL(k_random_gather_lane<2>, dim3(blocks), dim3(256), 0, s, b, chunks, n_access, seed, sink); break;
+   case 64: hipLaunchKernelGGL(k_random_gather_lane<4>, dim3(blocks), dim3(256), 0, s, b, chunks, n_access, seed, sink); break;
+   case 128: hipLaunchKernelGGL(k_random_gather_lane<8>, dim3(blocks), dim3(256), 0, s, b, chunks, n_access, seed, sink); break;
+   default: return HN_EINVAL;
+   }
+   return static_cast<int>(hipGetLastError());
+}
 
 extern "C" int hn_calib_random_gather(const void* buf, size_t bytes, int gran, size_t n_access, int chain,
                                       uint64_t seed, uint64_t* sink, hn_stream_t s) {

@@ -88,6 +88,49 @@ __global__ __launch_bounds__(1024) void k_lookup(LookupArgs<K> a) {
    }
 }
 
+// Lane-window lookups (one key per lane, see lane_lookup): one wave per
+// block (no block-level cooperation; single-wave blocks pack the LDS of a CU
+// best), each with its own staging and its own contiguous share of the
+// queries.  Tables smaller than one 128-byte window (and a zero probe bound)
+// take the group loop with one lane per key.
+constexpr int kLaneBlock = 64;
+template <typename K, bool PAIRS, int MODE>
+__global__ __launch_bounds__(kLaneBlock) void k_lookup_lane(LookupArgs<K> a) {
+   constexpr int kWaves = kLaneBlock / kWave;
+   constexpr int kRing = MODE == LK_RETRIEVE ? kLaneRing : 1;
+   __shared__ __align__(16) char win_s[kWaves][kLaneWaveBytes];
+   __shared__ uint32_t wb_s[kWaves][kWave];
+   __shared__ K ring_v[kWaves][kRing];
+   __shared__ uint8_t ring_s[kWaves][kRing];
+   const int sp = a.info->size_power;
+   const bool sp_ok = sp >= 1 && sp <= max_size_power<K>();
+   const uint64_t ovf = a.info->current_max_bucket_overflow;
+   const bool ordered = (a.info->reserved1 & kInfoOrdered) != 0;
+   uint64_t erased = 0;
+   if (sp_ok) {
+      if (sp >= LaneWin<K>::kLog && ovf > 0) {
+         const int wv = threadIdx.x / kWave;
+         const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * (kLaneBlock / kWave);
+         const uint64_t w = static_cast<uint64_t>(blockIdx.x) * (kLaneBlock / kWave) + wv;
+         const uint64_t per = a.len / nwaves, rem = a.len % nwaves;
+         const uint64_t q0 = w * per + (w < rem ? w : rem);
+         const uint64_t q1 = q0 + per + (w < rem ? 1 : 0);
+         erased = lane_lookup<K, PAIRS, MODE>(a, sp, ovf, ordered, q0, q1, win_s[wv], wb_s[wv], ring_v[wv], ring_s[wv]);
+      } else {
+         const uint64_t gstride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+         const uint64_t blk = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+         erased = lookup_range<K, 1, PAIRS, MODE>(a, sp, ovf, a.len, blk, gstride, ordered);
+      }
+   }
+   if constexpr (MODE == LK_ERASE) {
+      erased = wave_sum(erased);
+      if (__lane_id() == 0 && erased) {
+         atomic_add_relaxed(&a.info->tombstone_counter, erased);
+         __hip_atomic_fetch_sub(&a.info->fill, erased, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+   }
+}
+
 // reset_all_to_empty (kernels_AMD.h:26-43): keys -> EMPTY, values untouched.
 template <typename K>
 __global__ __launch_bounds__(256) void k_reset_all(Slot<K>* b, hn_info* info, uint64_t len) {
@@ -321,6 +364,30 @@ int launch_insert(InsertArgs<K> a, uint64_t grid_len, hipStream_t s, const hn_tu
 
 template <typename K, bool PAIRS, int MODE>
 int launch_lookup(LookupArgs<K> a, hipStream_t s, const hn_tune* t, OpKind op) {
+   if constexpr (MODE != LK_RESET) {
+      if (!t || t->probe_width == 0) { // automatic: lane windows
+         // block_size does not apply: the lane kernel runs one wave per block
+         uint64_t need = (a.len + 255) / 256; // at least 256 queries per wave
+         if (need == 0) need = 1;
+         // one resident wave per block slot: the waves split the queries
+         // statically, so a block that waits for a free slot would finish late
+         static int per_cu = 0;
+         if (per_cu == 0) {
+            int nb = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_lookup_lane<K, PAIRS, MODE>, kLaneBlock, 0) !=
+                    hipSuccess ||
+                nb < 1)
+               nb = 8;
+            per_cu = nb;
+         }
+         uint64_t cap = (t && t->grid_blocks > 0) ? static_cast<uint64_t>(t->grid_blocks)
+                                                  : static_cast<uint64_t>(cu_count()) * per_cu;
+         if (cap > (1u << 20)) cap = 1u << 20;
+         const uint32_t blocks = static_cast<uint32_t>(need < cap ? need : cap);
+         hipLaunchKernelGGL((k_lookup_lane<K, PAIRS, MODE>), dim3(blocks), dim3(kLaneBlock), 0, s, a);
+         return static_cast<int>(hipGetLastError());
+      }
+   }
    Launch L;
    int rc = resolve_launch<K>(t, op, a.len, &L);
    if (rc) return rc;

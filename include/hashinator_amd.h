@@ -68,6 +68,10 @@ enum { HN_STATUS_SUCCESS = 0, HN_STATUS_FAIL = 1, HN_STATUS_INVALID = 2 };
 /* Launch tuning.  NULL means "use the tuned defaults".  probe_width is the
  * number of lanes that probe one key together (1, 2, 4, 8, 16, 32 or 64; a
  * window of probe_width consecutive, aligned slots is read per step).
+ * probe_width 0 (and NULL) selects the default: for inserts 8 lanes (u64) or
+ * 16 (u32); for retrieve and erase the lane kernel, one key per lane, whose
+ * 128-byte windows are loaded by 8 lanes together and handed to their lanes
+ * through LDS (it runs one wave per block and ignores block_size).
  * block_size is threads per workgroup (64..1024, multiple of 64); grid_blocks
  * 0 means derived from len and the device's CU count. */
 typedef struct hn_tune {

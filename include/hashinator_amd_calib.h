@@ -23,6 +23,13 @@ extern "C" {
 HN_API int hn_calib_random_gather(const void* buf, size_t bytes, int gran, size_t n_access, int chain, uint64_t seed,
                                   uint64_t* sink, hn_stream_t s);
 
+/* Per-lane random gather: n_access reads of `gran` bytes (16, 32, 64 or 128)
+ * at uniformly random gran-aligned offsets of buf[0, bytes), each read by ONE
+ * lane as gran/16 consecutive 16-byte loads; bytes/gran must be a power of
+ * two.  Every lane keeps 2 chunks in flight. */
+HN_API int hn_calib_random_gather_lane(const void* buf, size_t bytes, int gran, size_t n_access, uint64_t seed,
+                                       uint64_t* sink, hn_stream_t s);
+
 #ifdef __cplusplus
 }
 #endif
