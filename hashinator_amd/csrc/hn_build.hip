@@ -274,6 +274,18 @@ __global__ __launch_bounds__(kPartBS) void k_part_scatter(PartIn<K> in, uint64_t
    const uint32_t d0 = threadIdx.x * 2; // this thread's two digits in the digit loops
    for (uint32_t d = d0; d < d0 + 2 && d < D; ++d) goff_s[d] = offs[static_cast<uint64_t>(d) * nblocks + blockIdx.x];
    const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kPartSuper;
+   // The pairs of tile `sub` are loaded while tile sub-1 is written out.
+   Slot<K> el[ROUNDS];
+   auto load_tile = [&](int sub) {
+      const uint64_t w0 = base + static_cast<uint64_t>(sub) * kPartTile + static_cast<uint64_t>(wave) * (kPartTile / NW) +
+                          static_cast<uint64_t>(lane);
+#pragma unroll
+      for (int r = 0; r < ROUNDS; ++r) {
+         const uint64_t e = w0 + static_cast<uint64_t>(r) * kWave;
+         if (e < n) el[r] = part_pair<K, SRC>(in, e);
+      }
+   };
+   load_tile(0);
    for (int sub = 0; sub < kPartSub; ++sub) {
       const uint64_t t0 = base + static_cast<uint64_t>(sub) * kPartTile;
       if (t0 >= n) break;
@@ -282,7 +294,6 @@ __global__ __launch_bounds__(kPartBS) void k_part_scatter(PartIn<K> in, uint64_t
          for (int w = 0; w < NW; ++w) wcnt_s[w][d] = 0;
       }
       __syncthreads();
-      Slot<K> el[ROUNDS];
       uint32_t dg[ROUNDS], rk[ROUNDS];
       bool ok[ROUNDS];
       const uint64_t w0 = t0 + static_cast<uint64_t>(wave) * (kPartTile / NW) + lane;
@@ -292,7 +303,6 @@ __global__ __launch_bounds__(kPartBS) void k_part_scatter(PartIn<K> in, uint64_t
          dg[r] = 0;
          ok[r] = false;
          if (e < n) {
-            el[r] = part_pair<K, SRC>(in, e);
             ok[r] = !FILTER || live_key(el[r].key);
             dg[r] = (region_of(el[r].key, sp, rlog) >> dshift) & dmask;
          }
@@ -350,6 +360,7 @@ __global__ __launch_bounds__(kPartBS) void k_part_scatter(PartIn<K> in, uint64_t
             dig_s[p] = static_cast<uint16_t>(dg[r]);
          }
       }
+      if (sub + 1 < kPartSub) load_tile(sub + 1);
       __syncthreads();
       uint32_t cnt = 0; // pairs of this tile that are written: the digit scan's total
       for (int w = 0; w < NW; ++w) cnt += wsum_s[w];
@@ -486,14 +497,19 @@ static_assert(sizeof(int32_t) * kRegionCap <= 2 * sizeof(uint32_t) * (kRegionMax
 // occurrence of a repeated key.  Thread t owns keys t + k*kRegionBS; ff[k] =
 // its sorted index among the survivors, or -1.  Leaves v[f] = home offset - f.
 // Returns the number of survivors.  Called by the whole workgroup.
+//
+// A counting sort by home slot puts the keys, with their input indices, in
+// home order in LDS; a key's rank among its home's keys is then one pass over
+// that home's few entries (0.9 others on average at load factor 0.9), which
+// also spots repeated keys.  Without repeats in the region (the block agrees
+// with one __syncthreads_or) every key survives and the sorted index is
+// home start + rank; otherwise the survivors are counted per home and ranked
+// again among survivors.
 template <typename K>
 __device__ uint32_t region_sort(const Slot<K>* S, uint64_t lo, uint32_t m, int sp, uint64_t R, RegionLds<K>& L,
                                 K* kk, K* vv, uint32_t* uu, int32_t* ff) {
    __syncthreads(); // the previous user of L is done
-   for (uint32_t i = threadIdx.x; i <= R; i += kRegionBS) {
-      L.u2.c.cnt[i] = 0;
-      L.u2.c.scnt[i] = 0;
-   }
+   for (uint32_t i = threadIdx.x; i <= R; i += kRegionBS) L.u2.c.cnt[i] = 0;
    __syncthreads();
    uint32_t sub[kRegionIPT];
 #pragma unroll
@@ -504,45 +520,74 @@ __device__ uint32_t region_sort(const Slot<K>* S, uint64_t lo, uint32_t m, int s
          const Slot<K> s = S[lo + e];
          kk[k] = s.key;
          vv[k] = s.val;
-         L.key[e] = s.key;
          uu[k] = static_cast<uint32_t>((fib_home(s.key, sp) & ((1ull << sp) - 1)) & (R - 1));
          sub[k] = atomicAdd(&L.u2.c.cnt[uu[k]], 1u);
       }
    }
    __syncthreads();
    lds_exclusive_scan<kRegionBS>(L.u2.c.cnt, static_cast<uint32_t>(R) + 1, L.wsum);
-#pragma unroll
-   for (int k = 0; k < kRegionIPT; ++k) {
-      const uint32_t e = threadIdx.x + k * kRegionBS;
-      if (e < m) L.order[L.u2.c.cnt[uu[k]] + sub[k]] = static_cast<uint16_t>(e);
-   }
-   __syncthreads();
-   // Dropped: a later key of the batch (higher index) is the same key.
+   // keys and input indices in home order
 #pragma unroll
    for (int k = 0; k < kRegionIPT; ++k) {
       const uint32_t e = threadIdx.x + k * kRegionBS;
       if (e < m) {
-         uint8_t dup = 0;
-         for (uint32_t j = L.u2.c.cnt[uu[k]]; j < L.u2.c.cnt[uu[k] + 1]; ++j) {
-            const uint32_t o = L.order[j];
-            dup |= (L.key[o] == kk[k] && o > e) ? 1 : 0;
-         }
-         L.u1.dup[e] = dup;
-         if (!dup) atomicAdd(&L.u2.c.scnt[uu[k]], 1u);
+         const uint32_t pos = L.u2.c.cnt[uu[k]] + sub[k];
+         L.key[pos] = kk[k];
+         L.order[pos] = static_cast<uint16_t>(e);
       }
    }
    __syncthreads();
-   const uint32_t ms = lds_exclusive_scan<kRegionBS>(L.u2.c.scnt, static_cast<uint32_t>(R) + 1, L.wsum);
+   // rank among the home's keys; a later copy of the same key drops this one
+   uint32_t dupm = 0; // bit k: key k is dropped
+   int repeats = 0;
 #pragma unroll
    for (int k = 0; k < kRegionIPT; ++k) {
       const uint32_t e = threadIdx.x + k * kRegionBS;
-      if (e < m && !L.u1.dup[e]) {
-         uint32_t rank = 0;
-         for (uint32_t j = L.u2.c.cnt[uu[k]]; j < L.u2.c.cnt[uu[k] + 1]; ++j) {
-            const uint32_t o = L.order[j];
-            rank += (!L.u1.dup[o] && L.key[o] < kk[k]) ? 1u : 0u;
+      if (e < m) {
+         const uint32_t j0 = L.u2.c.cnt[uu[k]], j1 = L.u2.c.cnt[uu[k] + 1];
+         int32_t rank = 0;
+         for (uint32_t j = j0; j < j1; ++j) {
+            const K kj = L.key[j];
+            const uint32_t oj = L.order[j];
+            rank += kj < kk[k] ? 1 : 0;
+            if (kj == kk[k] && oj != e) {
+               repeats = 1;
+               if (oj > e) dupm |= 1u << k;
+            }
          }
-         ff[k] = static_cast<int32_t>(L.u2.c.scnt[uu[k]] + rank);
+         ff[k] = rank;
+      }
+   }
+   uint32_t ms;
+   if (!__syncthreads_or(repeats)) {
+      ms = m;
+#pragma unroll
+      for (int k = 0; k < kRegionIPT; ++k)
+         if (ff[k] >= 0) ff[k] += static_cast<int32_t>(L.u2.c.cnt[uu[k]]);
+   } else {
+      for (uint32_t i = threadIdx.x; i <= R; i += kRegionBS) L.u2.c.scnt[i] = 0;
+#pragma unroll
+      for (int k = 0; k < kRegionIPT; ++k) {
+         const uint32_t e = threadIdx.x + k * kRegionBS;
+         if (e < m) L.u1.dup[e] = (dupm >> k) & 1u;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < kRegionIPT; ++k)
+         if (ff[k] >= 0 && !((dupm >> k) & 1u)) atomicAdd(&L.u2.c.scnt[uu[k]], 1u);
+      __syncthreads();
+      ms = lds_exclusive_scan<kRegionBS>(L.u2.c.scnt, static_cast<uint32_t>(R) + 1, L.wsum);
+#pragma unroll
+      for (int k = 0; k < kRegionIPT; ++k) {
+         if (ff[k] < 0) continue;
+         if ((dupm >> k) & 1u) {
+            ff[k] = -1;
+            continue;
+         }
+         int32_t rank = 0;
+         for (uint32_t j = L.u2.c.cnt[uu[k]]; j < L.u2.c.cnt[uu[k] + 1]; ++j)
+            rank += (!L.u1.dup[L.order[j]] && L.key[j] < kk[k]) ? 1 : 0;
+         ff[k] = static_cast<int32_t>(L.u2.c.scnt[uu[k]]) + rank;
       }
    }
    __syncthreads(); // count tables dead from here; v takes their place
@@ -553,8 +598,6 @@ __device__ uint32_t region_sort(const Slot<K>* S, uint64_t lo, uint32_t m, int s
    return ms;
 }
 
-// Carry function of a sorted region: carry out = max(0, ms - R + max(M, c)),
-// M = max_f (home offset - f).
 template <typename K>
 __device__ RegionSum region_summary(RegionLds<K>& L, uint32_t ms, uint64_t R, const int32_t* ff) {
    int32_t M = INT32_MIN;

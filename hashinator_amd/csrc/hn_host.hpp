@@ -18,10 +18,16 @@ struct Launch {
 // Compute units of the current device (cached per device ordinal).
 int cu_count();
 
-// Default probe width: one 128-byte window per step (8 slots of 16 B for
-// u64 maps, 16 slots of 8 B for u32 maps), the fastest on MI355X at LF 0.9.
+// Default probe width of the group kernels.  Inserts: 4 lanes (windows of 64
+// B for u64, 32 B for u32), measured fastest on MI355X for tables beyond the
+// caches at LF 0.5-0.9 (u64 at 2^28 slots, LF 0.9: 21.8 ms against 22.9 ms
+// with 8 lanes and 30.9 ms with 16; u32 at 2^26: 5.6 ms against 7.5 ms with
+// 16).  Group lookups (reset_to_empty, and any caller that asks for the
+// group path): one 128-byte window per step.  Retrieve and erase default to
+// the lane kernel, which has no width.
 template <typename K>
-constexpr int default_probe_width(OpKind) {
+constexpr int default_probe_width(OpKind op) {
+   if (op == OP_INSERT || op == OP_REINSERT) return 4;
    return sizeof(K) == 8 ? 8 : 16;
 }
 
