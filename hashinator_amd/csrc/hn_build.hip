@@ -52,13 +52,13 @@ namespace hn {
 
 constexpr int kRegionLog = 11;   // 2048 slots per region
 constexpr int kPartBS = 256;     // partition workgroup
-constexpr int kPartIPT = 8;      // pairs per thread and tile
+constexpr int kPartIPT = 12;     // pairs per thread and tile (3072-pair tiles: ~6 pairs per digit run)
 constexpr int kPartTile = kPartBS * kPartIPT;
 constexpr int kPartSub = 8;      // tiles per partition block
 constexpr int kPartSuper = kPartTile * kPartSub;
 constexpr int kMaxDigitBits = 9; // 512 digits per pass at most
-constexpr int kRegionBS = 256;
-constexpr int kRegionIPT = 10;
+constexpr int kRegionBS = 512;
+constexpr int kRegionIPT = 5;
 constexpr int kRegionCap = kRegionBS * kRegionIPT; // keys per region held in LDS
 
 struct BuildHeader {
@@ -247,7 +247,7 @@ __global__ __launch_bounds__(kPartBS) void k_part_count(PartIn<K> in, uint64_t n
 
 // Stable multisplit.  A block walks its kPartSub tiles in order, keeping the
 // running output offset of every digit in LDS.  In a tile, wave w ranks the
-// pairs [512w, 512w + 512) in eight rounds of 64: lanes with equal digits are
+// pairs [768w, 768w + 768) in twelve rounds of 64: lanes with equal digits are
 // found with one ballot per digit bit, and the wave's own LDS counters carry
 // the earlier rounds (no block barrier per round).  Per-wave counts then give
 // each pair's rank among equal digits in tile order; the tile is staged in
