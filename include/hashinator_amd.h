@@ -254,7 +254,9 @@ HN_API int hn_reorder_u64(void* buckets, hn_info* info, int size_power, void* sc
  * an LDS sort per region and a carry scan (about 10 streaming passes' worth
  * of the pair array).  Needs len < 2^size_power and `scratch` of
  * hn_build_scratch_bytes_*(size_power, len) bytes; the table's previous slot
- * contents are overwritten.  When the table is not empty the call sets
+ * contents are overwritten, every slot: the bucket memory of a fresh table
+ * need not be initialised (hn_table_init_*) first, only its Info
+ * (hn_info_init).  When the table is not empty the call sets
  * err = INVALID and leaves it untouched (insert with hn_insert_* instead).
  * When one 2048-slot region receives more than 2560 keys (strongly skewed
  * keys) the table is filled by the probing insert instead, on the device:

@@ -176,6 +176,35 @@ def test_build_overfull_region_falls_back_on_device(gpu, bits):
 
 
 @pytest.mark.parametrize("bits", [32, 64])
+@pytest.mark.parametrize("case", ["uniform", "overfull"])
+def test_build_needs_no_initialised_table(gpu, bits, case):
+    # Every slot is written (keys and EMPTY slots; the overfull fallback empties
+    # the table itself), so bucket memory holding garbage, keys included, gives
+    # the image an initialised table gives.
+    sp = 14
+    if case == "uniform":
+        keys = keyset(int(0.9 * (1 << sp)), bits, seed=41)
+    else:
+        keys = concentrated_keys(bits, sp, 3000, 2048, seed=6)
+    vals = (keys ^ NDT[bits](0x77)).astype(NDT[bits])
+    ref_img, ref_inf, rc = run_build(bits, sp, keys, vals)
+    assert rc == 0
+    rng = np.random.default_rng(5)
+    junk = rng.integers(0, 2**62, size=(1 << sp) * 2, dtype=np.uint64).astype(NDT[bits])
+    junk[0:64:2] = keys[:32]  # live-looking keys too
+    table = dev(junk, bits).reshape(-1, 2)
+    img, inf, rc = run_build(bits, sp, keys, vals, table=table)
+    assert rc == 0
+    if case == "uniform":
+        assert np.array_equal(img, ref_img)
+        assert np.array_equal(inf, ref_inf)
+    else:  # the probing fallback places in arrival order: same live set and occupancy
+        assert np.array_equal(live_pairs(img, bits), live_pairs(ref_img, bits))
+        assert np.array_equal(occupancy(img, bits), occupancy(ref_img, bits))
+        assert int(inf[1]) == keys.size and (int(inf[4]) & 0xFFFFFFFF) == 0
+
+
+@pytest.mark.parametrize("bits", [32, 64])
 @pytest.mark.parametrize("sp,lf", [(10, 0.9), (14, 0.9), (20, 0.95), (21, 0.85)])
 def test_clean_rebuild_image_exact(gpu, bits, sp, lf):
     """clean_tombstones with whole-table scratch rebuilds the live slots: the
