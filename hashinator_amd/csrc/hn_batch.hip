@@ -1,4 +1,4 @@
-// hn_batch.hip -- many independent maps in one persistent launch.
+// hn_batch.hip -- many independent maps in a few persistent launches.
 //
 // No counterpart kernel in the reference (its API is one launch per map per
 // operation, hashers.h:46-163); the per-map semantics are the reference's:
@@ -6,12 +6,14 @@
 // retrieve (retrieve_kernel 647-697), erase (delete_kernel 412-505), then
 // clean_tombstones (hashinator.h:1261-1315).
 //
-// One workgroup owns one map at a time and runs the requested phases on it
-// back to back, separated by workgroup barriers with workgroup-scope fences
-// (no other workgroup touches the map during the launch);
-// maps are handed out by an atomic ticket in the order the caller lists them
-// (largest first balances the grid).  The probe loops are the single-map
-// ones (hn_probe.hpp), so a map's results are those of the plain calls.
+// Insert, and erase + clean: one workgroup owns one map at a time and runs
+// the phases on it back to back, separated by workgroup barriers with
+// workgroup-scope fences (no other workgroup touches the map during the
+// launch); maps are handed out by an atomic ticket in the order the caller
+// lists them (largest first balances the grid).  Retrieve runs between the
+// two as its own launch, one key per lane over units of every map's queries
+// (k_batch_retrieve_lane).  The probe loops are the single-map ones
+// (hn_probe.hpp), so a map's results are those of the plain calls.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -197,6 +199,86 @@ __global__ __launch_bounds__(1024) void k_batch(BatchArgs<K> a) {
 
 __global__ void k_zero_u32(uint32_t* p) { *p = 0; }
 
+// The next work unit of a wave, wave-uniform (a scalar register).  Every
+// lane adds 1, so a wave takes kWave tickets at once (one atomic after the
+// compiler folds the wave's adds) and its unit is the first ticket / kWave.
+// No lane-0-only branch: with one, and the unit taken from a shuffle, the
+// compiler treats the unit as divergent and restructures the work loop
+// around the whole-wave probe loops so that a skipped unit does not take a
+// new ticket (the loop never ends).
+__device__ __forceinline__ uint32_t next_unit(uint32_t* ticket) {
+   const uint32_t t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+   return __builtin_amdgcn_readfirstlane(t) / kWave;
+}
+
+// Longest query slice of any map; the unit ticket starts at 0.
+__global__ __launch_bounds__(256) void k_batch_prep(const hn_batch_map* maps, uint32_t n_maps, uint64_t* max_qry,
+                                                    uint32_t* t_qry) {
+   uint64_t mq = 0;
+   for (uint32_t m = threadIdx.x; m < n_maps; m += blockDim.x) mq = maps[m].qry_len > mq ? maps[m].qry_len : mq;
+   mq = wave_max(mq);
+   __shared__ uint64_t wq[256 / kWave];
+   if (__lane_id() == 0) wq[threadIdx.x / kWave] = mq;
+   __syncthreads();
+   if (threadIdx.x == 0) {
+      uint64_t b = 0;
+      for (int w = 0; w < 256 / kWave; ++w) b = wq[w] > b ? wq[w] : b;
+      *max_qry = b;
+      *t_qry = 0;
+   }
+}
+
+// ---------------------------------------------------------- retrieve phase
+//
+// The retrieve phase runs on its own, one key per lane (lane_lookup): every
+// map's queries are cut into units of kBatchUnit; unit u is chunk u % umax of
+// map u / umax (umax: the most units any map has), so units come in map order
+// (largest maps first) and a wave takes the next one by an atomic ticket.
+// Units past the end of a map's queries are skipped.  Results per map are
+// those of the plain retrieve: maps are independent, and the phases still run
+// insert -> retrieve -> erase -> clean for each of them.
+constexpr uint64_t kBatchUnit = 4096;
+
+struct BatchScratch {
+   uint32_t ticket;     // map ticket of k_batch
+   uint32_t qry_ticket; // unit ticket of the retrieve phase (kWave tickets per unit)
+   uint64_t max_qry;    // longest query slice of any map
+};
+static_assert(sizeof(BatchScratch) <= 256, "fits hn_batch_scratch_bytes");
+
+template <typename K>
+__global__ __launch_bounds__(kWave) void k_batch_retrieve_lane(BatchArgs<K> a, BatchScratch* sc) {
+   __shared__ __align__(16) char win_s[kLaneWaveBytes];
+   __shared__ uint32_t wb_s[kWave];
+   __shared__ K ring_v[kLaneRing];
+   __shared__ uint8_t ring_s[kLaneRing];
+   const uint64_t umax = (sc->max_qry + kBatchUnit - 1) / kBatchUnit;
+   const uint64_t units = umax * a.n_maps;
+   for (;;) {
+      const uint32_t u = next_unit(&sc->qry_ticket);
+      if (u >= units) break;
+      const uint32_t m = static_cast<uint32_t>(u / umax);
+      const uint64_t q0 = (u % umax) * kBatchUnit;
+      const hn_batch_map M = a.maps[m];
+      hn_info* info = M.info;
+      const int sp = info->size_power;
+      if (q0 < M.qry_len && sp >= 1 && sp <= max_size_power<K>()) {
+         const uint64_t q1 = q0 + kBatchUnit < M.qry_len ? q0 + kBatchUnit : M.qry_len;
+         const uint64_t ovf = info->current_max_bucket_overflow;
+         const bool ordered = (info->reserved1 & kInfoOrdered) != 0;
+         LookupArgs<K> la{a.qry_keys + M.qry_off, a.qry_vals + M.qry_off, nullptr,
+                          a.qry_hit ? a.qry_hit + M.qry_off : nullptr, static_cast<Slot<K>*>(M.buckets), info,
+                          M.qry_len};
+         if (sp >= LaneWin<K>::kLog && ovf > 0) {
+            lane_lookup<K, false, LK_RETRIEVE>(la, sp, ovf, ordered, q0, q1, win_s, wb_s, ring_v, ring_s);
+         } else {
+            lookup_range<K, 1, false, LK_RETRIEVE>(la, sp, ovf, q1, q0 + __lane_id(), kWave, ordered);
+         }
+         wave_lds_sync();
+      }
+   }
+}
+
 // Fresh tables and Info(sp) for every map of a batch, one launch.
 template <typename K>
 __global__ __launch_bounds__(256) void k_batch_init(const hn_batch_map* maps, uint32_t n_maps, const int32_t* sps) {
@@ -256,10 +338,34 @@ int batch_run(const hn_batch_map* maps, uint32_t n_maps, uint32_t ops, const K* 
                                                : static_cast<uint32_t>(cu_count()) * (2048 / bs);
    if (blocks > n_maps) blocks = n_maps;
    BatchKernel<K> k = pick_batch<K>(G);
-   uint32_t* ticket = static_cast<uint32_t*>(scratch);
-   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, ticket);
+   BatchScratch* sc = static_cast<BatchScratch*>(scratch);
+   uint32_t* ticket = &sc->ticket;
    BatchArgs<K> a{maps, n_maps, ops, ik, iv, qk, qv, qh, ek, static_cast<Slot<K>*>(clean_scratch), ticket};
-   hipLaunchKernelGGL(k, dim3(blocks), dim3(bs), 0, s, a);
+   // insert; retrieve (one key per lane, its own launch); erase and clean
+   if (ops & HN_BATCH_INSERT) {
+      a.ops = HN_BATCH_INSERT;
+      hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, ticket);
+      hipLaunchKernelGGL(k, dim3(blocks), dim3(bs), 0, s, a);
+   }
+   if (ops & HN_BATCH_RETRIEVE) {
+      static int per_cu = 0;
+      if (per_cu == 0) {
+         int nb = 0;
+         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch_retrieve_lane<K>, kWave, 0) != hipSuccess ||
+             nb < 1)
+            nb = 8;
+         per_cu = nb;
+      }
+      a.ops = HN_BATCH_RETRIEVE;
+      hipLaunchKernelGGL(k_batch_prep, dim3(1), dim3(256), 0, s, maps, n_maps, &sc->max_qry, &sc->qry_ticket);
+      hipLaunchKernelGGL(k_batch_retrieve_lane<K>, dim3(static_cast<uint32_t>(cu_count()) * per_cu), dim3(kWave), 0, s,
+                         a, sc);
+   }
+   if (ops & (HN_BATCH_ERASE | HN_BATCH_CLEAN)) {
+      a.ops = ops & (HN_BATCH_ERASE | HN_BATCH_CLEAN);
+      hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, ticket);
+      hipLaunchKernelGGL(k, dim3(blocks), dim3(bs), 0, s, a);
+   }
    return static_cast<int>(hipGetLastError());
 }
 

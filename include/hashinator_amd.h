@@ -336,8 +336,10 @@ HN_API int hn_exclusive_scan_u64(const uint64_t* in, uint64_t* out, size_t n, vo
  * map the requested phases run in the reference's order -- insert
  * (hashers.h:46-65), retrieve (111-119), erase (132-142), clean_tombstones
  * (hashinator.h:1261-1315) -- with the single-map semantics above; one
- * workgroup owns one map at a time, maps are taken in array order (list the
- * largest first), all in one persistent launch (graph-capturable: no host
+ * workgroup owns one map at a time for insert and for erase + clean, maps are
+ * taken in array order (list the largest first); retrieve runs in between,
+ * one key per lane over units of every map's queries.  Up to three persistent
+ * launches plus small set-up kernels (graph-capturable: no host
  * synchronisation, no allocation, no memset). */
 typedef struct hn_batch_map {
    void* buckets;        /* 2^size_power slots; size_power is read from *info */
