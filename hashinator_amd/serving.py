@@ -46,6 +46,9 @@ class Request:
     keys: np.ndarray
     vals: np.ndarray | None = None
     arrival: float = 0.0  # seconds from the start of the run
+    # host buffers the keys / values live in, when pinned (see pin_request)
+    pinned_keys: torch.Tensor | None = None
+    pinned_vals: torch.Tensor | None = None
     # filled in by the engine
     out_vals: np.ndarray | None = None
     out_hit: np.ndarray | None = None
@@ -55,6 +58,23 @@ class Request:
     @property
     def latency(self):
         return self.t_done - self.arrival
+
+
+def pin_request(r: Request) -> Request:
+    """Move a request's keys (and values) into pinned host memory, as a
+    server receives them into pinned buffers: the engine then copies them to
+    the device straight from there, with no host-side staging copy."""
+    kb = 64 if r.keys.dtype.itemsize == 8 else 32
+    tk = torch.empty(r.keys.size, dtype=_TDT[kb], pin_memory=True)
+    a = tk.numpy().view(_NDT[kb])
+    a[:] = r.keys
+    r.keys, r.pinned_keys = a, tk
+    if r.vals is not None:
+        tv = torch.empty(r.vals.size, dtype=_TDT[kb], pin_memory=True)
+        b = tv.numpy().view(_NDT[kb])
+        b[:] = r.vals
+        r.vals, r.pinned_vals = b, tv
+    return r
 
 
 @dataclass
@@ -89,6 +109,7 @@ class ServingEngine:
             check(self.L.hn_info_init(info.data_ptr(), int(sp), s0.cuda_stream), "info")
             self.maps.append(_Map(int(sp), b, info, 0, torch.zeros(5, dtype=torch.int64, pin_memory=True)))
         torch.cuda.synchronize()
+        self._warm_up()
         self.streams = [torch.cuda.Stream(device=self.dev) for _ in range(n_streams)]
         self.free = list(range(n_streams))
         self.inflight = []  # (event, stream index, map id, [requests], bytes, keepalive tensors, t_admit)
@@ -96,6 +117,37 @@ class ServingEngine:
         self.rehashes = 0
         self.launches = 0
         self.completed = 0
+
+    def _warm_up(self):
+        """Run every entry point the loop uses once on a small scratch map, so
+        that loading code objects and the launch-shape queries done on a
+        kernel's first call do not land on the first requests."""
+        L, sfx, s = self.L, self.sfx, torch.cuda.current_stream().cuda_stream
+        sp = 10
+        b = torch.empty((1 << sp, 2), dtype=_TDT[self.kb], device=self.dev)
+        nb = torch.empty((1 << (sp + 1), 2), dtype=_TDT[self.kb], device=self.dev)
+        info = torch.zeros(5, dtype=torch.int64, device=self.dev)
+        k = torch.arange(1, 513, dtype=_TDT[self.kb], device=self.dev)
+        v = torch.empty_like(k)
+        h = torch.empty(k.numel(), dtype=torch.uint8, device=self.dev)
+        sc = torch.empty(L.hn_rehash_scratch_bytes(), dtype=torch.uint8, device=self.dev)
+        check(getattr(L, f"hn_table_init_{sfx}")(b.data_ptr(), 1 << sp, s), "warm-up")
+        check(L.hn_info_init(info.data_ptr(), sp, s), "warm-up")
+        check(getattr(L, f"hn_insert_{sfx}")(k.data_ptr(), k.data_ptr(), b.data_ptr(), info.data_ptr(), k.numel(), s,
+                                              self.tune), "warm-up")
+        check(getattr(L, f"hn_retrieve_{sfx}")(k.data_ptr(), v.data_ptr(), h.data_ptr(), b.data_ptr(),
+                                                info.data_ptr(), k.numel(), s, self.tune), "warm-up")
+        check(getattr(L, f"hn_erase_{sfx}")(k.data_ptr(), b.data_ptr(), info.data_ptr(), 16, s, self.tune), "warm-up")
+        check(getattr(L, f"hn_rehash_{sfx}")(b.data_ptr(), sp, nb.data_ptr(), sp + 1, info.data_ptr(), sc.data_ptr(), s),
+              "warm-up")
+        torch.cuda.synchronize()
+        # pinned result buffers come from torch's caching host allocator: fill
+        # its size classes (powers of two, 4 KiB .. 64 MiB, two of each) now
+        held = [torch.empty(1 << e, dtype=torch.uint8, pin_memory=True) for e in range(12, 27) for _ in range(2)]
+        # and the device-side request buffers come from torch's caching
+        # allocator: one cached 1 GiB segment serves them without hipMalloc
+        held.append(torch.empty(1 << 30, dtype=torch.uint8, device=self.dev))
+        del held
 
     # ------------------------------------------------------------ preload (not timed)
     def preload(self, map_id, keys, vals):
@@ -131,6 +183,16 @@ class ServingEngine:
                 group.append(r)
         return group
 
+    def _host_input(self, reqs, what):
+        """The group's keys or values in pinned host memory: a single pinned
+        request's own buffer, else one pinned buffer filled in order."""
+        if len(reqs) == 1 and getattr(reqs[0], f"pinned_{what}") is not None:
+            return getattr(reqs[0], f"pinned_{what}")
+        parts = [getattr(r, what) for r in reqs]
+        h = torch.empty(sum(p.size for p in parts), dtype=_TDT[self.kb], pin_memory=True)
+        np.concatenate(parts, out=h.numpy().view(_NDT[self.kb]))
+        return h
+
     def _launch(self, map_id, reqs, si, now, nbytes):
         m = self.maps[map_id]
         s = self.streams[si]
@@ -138,9 +200,8 @@ class ServingEngine:
         kind = reqs[0].kind
         keep = []
         with torch.cuda.stream(s):
-            keys = np.concatenate([r.keys for r in reqs])
-            n = keys.size
-            hk = torch.from_numpy(keys.view(np.int64 if self.kb == 64 else np.int32)).pin_memory()
+            hk = self._host_input(reqs, "keys")
+            n = hk.numel()
             dk = hk.to(self.dev, non_blocking=True)
             keep += [hk, dk]
             if kind == INSERT:
@@ -156,8 +217,7 @@ class ServingEngine:
                     keep += [m.buckets, sc]
                     m.buckets, m.sp = nb, new_sp
                     self.rehashes += 1
-                hv = torch.from_numpy(np.concatenate([r.vals for r in reqs]).view(
-                    np.int64 if self.kb == 64 else np.int32)).pin_memory()
+                hv = self._host_input(reqs, "vals")
                 dv = hv.to(self.dev, non_blocking=True)
                 keep += [hv, dv]
                 check(getattr(L, f"hn_insert_{sfx}")(dk.data_ptr(), dv.data_ptr(), m.buckets.data_ptr(),
@@ -196,14 +256,15 @@ class ServingEngine:
                 still.append(item)
                 continue
             if reqs[0].kind == RETRIEVE:
+                # results stay in the pinned buffers they arrived in (views, no copy)
                 hv, hh = keep[-2], keep[-1]
                 vals = hv.numpy().view(_NDT[self.kb])
-                hit = hh.numpy().astype(bool)
+                hit = hh.numpy().view(np.bool_)
                 off = 0
                 for r in reqs:
                     k = r.keys.size
-                    r.out_vals = vals[off:off + k].copy()
-                    r.out_hit = hit[off:off + k].copy()
+                    r.out_vals = vals[off:off + k]
+                    r.out_hit = hit[off:off + k]
                     off += k
             for r in reqs:
                 r.t_done = now
@@ -271,11 +332,12 @@ class ServingEngine:
 
 
 def make_requests(engine_sps, n_requests, rate_hz, seed, key_bits=64, min_batch=1000, max_batch=4_000_000,
-                  mix=(0.45, 0.45, 0.10), repeat_frac=0.3, live_keys=None):
+                  mix=(0.45, 0.45, 0.10), repeat_frac=0.3, live_keys=None, pinned=True):
     """A Poisson stream of requests over the maps: kind by `mix`, batch size
     log-uniform in [min_batch, max_batch], keys uniform (unique within a
     batch) with `repeat_frac` drawn from keys already live in the map;
-    values are a function of (key, request id)."""
+    values are a function of (key, request id).  `pinned`: the keys and
+    values are delivered in pinned host memory (pin_request)."""
     from . import workloads as W
 
     rng = np.random.default_rng(seed)
@@ -302,7 +364,8 @@ def make_requests(engine_sps, n_requests, rate_hz, seed, key_bits=64, min_batch=
         vals = W.values(keys, rid).astype(nd) if kind == INSERT else None
         if kind == INSERT:
             live[mid].append(keys)
-        reqs.append(Request(rid, mid, str(kind), keys, vals, t))
+        r = Request(rid, mid, str(kind), keys, vals, t)
+        reqs.append(pin_request(r) if pinned else r)
     return reqs
 
 
