@@ -333,6 +333,8 @@ def main():
     if not args.no_serving:
         torch.cuda.empty_cache()
         extra["serving"] = run_serving(torch)
+    if not args.no_extra:
+        extra["small_map_u32"] = run_small_map(torch, L, dev, S, P, with_cpu=not args.no_cpu)
 
     line = {
         "metric": METRIC,
@@ -494,6 +496,72 @@ def run_compaction(torch, L, dev, S, P, buckets, fill):
                                    "GBps_alg": round(alg / (t * 1e-3) / 1e9, 1), "ok": int(cnt.item()) == fill}
     del out, sc
     torch.cuda.empty_cache()
+    return res
+
+
+def run_small_map(torch, L, dev, S, P, with_cpu=True, reps=20):
+    """The small single map: Hashmap<u32,u32> of 2^20 slots, insert 524,288
+    distinct uniform keys (uniform values), then retrieve them plus 524,288
+    absent keys.  The insert takes the maps' policy for a batch of half the
+    table into an empty map (the bulk build).  The 8 MiB table sits in the
+    caches: no HBM claim.  GPU: best of `reps` HIP-event-timed runs; beside it
+    the reference's CPU-only build (oracle/_ref, else the C restatement) on
+    the same inputs, one core."""
+    from hashinator_amd import _lib
+    from hashinator_amd import workloads as Wl
+
+    sp, n = 20, 524_288
+    seed = Wl.SEED_BASE + 1
+    keys = Wl.unique_keys(n, 32, seed)
+    vals = Wl.values(keys, seed)
+    absent = Wl.unique_keys(n, 32, seed, start=n + 2)
+    q = Wl.shuffled(np.concatenate([keys, absent]), seed)
+    dk = torch.from_numpy(keys.view(np.int32)).to(dev)
+    dv = torch.from_numpy(vals.view(np.int32)).to(dev)
+    dq = torch.from_numpy(q.view(np.int32)).to(dev)
+    out = torch.empty_like(dq)
+    hit = torch.empty(dq.numel(), dtype=torch.uint8, device=dev)
+    b = torch.empty((1 << sp, 2), dtype=torch.int32, device=dev)
+    info = torch.zeros(5, dtype=torch.int64, device=dev)
+    bsb = L.hn_build_scratch_bytes_u32(sp, n)
+    bs = torch.empty(bsb, dtype=torch.uint8, device=dev)
+    stream = torch.cuda.current_stream()
+    t_ins, t_ret = [], []
+    for _ in range(reps):
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        e[0].record(stream)
+        _lib.check(L.hn_info_init(P(info), sp, S), "info_init")
+        _lib.check(L.hn_build_u32(P(dk), P(dv), P(b), P(info), n, sp, P(bs), bsb, S, None), "build")
+        e[1].record(stream)
+        _lib.check(L.hn_retrieve_u32(P(dq), P(out), P(hit), P(b), P(info), dq.numel(), S, None), "retrieve")
+        e[2].record(stream)
+        torch.cuda.synchronize()
+        t_ins.append(e[0].elapsed_time(e[1]))
+        t_ret.append(e[1].elapsed_time(e[2]))
+    h = hit.cpu().numpy().astype(bool)
+    got = out.cpu().numpy().view(np.uint32)
+    exp_hit = np.isin(q, keys)
+    order = np.argsort(keys)
+    exp_val = vals[order][np.searchsorted(keys[order], q[exp_hit])]
+    ok = bool(np.array_equal(h, exp_hit) and np.array_equal(got[h], exp_val))
+    ti, tr = min(t_ins), min(t_ret)
+    res = {"workload": f"Hashmap<u32,u32>(sp={sp}): insert {n} keys, retrieve {n} present + {n} absent; "
+                       f"8 MiB table, cache-resident", "insert_ms": round(ti, 4), "retrieve_ms": round(tr, 4),
+           "insert_retrieve_Mops": round(3 * n / (ti + tr) / 1e3, 1), "ok": ok}
+    if with_cpu:
+        import oracle as O
+
+        kind = "reference" if O.ref_available() else "port"
+        m = O.ref_map(sp, 32) if kind == "reference" else O.CpuMap(sp, 32)
+        t0 = time.perf_counter()
+        m.insert(keys, vals)
+        t1 = time.perf_counter()
+        cv, ch = m.find(q)
+        t2 = time.perf_counter()
+        assert np.array_equal(ch, exp_hit) and np.array_equal(cv[ch], exp_val)
+        res["cpu"] = {"kind": kind, "cores": 1, "insert_ms": round((t1 - t0) * 1e3, 2),
+                      "retrieve_ms": round((t2 - t1) * 1e3, 2),
+                      "insert_retrieve_Mops": round(3 * n / (t2 - t0) / 1e6, 2)}
     return res
 
 

@@ -55,6 +55,7 @@ constexpr int kPartBS = 256;     // partition workgroup
 constexpr int kPartIPT = 12;     // pairs per thread and tile (3072-pair tiles: ~6 pairs per digit run)
 constexpr int kPartTile = kPartBS * kPartIPT;
 constexpr int kPartSub = 8;      // tiles per partition block
+constexpr int kScatBS = 256;     // threads of a scatter block (512 measured the same)
 constexpr int kPartSuper = kPartTile * kPartSub;
 constexpr int kMaxDigitBits = 9; // 512 digits per pass at most
 constexpr int kRegionBS = 512;
@@ -253,13 +254,14 @@ __global__ __launch_bounds__(kPartBS) void k_part_count(PartIn<K> in, uint64_t n
 // each pair's rank among equal digits in tile order; the tile is staged in
 // LDS in digit order and leaves in runs.
 template <typename K, int SRC, bool FILTER>
-__global__ __launch_bounds__(kPartBS) void k_part_scatter(PartIn<K> in, uint64_t n_in, const uint64_t* d_n,
+__global__ __launch_bounds__(kScatBS) void k_part_scatter(PartIn<K> in, uint64_t n_in, const uint64_t* d_n,
                                                           Slot<K>* out, const uint32_t* offs, uint32_t nblocks, int sp,
                                                           int rlog, int dshift, int dbits, const BuildHeader* h) {
-   constexpr int NW = kPartBS / kWave;
-   constexpr int ROUNDS = kPartTile / kPartBS;
+   constexpr int NW = kScatBS / kWave;
+   constexpr int ROUNDS = kPartTile / kScatBS;
    constexpr uint32_t DMAX = 1u << kMaxDigitBits;
-   static_assert(DMAX == 2 * kPartBS, "two digits per thread in the digit loops");
+   constexpr int DPT = DMAX / kScatBS; // digits per thread in the digit loops
+   static_assert(DPT * kScatBS == DMAX && ROUNDS * kScatBS == kPartTile, "digit and tile split");
    __shared__ Slot<K> tile_s[kPartTile];
    __shared__ uint16_t dig_s[kPartTile];
    __shared__ uint32_t wcnt_s[NW][DMAX];
@@ -271,8 +273,8 @@ __global__ __launch_bounds__(kPartBS) void k_part_scatter(PartIn<K> in, uint64_t
    const uint32_t D = 1u << dbits, dmask = D - 1;
    const int lane = __lane_id(), wave = threadIdx.x / kWave;
    const uint64_t lt = (1ull << lane) - 1ull;
-   const uint32_t d0 = threadIdx.x * 2; // this thread's two digits in the digit loops
-   for (uint32_t d = d0; d < d0 + 2 && d < D; ++d) goff_s[d] = offs[static_cast<uint64_t>(d) * nblocks + blockIdx.x];
+   const uint32_t d0 = threadIdx.x * DPT; // this thread's digits in the digit loops
+   for (uint32_t d = d0; d < d0 + DPT && d < D; ++d) goff_s[d] = offs[static_cast<uint64_t>(d) * nblocks + blockIdx.x];
    const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kPartSuper;
    // The pairs of tile `sub` are loaded while tile sub-1 is written out.
    Slot<K> el[ROUNDS];
@@ -289,7 +291,7 @@ __global__ __launch_bounds__(kPartBS) void k_part_scatter(PartIn<K> in, uint64_t
    for (int sub = 0; sub < kPartSub; ++sub) {
       const uint64_t t0 = base + static_cast<uint64_t>(sub) * kPartTile;
       if (t0 >= n) break;
-      for (uint32_t d = d0; d < d0 + 2 && d < D; ++d) {
+      for (uint32_t d = d0; d < d0 + DPT && d < D; ++d) {
 #pragma unroll
          for (int w = 0; w < NW; ++w) wcnt_s[w][d] = 0;
       }
@@ -324,8 +326,11 @@ __global__ __launch_bounds__(kPartBS) void k_part_scatter(PartIn<K> in, uint64_t
       }
       __syncthreads();
       // Per digit: waves' counts -> exclusive offsets over waves; tile totals -> exclusive scan.
-      uint32_t tot[2] = {0, 0};
-      for (int i = 0; i < 2; ++i) {
+      uint32_t tot[DPT];
+      uint32_t acc = 0;
+#pragma unroll
+      for (int i = 0; i < DPT; ++i) {
+         tot[i] = 0;
          const uint32_t d = d0 + i;
          if (d < D) {
 #pragma unroll
@@ -335,9 +340,9 @@ __global__ __launch_bounds__(kPartBS) void k_part_scatter(PartIn<K> in, uint64_t
                tot[i] += c;
             }
          }
+         acc += tot[i];
       }
       {
-         const uint32_t acc = tot[0] + tot[1];
          uint32_t inc = acc;
 #pragma unroll
          for (int o = 1; o < kWave; o <<= 1) {
@@ -348,8 +353,11 @@ __global__ __launch_bounds__(kPartBS) void k_part_scatter(PartIn<K> in, uint64_t
          __syncthreads();
          uint32_t bse = inc - acc;
          for (int w = 0; w < wave; ++w) bse += wsum_s[w];
-         if (d0 < D) lstart_s[d0] = bse;
-         if (d0 + 1 < D) lstart_s[d0 + 1] = bse + tot[0];
+#pragma unroll
+         for (int i = 0; i < DPT; ++i) {
+            if (d0 + i < D) lstart_s[d0 + i] = bse;
+            bse += tot[i];
+         }
       }
       __syncthreads();
 #pragma unroll
@@ -364,13 +372,14 @@ __global__ __launch_bounds__(kPartBS) void k_part_scatter(PartIn<K> in, uint64_t
       __syncthreads();
       uint32_t cnt = 0; // pairs of this tile that are written: the digit scan's total
       for (int w = 0; w < NW; ++w) cnt += wsum_s[w];
-      for (uint32_t p = threadIdx.x; p < cnt; p += kPartBS) {
+      for (uint32_t p = threadIdx.x; p < cnt; p += kScatBS) {
          const uint32_t d = dig_s[p];
          out[static_cast<uint64_t>(goff_s[d]) + (p - lstart_s[d])] = tile_s[p];
       }
       __syncthreads();
-      if (d0 < D) goff_s[d0] += tot[0];
-      if (d0 + 1 < D) goff_s[d0 + 1] += tot[1];
+#pragma unroll
+      for (int i = 0; i < DPT; ++i)
+         if (d0 + i < D) goff_s[d0 + i] += tot[i];
    }
 }
 
@@ -957,7 +966,7 @@ int launch_part(const BuildPlan& P, int pass, PartIn<K> in, uint64_t n_in, const
                       P.nblocks, P.sp, P.rlog, dshift, dbits, h);
    int rc = hn_exclusive_scan_u32(mat, mat, D * P.nblocks, scan_scratch, scan_bytes, reinterpret_cast<hn_stream_t>(s));
    if (rc) return rc;
-   hipLaunchKernelGGL((k_part_scatter<K, SRC, FILTER>), dim3(P.nblocks), dim3(kPartBS), 0, s, in, n_in, d_n, out, mat,
+   hipLaunchKernelGGL((k_part_scatter<K, SRC, FILTER>), dim3(P.nblocks), dim3(kScatBS), 0, s, in, n_in, d_n, out, mat,
                       P.nblocks, P.sp, P.rlog, dshift, dbits, h);
    return static_cast<int>(hipGetLastError());
 }

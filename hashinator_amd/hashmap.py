@@ -70,19 +70,24 @@ class Hashmap:
     """Hashinator::Hashmap<K,V> with K = V = uint32_t or uint64_t, accelerated API."""
 
     def __init__(self, sizePower: int = 5, key_bits: int = 32, device="cuda", tune: hn_tune | None = None,
-                 auto_reorder: float = 1.0 / 16, auto_build: float = 1.0 / 16):
+                 auto_reorder: float = 1.0 / 16, auto_build: float = 1.0 / 16, build_min_keys: int = 1 << 19):
         """``auto_reorder``: after an insert of at least this fraction of the
         bucket count, and after clean_tombstones and device_rehash, run the
         Robin Hood reorder pass (``reorder()``), which makes later misses
         stop early; 0 turns it off.  ``auto_build``: an insert of at least
         this fraction of the bucket count into an empty table takes the bulk
         path (hn_build_*: partition + per-region sort, no probing), which
-        writes the reordered image directly; 0 turns it off."""
+        writes the reordered image directly; 0 turns it off.
+        ``build_min_keys``: and holds at least this many keys -- below about
+        2^19 keys the build's dozen small launches cost more than the probing
+        insert and reorder (one MI355X: 0.23 ms against 0.09 ms at 131,072
+        keys, break-even near 524,288, 0.53 ms against 1.5 ms at 15 M)."""
         if key_bits not in (32, 64):
             raise ValueError("key_bits must be 32 or 64")
         self.kb = key_bits
         self.auto_reorder = auto_reorder
         self.auto_build = auto_build
+        self.build_min_keys = build_min_keys
         self.device = torch.device(device)
         self.tune = tune
         self._L = _lib.load()
@@ -206,7 +211,7 @@ class Hashmap:
         does not apply or the device declined it (err = INVALID: the table is
         then empty and the probing insert runs instead)."""
         N = self.bucket_count()
-        if not self.auto_build or n < self.auto_build * N or n >= N:
+        if not self.auto_build or n < self.auto_build * N or n < self.build_min_keys or n >= N:
             return False
         i = self.info()
         if i.fill != 0 or i.tombstone_counter != 0:

@@ -70,6 +70,7 @@ class Hashmap {
    bool _has_tune = false;
    float _auto_reorder = 1.0f / 16; // reorder after inserts of at least this fraction of the buckets
    float _auto_build = 1.0f / 16;   // bulk-build inserts of at least this fraction into an empty table
+   size_t _build_min_keys = size_t(1) << 19; // ... and of at least this many keys (below, launches dominate)
 
    uint32_t hash(KEY_TYPE in) const { return static_cast<uint32_t>(HashFunction::_hash(in, _info.sizePower)); }
    const hn_tune* tune() const { return _has_tune ? &_tune : nullptr; }
@@ -476,15 +477,20 @@ public:
 
    /* Fraction of the bucket count an insert into an empty table must reach
     * to take the bulk path (hn_build_*: partition + per-region sort, no
-    * probing; writes the reordered image directly); 0 turns it off. */
-   void setAutoBuild(float fraction) noexcept { _auto_build = fraction; }
+    * probing; writes the reordered image directly), and the fewest keys it
+    * must hold (below about 2^19 keys the build's launches cost more than the
+    * probing insert and reorder); fraction 0 turns it off. */
+   void setAutoBuild(float fraction, size_t min_keys = size_t(1) << 19) noexcept {
+      _auto_build = fraction;
+      _build_min_keys = min_keys;
+   }
 
 private:
    /* The bulk path; false when it does not apply or the device declined it
     * (then the table is empty and the probing insert runs). */
    bool try_build(const void* keys, const void* vals, const void* pairs, size_t len, hipStream_t s) {
       if (_auto_build <= 0.0f || static_cast<double>(len) < _auto_build * static_cast<double>(_nbuckets) ||
-          len >= _nbuckets || _info.fill != 0 || _info.tombstoneCounter != 0)
+          len < _build_min_keys || len >= _nbuckets || _info.fill != 0 || _info.tombstoneCounter != 0)
          return false;
       const int sp = _info.sizePower;
       const size_t sb = W64 ? hn_build_scratch_bytes_u64(sp, len) : hn_build_scratch_bytes_u32(sp, len);

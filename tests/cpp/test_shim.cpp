@@ -104,7 +104,7 @@ bool erase_clean_extract(int power) {
       vals[i] = static_cast<K>(i);
    }
    Map hmap(power);
-   hmap.setAutoBuild(0.0f); // the probing insert kernel (the other cases take the bulk build)
+   hmap.setAutoBuild(0.0f); // the probing insert kernel, whatever the batch size
    hmap.insert(keys.data(), vals.data(), n, 1.0f);
    // erase the even keys (test_hashmap_* "no even-valued survivors")
    split::SplitVector<K> even(n / 2);

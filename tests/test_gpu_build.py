@@ -240,7 +240,7 @@ def test_map_insert_uses_build(gpu, bits):
     sp = 16
     keys = keyset(int(0.9 * (1 << sp)), bits, seed=9)
     vals = (keys + NDT[bits](5)).astype(NDT[bits])
-    hm = Hashmap(sp, bits)
+    hm = Hashmap(sp, bits, build_min_keys=0)  # build even for this small a batch
     hm.insert(keys, vals, targetLF=1.0)
     m = expect(bits, sp, keys, vals)
     assert np.array_equal(hm.host_buckets().reshape(-1), m.b)
