@@ -348,14 +348,8 @@ int batch_run(const hn_batch_map* maps, uint32_t n_maps, uint32_t ops, const K* 
       hipLaunchKernelGGL(k, dim3(blocks), dim3(bs), 0, s, a);
    }
    if (ops & HN_BATCH_RETRIEVE) {
-      static int per_cu = 0;
-      if (per_cu == 0) {
-         int nb = 0;
-         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch_retrieve_lane<K>, kWave, 0) != hipSuccess ||
-             nb < 1)
-            nb = 8;
-         per_cu = nb;
-      }
+      static std::atomic<int> occ{0};
+      const int per_cu = resident_blocks_per_cu(k_batch_retrieve_lane<K>, kWave, occ);
       a.ops = HN_BATCH_RETRIEVE;
       hipLaunchKernelGGL(k_batch_prep, dim3(1), dim3(256), 0, s, maps, n_maps, &sc->max_qry, &sc->qry_ticket);
       hipLaunchKernelGGL(k_batch_retrieve_lane<K>, dim3(static_cast<uint32_t>(cu_count()) * per_cu), dim3(kWave), 0, s,

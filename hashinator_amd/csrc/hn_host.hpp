@@ -1,6 +1,8 @@
 // hn_host.hpp -- host-side launch geometry shared by the C-ABI entry points.
 #pragma once
 #include <hip/hip_runtime.h>
+
+#include <atomic>
 #include <stdint.h>
 
 #include "hashinator_amd.h"
@@ -51,6 +53,19 @@ int resolve_launch(const hn_tune* t, OpKind op, uint64_t len, Launch* L) {
    L->block_size = bs;
    L->blocks = static_cast<uint32_t>(need < cap ? need : cap);
    return 0;
+}
+
+// Resident blocks per CU of `kernel` at `block` threads (its occupancy),
+// asked once per kernel and kept in `cache`: the lane kernels split their
+// work statically over exactly that many waves per CU.
+template <typename F>
+int resident_blocks_per_cu(F kernel, int block, std::atomic<int>& cache) {
+   int v = cache.load(std::memory_order_relaxed);
+   if (v > 0) return v;
+   int nb = 0;
+   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, block, 0) != hipSuccess || nb < 1) nb = 8;
+   cache.store(nb, std::memory_order_relaxed);
+   return nb;
 }
 
 // Grid for a grid-stride streaming kernel over n items.

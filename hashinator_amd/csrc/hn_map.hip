@@ -371,15 +371,8 @@ int launch_lookup(LookupArgs<K> a, hipStream_t s, const hn_tune* t, OpKind op) {
          if (need == 0) need = 1;
          // one resident wave per block slot: the waves split the queries
          // statically, so a block that waits for a free slot would finish late
-         static int per_cu = 0;
-         if (per_cu == 0) {
-            int nb = 0;
-            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_lookup_lane<K, PAIRS, MODE>, kLaneBlock, 0) !=
-                    hipSuccess ||
-                nb < 1)
-               nb = 8;
-            per_cu = nb;
-         }
+         static std::atomic<int> occ{0};
+         const int per_cu = resident_blocks_per_cu(k_lookup_lane<K, PAIRS, MODE>, kLaneBlock, occ);
          uint64_t cap = (t && t->grid_blocks > 0) ? static_cast<uint64_t>(t->grid_blocks)
                                                   : static_cast<uint64_t>(cu_count()) * per_cu;
          if (cap > (1u << 20)) cap = 1u << 20;
