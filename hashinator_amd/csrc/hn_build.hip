@@ -52,7 +52,7 @@ namespace hn {
 
 constexpr int kRegionLog = 11;   // 2048 slots per region
 constexpr int kPartBS = 256;     // partition workgroup
-constexpr int kPartIPT = 12;     // pairs per thread and tile (3072-pair tiles: ~6 pairs per digit run)
+constexpr int kPartIPT = 16;     // pairs per thread and tile (4096-pair tiles: ~8 pairs per 9-bit digit run)
 constexpr int kPartTile = kPartBS * kPartIPT;
 constexpr int kPartSub = 8;      // tiles per partition block
 constexpr int kScatBS = 256;     // threads of a scatter block (512 measured the same)
@@ -248,7 +248,7 @@ __global__ __launch_bounds__(kPartBS) void k_part_count(PartIn<K> in, uint64_t n
 
 // Stable multisplit.  A block walks its kPartSub tiles in order, keeping the
 // running output offset of every digit in LDS.  In a tile, wave w ranks the
-// pairs [768w, 768w + 768) in twelve rounds of 64: lanes with equal digits are
+// pairs [1024w, 1024w + 1024) in sixteen rounds of 64: lanes with equal digits are
 // found with one ballot per digit bit, and the wave's own LDS counters carry
 // the earlier rounds (no block barrier per round).  Per-wave counts then give
 // each pair's rank among equal digits in tile order; the tile is staged in
@@ -263,7 +263,6 @@ __global__ __launch_bounds__(kScatBS) void k_part_scatter(PartIn<K> in, uint64_t
    constexpr int DPT = DMAX / kScatBS; // digits per thread in the digit loops
    static_assert(DPT * kScatBS == DMAX && ROUNDS * kScatBS == kPartTile, "digit and tile split");
    __shared__ Slot<K> tile_s[kPartTile];
-   __shared__ uint16_t dig_s[kPartTile];
    __shared__ uint32_t wcnt_s[NW][DMAX];
    __shared__ uint32_t lstart_s[DMAX];
    __shared__ uint32_t goff_s[DMAX];
@@ -365,7 +364,6 @@ __global__ __launch_bounds__(kScatBS) void k_part_scatter(PartIn<K> in, uint64_t
          if (ok[r]) {
             const uint32_t p = lstart_s[dg[r]] + wcnt_s[wave][dg[r]] + rk[r];
             tile_s[p] = el[r];
-            dig_s[p] = static_cast<uint16_t>(dg[r]);
          }
       }
       if (sub + 1 < kPartSub) load_tile(sub + 1);
@@ -373,27 +371,14 @@ __global__ __launch_bounds__(kScatBS) void k_part_scatter(PartIn<K> in, uint64_t
       uint32_t cnt = 0; // pairs of this tile that are written: the digit scan's total
       for (int w = 0; w < NW; ++w) cnt += wsum_s[w];
       for (uint32_t p = threadIdx.x; p < cnt; p += kScatBS) {
-         const uint32_t d = dig_s[p];
-         out[static_cast<uint64_t>(goff_s[d]) + (p - lstart_s[d])] = tile_s[p];
+         const Slot<K> e = tile_s[p];
+         const uint32_t d = (region_of(e.key, sp, rlog) >> dshift) & dmask; // recomputed: cheaper than LDS for it
+         out[static_cast<uint64_t>(goff_s[d]) + (p - lstart_s[d])] = e;
       }
       __syncthreads();
 #pragma unroll
       for (int i = 0; i < DPT; ++i)
          if (d0 + i < D) goff_s[d0 + i] += tot[i];
-   }
-}
-
-// start[r] = first index of region r in the partitioned array, start[NR] = n.
-template <typename K>
-__global__ __launch_bounds__(256) void k_region_starts(const Slot<K>* S, uint64_t* start, uint64_t NR, int sp, int rlog,
-                                                       const BuildHeader* h) {
-   if (!h->go) return;
-   const uint64_t n = h->n;
-   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i <= n; i += stride) {
-      const int64_t prev = i == 0 ? -1 : static_cast<int64_t>(region_of(S[i - 1].key, sp, rlog));
-      const int64_t cur = i == n ? static_cast<int64_t>(NR) : static_cast<int64_t>(region_of(S[i].key, sp, rlog));
-      for (int64_t r = prev + 1; r <= cur; ++r) start[r] = i;
    }
 }
 
@@ -461,15 +446,32 @@ __device__ __forceinline__ RegionSum load_sum(const RegionLB& lb, uint64_t r, in
    return unpack_sum(lb.word[r], R);
 }
 
-__global__ __launch_bounds__(256) void k_region_lb_init(RegionLB lb, uint64_t NR, BuildHeader* h) {
+// start[r] = first index of region r in the partitioned array (the first
+// index whose region is >= r, so an empty region starts where the next one
+// does), start[NR] = n: a binary search per region over the sorted array.
+// Neighbouring lanes search neighbouring regions and share the upper levels
+// of their searches in cache; the array itself is not streamed.  The same
+// launch clears the region bookkeeping of the placing step.
+template <typename K>
+__global__ __launch_bounds__(256) void k_region_starts(const Slot<K>* S, uint64_t* start, RegionLB lb, uint64_t NR,
+                                                       int sp, int rlog, BuildHeader* h) {
+   if (!h->go) return;
+   const uint64_t n = h->n;
    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < NR; i += stride) {
-      lb.word[i] = 0;
-      lb.mark[i] = 0;
+   for (uint64_t r = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; r <= NR; r += stride) {
+      uint64_t lo = 0, hi = n;
+      while (lo < hi) {
+         const uint64_t mid = lo + (hi - lo) / 2;
+         if (region_of(S[mid].key, sp, rlog) < r) lo = mid + 1;
+         else hi = mid;
+      }
+      start[r] = lo;
+      if (r < NR) {
+         lb.word[r] = 0;
+         lb.mark[r] = 0;
+      }
    }
-   if (blockIdx.x == 0 && threadIdx.x == 0) {
-      h->n_fix = 0;
-   }
+   if (blockIdx.x == 0 && threadIdx.x == 0) h->n_fix = 0;
 }
 
 constexpr int kRegionNW = kRegionBS / kWave;
@@ -1028,9 +1030,8 @@ int build_core(const BuildPlan& P, int mode, int src0, PartIn<K> in0, uint64_t n
       if (rc) return rc;
       src = dst;
    }
-   hipLaunchKernelGGL(k_region_starts<K>, dim3(stream_blocks(P.n + 1, 256)), dim3(256), 0, s, S, start, P.NR, P.sp,
-                      P.rlog, h);
-   hipLaunchKernelGGL(k_region_lb_init, dim3(stream_blocks(P.NR, 256)), dim3(256), 0, s, lb, P.NR, h);
+   hipLaunchKernelGGL(k_region_starts<K>, dim3(stream_blocks(P.NR + 1, 256)), dim3(256), 0, s, S, start, lb, P.NR,
+                      P.sp, P.rlog, h);
    hipLaunchKernelGGL(k_region_build<K>, dim3(static_cast<uint32_t>((P.NR + kRegionChunk - 1) / kRegionChunk)),
                       dim3(kRegionBS), 0, s, S, start, lb, T, P.sp, P.rlog, P.NR, h);
    RegionSum* blk = reinterpret_cast<RegionSum*>(base + P.off_blk);

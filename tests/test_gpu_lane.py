@@ -115,7 +115,7 @@ def test_lane_large_ordered(gpu, bits):
 
 @pytest.mark.parametrize("grid", [1, 3, 1000])
 def test_lane_grid_sizes(gpu, grid):
-    # one block (4 waves, each a quarter of the queries), an odd count, more waves than queries
+    # one one-wave block for all queries, an odd count of blocks, more blocks than queries
     bits, sp = 64, 12
     hm, model, keys = make(bits, sp, 0.9, seed=grid, ordered=False, tune=hn_tune(0, 0, grid, 0))
     q = queries(keys, bits, seed=grid)[:2000]
@@ -132,3 +132,17 @@ def test_lane_ignores_block_size(gpu):
     v, h = retrieve_np(hm, q, bits, NDT[bits](1))
     ev, eh = model.retrieve(q, np.ones(q.size, dtype=NDT[bits]))
     assert np.array_equal(h, eh) and np.array_equal(v, ev)
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+@pytest.mark.parametrize("nq", [1, 2, 63, 64, 65, 1000])
+def test_lane_query_counts(gpu, bits, nq):
+    # query counts around one wave's 64 lanes, and fewer queries than lanes
+    hm, model, keys = make(bits, 12, 0.9, seed=nq + bits, ordered=True)
+    q = queries(keys, bits, seed=nq)[:nq]
+    v, h = retrieve_np(hm, q, bits, NDT[bits](9))
+    ev, eh = model.retrieve(q, np.full(q.size, 9, dtype=NDT[bits]))
+    assert np.array_equal(h, eh) and np.array_equal(v, ev)
+    hm.erase(q)
+    model.erase(q)
+    assert np.array_equal(hm.host_buckets().reshape(-1), model.b)
