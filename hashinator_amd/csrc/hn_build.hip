@@ -13,7 +13,8 @@
 //      (region = home >> log2 R, R = 2048 slots): per pass a per-tile digit
 //      count, an exclusive scan of the digit-major count matrix, and a stable
 //      multisplit scatter through LDS (digits of at most 9 bits, 1-3 passes);
-//   2. region boundaries in the partitioned array;
+//   2. region boundaries in the partitioned array (a binary search per
+//      region);
 //   3. per region, in LDS: sort by (home, key), drop all but the last
 //      occurrence of a repeated key (the value a serial insert leaves), and
 //      summarise the region as the carry function g(c) = max(a, c + b), where
@@ -30,10 +31,10 @@
 //      carry was wrong, or whose range wraps past the table's end, are
 //      placed again.
 //
-// Traffic per key: the partition passes read and write the 16 (u64) or 8
-// (u32) byte pair twice each, plus one read for the boundaries and one for
-// the region step; the table is written once.  No atomics on the table, no
-// probing.
+// Traffic per key: each partition pass reads the key for its count, then
+// reads and writes the 16 (u64) or 8 (u32) byte pair; the region step reads
+// the pair once more and the table is written once.  No atomics on the
+// table, no probing.
 //
 // The table's memory is the output of the first partition pass when there
 // are two, so the scratch holds one pair array, not two.  A region holding
