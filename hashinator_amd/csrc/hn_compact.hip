@@ -20,10 +20,16 @@
 
 namespace hn {
 
-constexpr int kBS = 256;         // threads per tile
+// Tiles are large because every tile pays a serial cost: its ticket (one
+// atomic on one word) and its step in the look-back chain.  At 2^28 elements,
+// 256-thread tiles ran copy_if in 2.0 ms and the u32 scan in 1.0 ms; 1024
+// threads with 8 pairs each (copy_if) and 256 bytes each (scan) take 1.33 and
+// 0.60 ms.  Fewer pairs per thread (4) or more (16) were slower for copy_if.
+constexpr int kBS = 1024;        // threads per tile
 constexpr int kWaves = kBS / kWave;
 constexpr int kCompactIPT = 8;   // elements per thread in a compaction tile
 constexpr int kCompactTile = kBS * kCompactIPT;
+constexpr int kScanBytes = 256;  // consecutive bytes per thread in a scan tile
 constexpr uint64_t kFlagAgg = 1, kFlagInc = 2;
 
 // Per-tile look-back state: the 64-bit value in two words, each carrying
@@ -108,6 +114,27 @@ __device__ uint64_t lookback_prefix(const Lookback& L, uint32_t tile, uint64_t m
    return prefix;
 }
 
+template <typename T>
+__device__ __forceinline__ T wave_incl_scan(T x) {
+   for (int o = 1; o < kWave; o <<= 1) {
+      T y = __shfl_up(x, o, kWave);
+      if (__lane_id() >= o) x += y;
+   }
+   return x;
+}
+// u32: DPP row shifts inside 16-lane rows, then row broadcasts across rows.
+template <>
+__device__ __forceinline__ uint32_t wave_incl_scan<uint32_t>(uint32_t x) {
+   int v = static_cast<int>(x);
+   v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false); // row_shr:1
+   v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false); // row_shr:2
+   v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false); // row_shr:4
+   v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false); // row_shr:8
+   v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false); // row_bcast:15 -> rows 1, 3
+   v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false); // row_bcast:31 -> rows 2, 3
+   return static_cast<uint32_t>(v);
+}
+
 // ------------------------------------------------------------ compaction
 
 template <typename K>
@@ -175,20 +202,29 @@ __global__ __launch_bounds__(kBS) void k_compact(const In* __restrict__ in, uint
       if (lane == 0) s_cnt[j][wave] = static_cast<uint32_t>(__popcll(bal[j]));
    }
    __syncthreads();
-   if (threadIdx.x == 0) { // j-major, wave-minor exclusive prefix = input order
-      uint32_t run = 0;
-      for (int j = 0; j < kCompactIPT; ++j)
-         for (int w = 0; w < kWaves; ++w) {
-            const uint32_t c = s_cnt[j][w];
-            s_cnt[j][w] = run;
-            run += c;
-         }
-      s_prefix = run;
-   }
-   __syncthreads();
-   const uint64_t total = s_prefix;
-   __syncthreads();
    if (wave == 0) {
+      // Exclusive prefix of the counts, j-major and wave-minor (= input
+      // order), by one wave: each lane scans PER consecutive counts.
+      constexpr int E = kCompactIPT * kWaves;
+      constexpr int PER = (E + kWave - 1) / kWave;
+      uint32_t* flat = &s_cnt[0][0];
+      uint32_t c[PER];
+      uint32_t sum = 0;
+#pragma unroll
+      for (int i = 0; i < PER; ++i) {
+         const int idx = lane * PER + i;
+         c[i] = idx < E ? flat[idx] : 0u;
+         sum += c[i];
+      }
+      const uint32_t incl = wave_incl_scan<uint32_t>(sum);
+      uint32_t run = incl - sum;
+#pragma unroll
+      for (int i = 0; i < PER; ++i) {
+         const int idx = lane * PER + i;
+         if (idx < E) flat[idx] = run;
+         run += c[i];
+      }
+      const uint64_t total = static_cast<uint32_t>(__shfl(static_cast<int>(incl), kWave - 1, kWave));
       const uint64_t pre = lookback_prefix(L, tile, total);
       if (lane == 0) {
          s_prefix = pre;
@@ -209,31 +245,10 @@ __global__ __launch_bounds__(kBS) void k_compact(const In* __restrict__ in, uint
 
 // ------------------------------------------------------------ scan
 
-template <typename T>
-__device__ __forceinline__ T wave_incl_scan(T x) {
-   for (int o = 1; o < kWave; o <<= 1) {
-      T y = __shfl_up(x, o, kWave);
-      if (__lane_id() >= o) x += y;
-   }
-   return x;
-}
-// u32: DPP row shifts inside 16-lane rows, then row broadcasts across rows.
-template <>
-__device__ __forceinline__ uint32_t wave_incl_scan<uint32_t>(uint32_t x) {
-   int v = static_cast<int>(x);
-   v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false); // row_shr:1
-   v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false); // row_shr:2
-   v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false); // row_shr:4
-   v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false); // row_shr:8
-   v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false); // row_bcast:15 -> rows 1, 3
-   v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false); // row_bcast:31 -> rows 2, 3
-   return static_cast<uint32_t>(v);
-}
-
-// Each thread owns IPT consecutive elements (64 bytes); a tile is kBS*IPT.
+// Each thread owns IPT consecutive elements (kScanBytes); a tile is kBS*IPT.
 template <typename T, bool VEC>
 __global__ __launch_bounds__(kBS) void k_scan(const T* in, T* out, uint64_t n, Lookback L, uint64_t tiles) {
-   constexpr int IPT = 64 / sizeof(T);
+   constexpr int IPT = kScanBytes / sizeof(T);
    constexpr uint64_t TILE = static_cast<uint64_t>(kBS) * IPT;
    __shared__ uint32_t s_tile;
    __shared__ T s_wave[kWaves];
@@ -249,7 +264,7 @@ __global__ __launch_bounds__(kBS) void k_scan(const T* in, T* out, uint64_t n, L
    if (VEC && i0 + IPT <= n) {
       const uint4* src = reinterpret_cast<const uint4*>(in + i0);
 #pragma unroll
-      for (int c = 0; c < 4; ++c) reinterpret_cast<uint4*>(x)[c] = src[c];
+      for (int c = 0; c < kScanBytes / 16; ++c) reinterpret_cast<uint4*>(x)[c] = src[c];
    } else {
 #pragma unroll
       for (int j = 0; j < IPT; ++j) x[j] = (i0 + j < n) ? in[i0 + j] : T(0);
@@ -282,7 +297,7 @@ __global__ __launch_bounds__(kBS) void k_scan(const T* in, T* out, uint64_t n, L
    if (VEC && i0 + IPT <= n) {
       uint4* dst = reinterpret_cast<uint4*>(out + i0);
 #pragma unroll
-      for (int c = 0; c < 4; ++c) dst[c] = reinterpret_cast<uint4*>(x)[c];
+      for (int c = 0; c < kScanBytes / 16; ++c) dst[c] = reinterpret_cast<uint4*>(x)[c];
    } else {
 #pragma unroll
       for (int j = 0; j < IPT; ++j)
@@ -315,7 +330,7 @@ template <typename T>
 int run_scan(const T* in, T* out, uint64_t n, void* scratch, size_t bytes, hipStream_t s) {
    if (n && (!in || !out)) return HN_EINVAL;
    if (n == 0) return 0;
-   constexpr uint64_t TILE = static_cast<uint64_t>(kBS) * (64 / sizeof(T));
+   constexpr uint64_t TILE = static_cast<uint64_t>(kBS) * (kScanBytes / sizeof(T));
    const uint64_t tiles = (n + TILE - 1) / TILE;
    if (!scratch || bytes < lookback_bytes(tiles)) return HN_ESCRATCH;
    Lookback L = lookback_layout(scratch, tiles);

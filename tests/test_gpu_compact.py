@@ -10,7 +10,8 @@ from tests.helpers import NDT, keyset
 
 pytestmark = pytest.mark.gpu
 
-SIZES = [0, 1, 63, 2047, 2048, 2049, 100_003, 1 << 20]
+# around one compaction tile (8192 elements) and a few tiles, plus sizes of many tiles
+SIZES = [0, 1, 63, 2047, 2048, 2049, 8191, 8192, 8193, 24_577, 100_003, 1 << 20]
 
 
 def _t(a, bits):
@@ -57,7 +58,9 @@ def test_copy_if_flags_clustered(gpu, eb, n):
 
 
 @pytest.mark.parametrize("bits", [32, 64])
-@pytest.mark.parametrize("n", [1, 15, 16, 4095, 4096, 4097, 1_000_003, 1 << 22])
+# around one scan tile (65536 u32, 32768 u64 elements), two tiles, many tiles
+@pytest.mark.parametrize("n", [1, 15, 16, 4095, 4096, 4097, 32_767, 32_768, 32_769, 65_535, 65_536, 65_537,
+                               131_073, 1_000_003, 1 << 22])
 def test_exclusive_scan(gpu, bits, n):
     rng = np.random.default_rng(n * bits)
     x = rng.integers(0, np.iinfo(NDT[bits]).max, size=n, dtype=NDT[bits], endpoint=True)  # wraps
