@@ -49,7 +49,7 @@ def test_scratch_sizing_host_only():
         b = L.hn_clean_tombstones_scratch_bytes_u64(sp, 1 << 20)
         assert min(L.hn_clean_tombstones_min_scratch(), full) <= b <= max(1 << 20, full)
     assert L.hn_clean_tombstones_scratch_bytes_u32(0, 0) == 0  # invalid size power
-    assert L.hn_compact_scratch_bytes(1 << 28) >= (1 << 28) // 2048 * 16  # two 8-byte state words per tile
+    assert L.hn_compact_scratch_bytes(1 << 28) >= (1 << 28) // 8192 * 16  # two 8-byte state words per 8192-element tile
     assert L.hn_rehash_scratch_bytes() >= 8
 
 
