@@ -115,6 +115,10 @@ SIGNATURES = {
     "hn_build_u64": (I, [P, P, P, P, S, I, P, S, P, TUNE]),
     "hn_build_pairs_u32": (I, [P, P, P, S, I, P, S, P, TUNE]),
     "hn_build_pairs_u64": (I, [P, P, P, S, I, P, S, P, TUNE]),
+    "hn_rehash_build_scratch_bytes_u32": (S, [I, I]),
+    "hn_rehash_build_scratch_bytes_u64": (S, [I, I]),
+    "hn_rehash_build_u32": (I, [P, I, P, I, P, P, S, P]),
+    "hn_rehash_build_u64": (I, [P, I, P, I, P, P, S, P]),
     "hn_compact_scratch_bytes": (S, [S]),
     "hn_copy_if_pairs_u32": (I, [P, S, PRED, P, P, P, S, P]),
     "hn_copy_if_pairs_u64": (I, [P, S, PRED, P, P, P, S, P]),

@@ -74,7 +74,8 @@ struct BuildHeader {
    int32_t fb_fail;    // the fallback insert found no slot for a key
    uint64_t n;         // pairs after the first partition pass (the live slots when the input is a table)
    uint64_t fb_len;    // pairs the fallback insert takes from the partitioned array (0: none)
-   uint32_t pad[50];
+   uint64_t prior_fill; // rehash: the old table's fill, which the new one must hold
+   uint32_t pad[48];
 };
 static_assert(sizeof(BuildHeader) == 256, "header is one 256-byte line");
 
@@ -188,24 +189,27 @@ __device__ __forceinline__ uint32_t region_of(K key, int sp, int rlog) {
    return static_cast<uint32_t>((fib_home(key, sp) & mask) >> rlog);
 }
 
-// Build from caller arrays into an empty table, or (rebuild) from the
-// table's own live slots, which is clean_tombstones.
-enum BuildMode { BUILD_EMPTY = 0, BUILD_REBUILD = 1 };
+// Build from caller arrays into an empty table, (rebuild) from the table's
+// own live slots, which is clean_tombstones, or (rehash) from the live slots
+// of another, smaller table into a new one.
+enum BuildMode { BUILD_EMPTY = 0, BUILD_REBUILD = 1, BUILD_REHASH = 2 };
 
 __global__ void k_build_begin(const hn_info* info, BuildHeader* h, int sp, uint64_t len, int mode) {
    if (threadIdx.x == 0) {
       const bool sp_ok = info->size_power == sp;
       if (mode == BUILD_EMPTY)
          h->go = (sp_ok && info->fill == 0 && info->tombstone_counter == 0) ? 1u : 0u;
-      else
+      else if (mode == BUILD_REBUILD)
          h->go = (sp_ok && info->tombstone_counter != 0) ? 1u : 0u;
+      else
+         h->go = 1u; // k_rehash_info has just set the Info up for the new table
       h->fail = 0;
       h->max_probe = 0;
       h->c0 = 0;
       h->n_distinct = 0;
       h->fb_fail = 0;
       h->fb_len = 0;
-      h->n = mode == BUILD_EMPTY ? len : 0; // a rebuild counts the live slots in its first pass
+      h->n = mode == BUILD_EMPTY ? len : 0; // a rebuild or rehash counts the live slots in its first pass
    }
 }
 
@@ -956,6 +960,24 @@ __global__ void k_build_end(hn_info* info, const BuildHeader* h, int mode) {
       info->err = HN_STATUS_SUCCESS;
       info->reserved1 = kInfoOrdered;
    }
+   // device_rehash reports success iff the new table holds the old fill (hashinator.h:333)
+   if (mode == BUILD_REHASH && info->err == HN_STATUS_SUCCESS && info->fill != h->prior_fill)
+      info->err = HN_STATUS_FAIL;
+}
+
+// Rehash: remember the old fill and give the Info the new table's size
+// (Info(newSizePower), hashinator.h:328).  The build writes every slot of the
+// new table, so it needs no initialisation.
+__global__ void k_rehash_info(hn_info* info, int new_sp, BuildHeader* h) {
+   if (threadIdx.x != 0) return;
+   h->prior_fill = info->fill;
+   info->size_power = new_sp;
+   info->reserved0 = 0;
+   info->fill = 0;
+   info->current_max_bucket_overflow = kBucketOverflow;
+   info->tombstone_counter = 0;
+   info->err = HN_STATUS_SUCCESS;
+   info->reserved1 = 0;
 }
 
 template <typename K, int SRC, bool FILTER>
@@ -1087,6 +1109,38 @@ int rebuild(void* buckets, hn_info* info, int sp, void* scratch, size_t bytes, h
    PartIn<K> in{nullptr, nullptr, static_cast<const Slot<K>*>(buckets)};
    return build_core<K>(P, BUILD_REBUILD, SRC_TABLE, in, P.N, buckets, info, scratch, s);
 }
+// device_rehash into a larger table as a build: the first partition pass
+// reads the old table's live slots (extractPattern + insert in the reference,
+// hashinator.h:284-336, without the intermediate array).  The new table is
+// Robin Hood ordered.  Growing only: the live pairs then fill at most half of
+// the new table.
+template <typename K>
+bool rehash_plan(int old_sp, int new_sp, BuildPlan* P) {
+   if (old_sp < 1 || new_sp <= old_sp || new_sp > std::min(max_size_power<K>(), 32)) return false;
+   return build_plan<K>(new_sp, 1ull << old_sp, kMaxDigitBits, false, P);
+}
+
+template <typename K>
+size_t rehash_build_scratch_bytes(int old_sp, int new_sp) {
+   BuildPlan P;
+   return rehash_plan<K>(old_sp, new_sp, &P) ? P.total : 0;
+}
+
+template <typename K>
+int rehash_build(const void* old_b, int old_sp, void* new_b, int new_sp, hn_info* info, void* scratch, size_t bytes,
+                 hipStream_t s) {
+   if (!old_b || !new_b || !info) return HN_EINVAL;
+   BuildPlan P;
+   if (!rehash_plan<K>(old_sp, new_sp, &P)) return HN_EINVAL;
+   const char* o0 = static_cast<const char*>(old_b);
+   const char* n0 = static_cast<const char*>(new_b);
+   if (o0 < n0 + P.N * sizeof(Slot<K>) && n0 < o0 + (1ull << old_sp) * sizeof(Slot<K>)) return HN_EINVAL;
+   if (!scratch || bytes < P.total) return HN_ESCRATCH;
+   hipLaunchKernelGGL(k_rehash_info, dim3(1), dim3(64), 0, s, info, new_sp, static_cast<BuildHeader*>(scratch));
+   PartIn<K> in{nullptr, nullptr, static_cast<const Slot<K>*>(old_b)};
+   return build_core<K>(P, BUILD_REHASH, SRC_TABLE, in, 1ull << old_sp, new_b, info, scratch, s);
+}
+
 template size_t rebuild_scratch_bytes<uint32_t>(int);
 template size_t rebuild_scratch_bytes<uint64_t>(int);
 template int rebuild<uint32_t>(void*, hn_info*, int, void*, size_t, hipStream_t);
@@ -1096,6 +1150,18 @@ template int rebuild<uint64_t>(void*, hn_info*, int, void*, size_t, hipStream_t)
 
 using namespace hn;
 extern "C" {
+size_t hn_rehash_build_scratch_bytes_u32(int old_sp, int new_sp) {
+   return rehash_build_scratch_bytes<uint32_t>(old_sp, new_sp);
+}
+size_t hn_rehash_build_scratch_bytes_u64(int old_sp, int new_sp) {
+   return rehash_build_scratch_bytes<uint64_t>(old_sp, new_sp);
+}
+int hn_rehash_build_u32(const void* ob, int osp, void* nb, int nsp, hn_info* i, void* sc, size_t bytes, hn_stream_t s) {
+   return rehash_build<uint32_t>(ob, osp, nb, nsp, i, sc, bytes, s);
+}
+int hn_rehash_build_u64(const void* ob, int osp, void* nb, int nsp, hn_info* i, void* sc, size_t bytes, hn_stream_t s) {
+   return rehash_build<uint64_t>(ob, osp, nb, nsp, i, sc, bytes, s);
+}
 size_t hn_build_scratch_bytes_u32(int sp, size_t len) { return build_scratch_bytes<uint32_t>(sp, len); }
 size_t hn_build_scratch_bytes_u64(int sp, size_t len) { return build_scratch_bytes<uint64_t>(sp, len); }
 int hn_build_u32(const void* k, const void* v, void* b, hn_info* i, size_t n, int sp, void* sc, size_t sb,

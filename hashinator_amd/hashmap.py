@@ -325,12 +325,24 @@ class Hashmap:
         if fill > 0:
             newSizePower = max(newSizePower, math.ceil(math.log2(fill)))
         nb = self._new_table(newSizePower)
-        scratch = torch.empty(self._L.hn_rehash_scratch_bytes(), dtype=torch.uint8, device=self.device)
-        check(self._fn("rehash")(_ptr(self.buckets), self._sp, _ptr(nb), newSizePower, _ptr(self._info),
-                                 _ptr(scratch), _stream(stream)), "device_rehash")
+        # Growing a map of many keys: one bulk build from the old table's live
+        # slots (Robin Hood order directly); else reinsert them by probing.
+        sb = 0
+        if self.auto_build and newSizePower > self._sp and fill >= self.build_min_keys:
+            sb = getattr(self._L, f"hn_rehash_build_scratch_bytes_{self._sfx}")(self._sp, newSizePower)
+        if sb:
+            scratch = torch.empty(sb, dtype=torch.uint8, device=self.device)
+            check(self._fn("rehash_build")(_ptr(self.buckets), self._sp, _ptr(nb), newSizePower, _ptr(self._info),
+                                           _ptr(scratch), sb, _stream(stream)), "device_rehash")
+        else:
+            scratch = torch.empty(self._L.hn_rehash_scratch_bytes(), dtype=torch.uint8, device=self.device)
+            check(self._fn("rehash")(_ptr(self.buckets), self._sp, _ptr(nb), newSizePower, _ptr(self._info),
+                                     _ptr(scratch), _stream(stream)), "device_rehash")
         self.buckets = nb
         self._sp = newSizePower
-        if self.auto_reorder:
+        if sb:
+            self._sync(stream)  # the order flag is read next (skewed keys leave the table unordered)
+        if self.auto_reorder and not (sb and self.ordered()):
             self._reorder(stream)
         self._sync(stream)
 

@@ -247,17 +247,30 @@ public:
       }
       pair_t* fresh = nullptr;
       SPLIT_CHECK_ERR(hipMalloc(&fresh, (size_t(1) << newSizePower) * sizeof(pair_t)));
+      const int osp = static_cast<int>(_info.sizePower);
+      // Growing a map of many keys: one bulk build from the old table's live
+      // slots (Robin Hood order directly); else reinsert them by probing.
+      size_t sb = 0;
+      if (_auto_build > 0.0f && newSizePower > osp && fill >= _build_min_keys)
+         sb = W64 ? hn_rehash_build_scratch_bytes_u64(osp, newSizePower)
+                  : hn_rehash_build_scratch_bytes_u32(osp, newSizePower);
       void* scratch = nullptr;
-      SPLIT_CHECK_ERR(hipMallocAsync(&scratch, hn_rehash_scratch_bytes(), s));
-      chk(W64 ? hn_rehash_u64(_buckets, _info.sizePower, fresh, newSizePower, dinfo(), scratch, st(s))
-              : hn_rehash_u32(_buckets, _info.sizePower, fresh, newSizePower, dinfo(), scratch, st(s)));
+      if (sb) {
+         SPLIT_CHECK_ERR(hipMallocAsync(&scratch, sb, s));
+         chk(W64 ? hn_rehash_build_u64(_buckets, osp, fresh, newSizePower, dinfo(), scratch, sb, st(s))
+                 : hn_rehash_build_u32(_buckets, osp, fresh, newSizePower, dinfo(), scratch, sb, st(s)));
+      } else {
+         SPLIT_CHECK_ERR(hipMallocAsync(&scratch, hn_rehash_scratch_bytes(), s));
+         chk(W64 ? hn_rehash_u64(_buckets, osp, fresh, newSizePower, dinfo(), scratch, st(s))
+                 : hn_rehash_u32(_buckets, osp, fresh, newSizePower, dinfo(), scratch, st(s)));
+      }
       SPLIT_CHECK_ERR(hipFreeAsync(scratch, s));
       SPLIT_CHECK_ERR(hipStreamSynchronize(s));
       SPLIT_CHECK_ERR(hipFree(_buckets));
       _buckets = fresh;
       _nbuckets = size_t(1) << newSizePower;
       changed(s);
-      if (_auto_reorder > 0.0f) reorder(s);
+      if (_auto_reorder > 0.0f && !(sb && ordered())) reorder(s);
    }
 
    template <bool prefetches = true>

@@ -274,6 +274,25 @@ HN_API int hn_build_pairs_u32(const void* pairs, void* buckets, hn_info* info, s
 HN_API int hn_build_pairs_u64(const void* pairs, void* buckets, hn_info* info, size_t len, int size_power,
                               void* scratch, size_t scratch_bytes, hn_stream_t s, const hn_tune* t);
 
+/* Hashmap::device_rehash into a LARGER table (hashinator.h:284-336: extract
+ * the valid elements, Info(newSizePower), insert them) as one bulk build
+ * whose first partition pass reads old_buckets' live slots: no intermediate
+ * array, no probing, and the new table comes out Robin Hood ordered
+ * (HN_INFO_ORDERED), the image hn_rehash_* followed by hn_reorder_* gives.
+ * Needs old_sp < new_sp <= 32 (else HN_EINVAL: hn_rehash_* handles equal and
+ * smaller sizes), buffers that do not overlap, and `scratch` of
+ * hn_rehash_build_scratch_bytes_*(old_sp, new_sp) bytes (about 16 (u64) or
+ * 8 (u32) bytes per OLD slot, plus bookkeeping).  new_buckets need not be
+ * initialised.  *info becomes Info(new_sp) plus the elements; err = SUCCESS
+ * iff the fill is preserved.  Strongly skewed keys (a region over capacity)
+ * are placed by the probing insert instead, as in hn_build_*. */
+HN_API size_t hn_rehash_build_scratch_bytes_u32(int old_sp, int new_sp);
+HN_API size_t hn_rehash_build_scratch_bytes_u64(int old_sp, int new_sp);
+HN_API int hn_rehash_build_u32(const void* old_buckets, int old_sp, void* new_buckets, int new_sp, hn_info* info,
+                               void* scratch, size_t scratch_bytes, hn_stream_t s);
+HN_API int hn_rehash_build_u64(const void* old_buckets, int old_sp, void* new_buckets, int new_sp, hn_info* info,
+                               void* scratch, size_t scratch_bytes, hn_stream_t s);
+
 /* ---------------------------------------------------------------- compaction and scan */
 
 /* Scratch for one compaction or scan of n elements

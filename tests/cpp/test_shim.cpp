@@ -162,6 +162,19 @@ bool resize_and_grow() {
    CHECK(hmap.getSizePower() == 16 && hmap.peek_status() == status::success && hmap.size() == n);
    hmap.resize(17, targets::host);
    CHECK(hmap.getSizePower() >= 17 && hmap.size() == n);
+   {
+      // growing a map of many keys runs the rehash as one bulk build (the key threshold lowered here)
+      Map hb(14);
+      hb.setAutoBuild(1.0f / 16, 0);
+      hb.insert(keys.data(), vals.data(), n, 1.0f);
+      hb.resize(16, targets::device);
+      CHECK(hb.getSizePower() == 16 && hb.peek_status() == status::success && hb.size() == n && hb.ordered());
+      hb.optimizeCPU();
+      for (size_t i = 0; i < n; ++i) {
+         auto it = hb.find(keys[i]);
+         CHECK(it != hb.end() && it->second == vals[i]);
+      }
+   }
    hmap.optimizeCPU();
    for (size_t i = 0; i < n; ++i) {
       auto it = hmap.find(keys[i]);

@@ -51,12 +51,17 @@ def test_scratch_sizing_host_only():
     assert L.hn_clean_tombstones_scratch_bytes_u32(0, 0) == 0  # invalid size power
     assert L.hn_compact_scratch_bytes(1 << 28) >= (1 << 28) // 8192 * 16  # two 8-byte state words per 8192-element tile
     assert L.hn_rehash_scratch_bytes() >= 8
+    # rehash as a build: growing only; about one pair per old slot plus bookkeeping
+    assert L.hn_rehash_build_scratch_bytes_u64(20, 20) == 0 and L.hn_rehash_build_scratch_bytes_u64(20, 19) == 0
+    assert L.hn_rehash_build_scratch_bytes_u64(20, 21) >= (1 << 20) * 16
+    assert L.hn_rehash_build_scratch_bytes_u32(20, 22) >= (1 << 20) * 8
 
 
 def test_invalid_arguments_are_rejected_without_a_gpu():
     L = _lib.load()
     assert L.hn_clean_tombstones_u64(None, None, 10, None, 0, None) == _lib.HN_EINVAL
     assert L.hn_rehash_u32(None, 10, None, 11, None, None, None) == _lib.HN_EINVAL
+    assert L.hn_rehash_build_u64(None, 10, None, 11, None, None, 0, None) == _lib.HN_EINVAL
     assert L.hn_insert_u64(None, None, None, None, 10, None, None) == _lib.HN_EINVAL
     t = _lib.hn_tune(3, 256, 0, 0)  # probe width must be a power of two
     buf = (C.c_uint64 * 8)()
