@@ -141,8 +141,8 @@ def main():
                     help="also run one plain 1 GiB device copy (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus > 1 is not supported "
-                                   "(maps shard by key hash across GPUs, see DESIGN.md)"}))
+        print(json.dumps({"error": "bench.py runs on one GPU; --gpus > 1 is not supported "
+                                   "(a map would shard by key hash across GPUs, one shard per GPU)"}))
         sys.exit(2)
 
     import torch

@@ -23,7 +23,7 @@
  * Instantiations: Hashmap<uint32_t,uint32_t> and Hashmap<uint64_t,uint64_t>
  * with the default sentinels and hash.
  *
- * Not part of this build (the reference's device-only API and host-side
+ * Not provided here (the reference's device-only API and host-side
  * mutation): warpInsert/warpFind/warpErase, device_insert/device_find/
  * device_erase and device iterators, upload/download handles, and the host
  * insertion path (_at, operator[], erase(key) on the host).

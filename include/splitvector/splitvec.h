@@ -9,7 +9,7 @@
  *
  * Differences by design: the size and capacity are host members (the
  * device-side mutators that needed them in unified memory -- device_push_back,
- * device_insert, device_resize -- are not part of this build); large
+ * device_insert, device_resize -- are not provided here); large
  * trivially-copyable fills run as a device kernel instead of a host loop.
  */
 #pragma once

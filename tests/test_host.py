@@ -96,7 +96,7 @@ def test_velocity_grid_keys():
 def test_algorithmic_bytes_model():
     import bench
 
-    # survey worked numbers at alpha = 0.9, 16 B slots: hit 152 B, miss 872 B, insert 216 B
+    # worked numbers of the byte model at alpha = 0.9, 16 B slots: hit 152 B, miss 872 B, insert 216 B
     assert abs(bench.alg_bytes_retrieve(1, 0, 0.9) - 152) < 1e-9
     assert abs(bench.alg_bytes_retrieve(0, 1, 0.9) - 872) < 1e-9
     assert abs(bench.alg_bytes_insert(1, 0.9) - 216) < 1e-9
