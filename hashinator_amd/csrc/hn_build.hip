@@ -95,15 +95,19 @@ struct BuildPlan {
    int dbits[3];        // digit bits per pass (low digit first)
    uint32_t nblocks;    // partition tiles
    size_t off_pairs, off_mat, off_scan, off_start, off_word, off_spec, off_carry, off_mark, off_list, off_blk, total;
+   size_t off_pairs2;   // a second pair array for the passes that cannot use the table's memory (0: none)
 };
 
 inline size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 
 // dcap: widest partition digit (1..9 bits); narrower digits mean more passes.
-// rebuild: the input is the table itself (n = 2^sp slots, live ones kept);
-// the pass count is made odd so that the first pass writes the scratch.
+// rebuild: the input is the table itself (n = 2^sp slots, live ones kept), so
+// no pass may write the table's memory before the last has read it: with
+// `two` a second pair array in the scratch takes the first of two passes,
+// otherwise the pass count is made odd (3) so that the first pass writes the
+// scratch and the second the table.
 template <typename K>
-bool build_plan(int sp, uint64_t n, int dcap, bool rebuild, BuildPlan* P) {
+bool build_plan(int sp, uint64_t n, int dcap, bool rebuild, BuildPlan* P, bool two = false) {
    if (dcap < 1 || dcap > kMaxDigitBits) return false;
    if (sp < 1 || sp > std::min(max_size_power<K>(), 32)) return false;
    P->sp = sp;
@@ -115,7 +119,8 @@ bool build_plan(int sp, uint64_t n, int dcap, bool rebuild, BuildPlan* P) {
    P->rbits = sp - P->rlog;
    P->NR = 1ull << P->rbits;
    P->passes = std::max(1, (P->rbits + dcap - 1) / dcap);
-   if (rebuild && P->passes == 2) P->passes = 3;
+   const bool second = rebuild && two && P->passes == 2;
+   if (rebuild && !second && P->passes == 2) P->passes = 3;
    if (P->passes > 3) return false;
    int left = P->rbits;
    for (int i = 0; i < P->passes; ++i) {
@@ -130,6 +135,11 @@ bool build_plan(int sp, uint64_t n, int dcap, bool rebuild, BuildPlan* P) {
    size_t o = sizeof(BuildHeader);
    P->off_pairs = o;
    o = align256(o + n * sizeof(Slot<K>));
+   P->off_pairs2 = 0;
+   if (second) {
+      P->off_pairs2 = o;
+      o = align256(o + n * sizeof(Slot<K>));
+   }
    P->off_mat = o;
    o = align256(o + mat * sizeof(uint32_t));
    P->off_scan = o;
@@ -1024,12 +1034,14 @@ int build_core(const BuildPlan& P, int mode, int src0, PartIn<K> in0, uint64_t n
    Slot<K>* T = static_cast<Slot<K>*>(buckets);
 
    hipLaunchKernelGGL(k_build_begin, dim3(1), dim3(64), 0, s, info, h, P.sp, static_cast<uint64_t>(P.n), mode);
-   // Pass i writes S when (passes - 1 - i) is even, the table's memory
-   // otherwise, so the last pass lands in S; a rebuild has an odd number of
+   // Pass i writes S when (passes - 1 - i) is even, the table's memory (or
+   // the second pair array, when the plan has one) otherwise, so the last
+   // pass lands in S; a rebuild without a second array has an odd number of
    // passes, so its first pass (which reads the table) writes S.
+   Slot<K>* const alt = P.off_pairs2 ? reinterpret_cast<Slot<K>*>(base + P.off_pairs2) : T;
    const Slot<K>* src = nullptr;
    for (int pass = 0; pass < P.passes; ++pass) {
-      Slot<K>* dst = ((P.passes - 1 - pass) % 2 == 0) ? S : T;
+      Slot<K>* dst = ((P.passes - 1 - pass) % 2 == 0) ? S : alt;
       int rc;
       if (pass == 0) {
          switch (src0) {
@@ -1094,17 +1106,20 @@ size_t build_scratch_bytes(int sp, size_t len) {
    return build_plan<K>(sp, len, kMaxDigitBits, false, &P) ? P.total : 0;
 }
 
-// clean_tombstones as a rebuild of the live slots (see hn_map.hip).
+// clean_tombstones as a rebuild of the live slots (see hn_map.hip).  `two`:
+// the plan with a second pair array (one partition pass fewer where the
+// plan has two, the same plan otherwise).
 template <typename K>
-size_t rebuild_scratch_bytes(int sp) {
+size_t rebuild_scratch_bytes(int sp, bool two) {
    BuildPlan P;
-   return (sp >= 1 && sp <= 32 && build_plan<K>(sp, 1ull << sp, kMaxDigitBits, true, &P)) ? P.total : 0;
+   return (sp >= 1 && sp <= 32 && build_plan<K>(sp, 1ull << sp, kMaxDigitBits, true, &P, two)) ? P.total : 0;
 }
 
 template <typename K>
 int rebuild(void* buckets, hn_info* info, int sp, void* scratch, size_t bytes, hipStream_t s) {
    BuildPlan P;
-   if (!build_plan<K>(sp, 1ull << sp, kMaxDigitBits, true, &P)) return HN_EINVAL;
+   if (!build_plan<K>(sp, 1ull << sp, kMaxDigitBits, true, &P, true)) return HN_EINVAL;
+   if (bytes < P.total && !build_plan<K>(sp, 1ull << sp, kMaxDigitBits, true, &P, false)) return HN_EINVAL;
    if (!scratch || bytes < P.total) return HN_ESCRATCH;
    PartIn<K> in{nullptr, nullptr, static_cast<const Slot<K>*>(buckets)};
    return build_core<K>(P, BUILD_REBUILD, SRC_TABLE, in, P.N, buckets, info, scratch, s);
@@ -1141,8 +1156,8 @@ int rehash_build(const void* old_b, int old_sp, void* new_b, int new_sp, hn_info
    return build_core<K>(P, BUILD_REHASH, SRC_TABLE, in, 1ull << old_sp, new_b, info, scratch, s);
 }
 
-template size_t rebuild_scratch_bytes<uint32_t>(int);
-template size_t rebuild_scratch_bytes<uint64_t>(int);
+template size_t rebuild_scratch_bytes<uint32_t>(int, bool);
+template size_t rebuild_scratch_bytes<uint64_t>(int, bool);
 template int rebuild<uint32_t>(void*, hn_info*, int, void*, size_t, hipStream_t);
 template int rebuild<uint64_t>(void*, hn_info*, int, void*, size_t, hipStream_t);
 

@@ -21,9 +21,11 @@
 namespace hn {
 
 // The rebuild path of clean_tombstones (hn_build.hip): every live slot goes
-// through the bulk build into the same table, in Robin Hood order.
+// through the bulk build into the same table, in Robin Hood order.  With
+// `two`, the scratch that also holds a second pair array (one partition pass
+// fewer at 2^21..2^29 slots).
 template <typename K>
-size_t rebuild_scratch_bytes(int sp);
+size_t rebuild_scratch_bytes(int sp, bool two);
 template <typename K>
 int rebuild(void* buckets, hn_info* info, int sp, void* scratch, size_t bytes, hipStream_t s);
 
@@ -498,8 +500,10 @@ size_t clean_scratch_bytes(int sp, size_t budget) {
    if (sp < 1 || sp > max_size_power<K>()) return 0;
    const uint64_t n = 1ull << sp;
    const size_t full = clean_header_bytes(1) + n * sizeof(Slot<K>);
-   const size_t rb = sp <= 32 ? rebuild_scratch_bytes<K>(sp) : 0;
-   if (budget == 0) return std::max(full, rb);
+   const size_t rb = sp <= 32 ? rebuild_scratch_bytes<K>(sp, false) : 0;
+   const size_t rb2 = sp <= 32 ? rebuild_scratch_bytes<K>(sp, true) : 0;
+   if (budget == 0) return std::max(full, rb2);
+   if (rb2 && budget >= rb2) return rb2;
    if (rb && budget >= rb) return rb;
    if (budget >= full) return full;
    uint64_t rs, nr;
@@ -526,7 +530,7 @@ template <typename K>
 int clean_tombstones(void* buckets, hn_info* info, int sp, void* scratch, size_t bytes, hipStream_t s) {
    if (!buckets || !info || !scratch) return HN_EINVAL;
    if (sp < 1 || sp > max_size_power<K>()) return HN_EINVAL;
-   const size_t rb = sp <= 32 ? rebuild_scratch_bytes<K>(sp) : 0;
+   const size_t rb = sp <= 32 ? rebuild_scratch_bytes<K>(sp, false) : 0;
    if (rb && bytes >= rb) return rebuild<K>(buckets, info, sp, scratch, bytes, s);
    const uint64_t n = 1ull << sp;
    uint64_t rs = 0, nr = 0;

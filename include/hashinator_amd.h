@@ -200,9 +200,11 @@ HN_API size_t hn_clean_tombstones_min_scratch(void);
  * fill is unchanged, overflow = max(64, nextOverflow(longest probe)), and the
  * occupied-slot set is the one a fresh insert of the live set gives.
  * With scratch of hn_clean_tombstones_scratch_bytes_*(size_power, 0) bytes
- * (about the table's size) the live slots are rebuilt in one pass through
- * the bulk build (hn_build_*), leaving the table in Robin Hood order
- * (HN_INFO_ORDERED).  With less, the table is processed region by region
+ * (about twice the table's size) the live slots are rebuilt through the bulk
+ * build (hn_build_*), leaving the table in Robin Hood order
+ * (HN_INFO_ORDERED); about the table's size also rebuilds, with one more
+ * partition pass at 2^21..2^29 slots (the scratch_bytes query with a budget
+ * between the two returns the smaller).  With less, the table is processed region by region
  * starting after an EMPTY slot: every live element not in its home slot is
  * lifted out and reinserted, so scratch is bounded by the region size; a
  * table with no EMPTY slot at all then needs scratch for the whole table,

@@ -205,10 +205,14 @@ def test_build_needs_no_initialised_table(gpu, bits, case):
 
 
 @pytest.mark.parametrize("bits", [32, 64])
-@pytest.mark.parametrize("sp,lf", [(10, 0.9), (14, 0.9), (20, 0.95), (21, 0.85)])
-def test_clean_rebuild_image_exact(gpu, bits, sp, lf):
+@pytest.mark.parametrize("sp,lf,scratch", [(10, 0.9, "two arrays"), (14, 0.9, "two arrays"), (20, 0.95, "two arrays"),
+                                           (21, 0.85, "two arrays"), (21, 0.85, "one array"),
+                                           (22, 0.9, "two arrays"), (22, 0.9, "one array")])
+def test_clean_rebuild_image_exact(gpu, bits, sp, lf, scratch):
     """clean_tombstones with whole-table scratch rebuilds the live slots: the
-    image is the oracle's clean followed by its reorder, bit for bit."""
+    image is the oracle's clean followed by its reorder, bit for bit.  With
+    scratch for two pair arrays a 2^21+ table takes two partition passes,
+    with one array three."""
     keys = keyset(int(lf * (1 << sp)), bits, seed=sp + bits)
     vals = (keys * NDT[bits](9)).astype(NDT[bits])
     hm = Hashmap(sp, bits, auto_build=0, auto_reorder=0)
@@ -217,7 +221,12 @@ def test_clean_rebuild_image_exact(gpu, bits, sp, lf):
     hm.erase(er)
     m = O.GpuModel(sp, bits, buckets=hm.host_buckets().reshape(-1).copy(), info=O.Info(
         sp, 0, hm.info().fill, hm.info().current_max_bucket_overflow, hm.info().tombstone_counter, 0, 0))
-    hm.clean_tombstones()  # default: whole-table scratch -> rebuild
+    L = _lib.load()
+    q = getattr(L, f"hn_clean_tombstones_scratch_bytes_u{bits}")
+    full = q(sp, 0)
+    budget = 0 if scratch == "two arrays" else full - 1
+    assert scratch == "two arrays" or full - q(sp, budget) >= (1 << sp) * (bits // 4)  # a smaller rebuild plan
+    hm.clean_tombstones(scratch_budget=budget)  # whole-table scratch -> rebuild
     m.clean_tombstones()
     assert m.reorder() == 1
     # every key slot exact; values exact in live slots (an EMPTY slot's value is unspecified:

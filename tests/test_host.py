@@ -46,6 +46,8 @@ def test_scratch_sizing_host_only():
     for sp in (10, 20, 28):
         full = L.hn_clean_tombstones_scratch_bytes_u64(sp, 0)
         assert full >= (1 << sp) * 16
+        one = L.hn_clean_tombstones_scratch_bytes_u64(sp, full - 1)  # rebuild through one pair array
+        assert (1 << sp) * 16 <= one <= full and (sp < 21 or full - one >= (1 << sp) * 16)
         b = L.hn_clean_tombstones_scratch_bytes_u64(sp, 1 << 20)
         assert min(L.hn_clean_tombstones_min_scratch(), full) <= b <= max(1 << 20, full)
     assert L.hn_clean_tombstones_scratch_bytes_u32(0, 0) == 0  # invalid size power
