@@ -90,7 +90,7 @@ class _Map:
 
 class ServingEngine:
     def __init__(self, sps, key_bits=64, n_streams=8, hbm_budget_bytes=32 << 30, grow_lf=0.85, device="cuda",
-                 tune=None):
+                 tune=None, build_rehash_min=1 << 19):
         self.kb = key_bits
         self.sfx = "u32" if key_bits == 32 else "u64"
         self.slot = 8 if key_bits == 32 else 16
@@ -98,6 +98,7 @@ class ServingEngine:
         self.L = _lib.load()
         self.dev = torch.device(device)
         self.grow_lf = grow_lf
+        self.build_rehash_min = build_rehash_min  # grow maps of at least this many keys by the bulk rehash
         self.budget = hbm_budget_bytes
         self.tune = C.byref(tune) if tune is not None else None
         self.maps = []
@@ -140,6 +141,11 @@ class ServingEngine:
         check(getattr(L, f"hn_erase_{sfx}")(k.data_ptr(), b.data_ptr(), info.data_ptr(), 16, s, self.tune), "warm-up")
         check(getattr(L, f"hn_rehash_{sfx}")(b.data_ptr(), sp, nb.data_ptr(), sp + 1, info.data_ptr(), sc.data_ptr(), s),
               "warm-up")
+        nb2 = torch.empty((1 << (sp + 2), 2), dtype=_TDT[self.kb], device=self.dev)
+        sb2 = getattr(L, f"hn_rehash_build_scratch_bytes_{sfx}")(sp + 1, sp + 2)
+        sc2 = torch.empty(sb2, dtype=torch.uint8, device=self.dev)
+        check(getattr(L, f"hn_rehash_build_{sfx}")(nb.data_ptr(), sp + 1, nb2.data_ptr(), sp + 2, info.data_ptr(),
+                                                   sc2.data_ptr(), sb2, s), "warm-up")
         torch.cuda.synchronize()
         # pinned result buffers come from torch's caching host allocator: fill
         # its size classes (powers of two, 4 KiB .. 64 MiB, two of each) now
@@ -166,11 +172,27 @@ class ServingEngine:
         kind = reqs[0].kind
         b = n * self.es * (2 if kind != ERASE else 1) + (n if kind == RETRIEVE else 0)
         if kind == INSERT and self._needs_grow(m, n):
-            b += (2 << m.sp) * self.slot + _lib.load().hn_rehash_scratch_bytes()
+            new_sp = self._grow_sp(m, n)
+            b += (1 << new_sp) * self.slot + self._rehash_scratch(m, new_sp)[0]
         return b
 
     def _needs_grow(self, m: _Map, n):
         return m.fill_upper + n > self.grow_lf * (1 << m.sp)
+
+    def _grow_sp(self, m: _Map, n):
+        new_sp = m.sp + 1
+        while m.fill_upper + n > self.grow_lf * (1 << new_sp):
+            new_sp += 1
+        return new_sp
+
+    def _rehash_scratch(self, m: _Map, new_sp):
+        """(bytes, bulk) of a growth's scratch: a map of many keys grows by the
+        bulk rehash (one build from its live slots), a small one by reinsertion."""
+        if m.fill_upper >= self.build_rehash_min:
+            sb = getattr(self.L, f"hn_rehash_build_scratch_bytes_{self.sfx}")(m.sp, new_sp)
+            if sb:
+                return sb, True
+        return self.L.hn_rehash_scratch_bytes(), False
 
     def _take(self, m: _Map):
         """Next group of requests for a map: one insert, or a run of same-kind retrieves/erases."""
@@ -206,13 +228,17 @@ class ServingEngine:
             keep += [hk, dk]
             if kind == INSERT:
                 if self._needs_grow(m, n):
-                    new_sp = m.sp + 1
-                    while m.fill_upper + n > self.grow_lf * (1 << new_sp):
-                        new_sp += 1
+                    new_sp = self._grow_sp(m, n)
                     nb = torch.empty((1 << new_sp, 2), dtype=_TDT[self.kb], device=self.dev)
-                    sc = torch.empty(L.hn_rehash_scratch_bytes(), dtype=torch.uint8, device=self.dev)
-                    check(getattr(L, f"hn_rehash_{sfx}")(m.buckets.data_ptr(), m.sp, nb.data_ptr(), new_sp,
-                                                         m.info.data_ptr(), sc.data_ptr(), s.cuda_stream), "rehash")
+                    sb, bulk = self._rehash_scratch(m, new_sp)
+                    sc = torch.empty(sb, dtype=torch.uint8, device=self.dev)
+                    if bulk:
+                        check(getattr(L, f"hn_rehash_build_{sfx}")(m.buckets.data_ptr(), m.sp, nb.data_ptr(), new_sp,
+                                                                   m.info.data_ptr(), sc.data_ptr(), sb, s.cuda_stream),
+                              "rehash")
+                    else:
+                        check(getattr(L, f"hn_rehash_{sfx}")(m.buckets.data_ptr(), m.sp, nb.data_ptr(), new_sp,
+                                                             m.info.data_ptr(), sc.data_ptr(), s.cuda_stream), "rehash")
                     m.buckets.record_stream(s)
                     keep += [m.buckets, sc]
                     m.buckets, m.sp = nb, new_sp

@@ -25,8 +25,8 @@ def build(seed=1):
     return pre, reqs
 
 
-def engine(pre, n_streams=8, budget=1 << 30):
-    e = ServingEngine(SPS, key_bits=64, n_streams=n_streams, hbm_budget_bytes=budget)
+def engine(pre, n_streams=8, budget=1 << 30, bulk_rehash_min=1 << 19):
+    e = ServingEngine(SPS, key_bits=64, n_streams=n_streams, hbm_budget_bytes=budget, build_rehash_min=bulk_rehash_min)
     for m, (k, v) in enumerate(pre):
         e.preload(m, k, v)
     return e
@@ -38,12 +38,14 @@ def clone(reqs):
     return [Request(r.rid, r.map_id, r.kind, r.keys, r.vals, r.arrival) for r in reqs]
 
 
-@pytest.mark.parametrize("realtime,budget", [(False, 1 << 30), (True, 1 << 30), (False, 200_000)])
-def test_mix_equals_alone_equals_oracle(gpu, realtime, budget):
+@pytest.mark.parametrize("realtime,budget,bulk_rehash_min", [(False, 1 << 30, 1 << 19), (True, 1 << 30, 1 << 19),
+                                                             (False, 200_000, 1 << 19), (False, 1 << 30, 0)])
+def test_mix_equals_alone_equals_oracle(gpu, realtime, budget, bulk_rehash_min):
+    # bulk_rehash_min 0: every growth is the bulk rehash (one build from the live slots)
     pre, reqs = build(seed=3)
     mixed = clone(reqs)
     alone = clone(reqs)
-    e1 = engine(pre, budget=budget)
+    e1 = engine(pre, budget=budget, bulk_rehash_min=bulk_rehash_min)
     wall = e1.serve(mixed, realtime=realtime)
     rep = latency_report(mixed, wall)
     assert rep["requests"] == len(reqs) and all(r.t_done >= r.arrival for r in mixed)
