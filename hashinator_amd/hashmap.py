@@ -399,6 +399,21 @@ class Hashmap:
     def extractAllKeys(self, stream=None):
         return self.extractKeysByPattern(valid_pred(), stream)
 
+    # The *Loop variants (hashinator.h:1186-1259): the reference runs them as
+    # one block looping over the buckets; the same elements and order come
+    # from the single-pass compaction here.
+    def extractPatternLoop(self, pred: hn_pred | None = None, stream=None):
+        return self.extractPattern(pred, stream)
+
+    def extractLoop(self, stream=None):
+        return self.extractPattern(valid_pred(), stream)
+
+    def extractKeysByPatternLoop(self, pred: hn_pred | None = None, stream=None):
+        return self.extractKeysByPattern(pred, stream)
+
+    def extractAllKeysLoop(self, stream=None):
+        return self.extractAllKeys(stream)
+
 
 def valid_pred() -> hn_pred:
     return hn_pred(_lib.HN_PRED_VALID, 0, 0, 0)

@@ -466,6 +466,26 @@ public:
       };
       return extractKeysByPattern<prefetches>(elements, rule, stack, max_size, s);
    }
+   /* The *Loop variants (hashinator.h:1186-1259): the reference runs them as
+    * one block looping over the buckets (split::tools::copy_if_loop) and
+    * resizes `elements` on the device.  Same elements and order here, through
+    * the single-pass compaction; `elements`' size is a host member, so the
+    * call ends with its stream synchronised. */
+   template <typename Rule>
+   void extractPatternLoop(split::SplitVector<pair_t>& elements, Rule rule, hipStream_t s = 0) {
+      extractPattern(elements, rule, s);
+   }
+   void extractLoop(split::SplitVector<pair_t>& elements, hipStream_t s = 0) {
+      auto rule = [] __host__ __device__(const pair_t& kval) -> bool {
+         return kval.first != EMPTYBUCKET && kval.first != TOMBSTONE;
+      };
+      extractPatternLoop(elements, rule, s);
+   }
+   template <typename Rule>
+   void extractKeysByPatternLoop(split::SplitVector<KEY_TYPE>& elements, Rule rule, hipStream_t s = 0) {
+      extractKeysByPattern(elements, rule, s);
+   }
+   void extractAllKeysLoop(split::SplitVector<KEY_TYPE>& elements, hipStream_t s = 0) { extractAllKeys(elements, s); }
 
    /* ------------------------------------------------------------ Robin Hood order (an addition) */
 

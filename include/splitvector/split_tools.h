@@ -189,6 +189,25 @@ void copy_keys_if(SplitVector<T, A>& input, SplitVector<U, B>& output, Rule rule
    output.erase(output.data() + len, output.end());
 }
 
+/* copy_if_loop / copy_if_keys_loop (split_tools.h:994-1020): the entry points
+ * the reference backs with one block looping over the input (loop_compact,
+ * 718-881) for callers that avoid host round trips.  Same result as copy_if
+ * (stable; output's size becomes the count), here through the single-pass
+ * kernel.  The reference resizes the output on the device; this SplitVector
+ * keeps its size on the host, so the call ends with its stream synchronised.
+ * The output grows to the input's size first when it is smaller (the
+ * reference requires that capacity and asserts it). */
+template <typename T, typename Rule, class A>
+void copy_if_loop(SplitVector<T, A>& input, SplitVector<T, A>& output, Rule rule, hipStream_t s = 0) {
+   if (output.size() < input.size()) output.resize(input.size(), true, s);
+   copy_if(input, output, rule, s);
+}
+template <typename T, typename U, typename Rule, class A, class B>
+void copy_if_keys_loop(SplitVector<T, A>& input, SplitVector<U, B>& output, Rule rule, hipStream_t s = 0) {
+   if (output.size() < input.size()) output.resize(input.size(), true, s);
+   copy_keys_if(input, output, rule, s);
+}
+
 /* split_prefix_scan (split_tools.h:363-418): exclusive sum, wrapping in T.
  * Any size (the reference asserts a power of two). */
 template <typename T, class A>

@@ -142,6 +142,32 @@ bool erase_clean_extract(int power) {
    split::SplitVector<K> all;
    CHECK(hmap.extractAllKeys(all) == m);
    for (size_t i = 0; i < m; ++i) CHECK(all[i] == expect[i].first);
+   // the *Loop variants (hashinator.h:1186-1259) give the same elements in the same order
+   split::SplitVector<hash_pair<K, K>> el2;
+   hmap.extractPatternLoop(el2, IsOdd<K>());
+   CHECK(el2.size() == m);
+   for (size_t i = 0; i < m; ++i) CHECK(el2[i] == expect[i]);
+   split::SplitVector<hash_pair<K, K>> el3;
+   hmap.extractLoop(el3);
+   CHECK(el3.size() == hmap.size());
+   split::SplitVector<K> ks2, all2;
+   hmap.extractKeysByPatternLoop(ks2, KeyBelow<K>{K(1000)});
+   CHECK(ks2.size() == kb);
+   for (size_t i = 0; i < kb; ++i) CHECK(ks2[i] == ks[i]);
+   hmap.extractAllKeysLoop(all2);
+   CHECK(all2.size() == m);
+   for (size_t i = 0; i < m; ++i) CHECK(all2[i] == all[i]);
+   // split::tools::copy_if_loop / copy_if_keys_loop on SplitVectors (split_tools.h:994-1020)
+   split::SplitVector<hash_pair<K, K>> in(hmap.bucket_count());
+   SPLIT_CHECK_ERR(hipMemcpy(in.data(), b.data(), b.size() * sizeof(b[0]), hipMemcpyHostToDevice));
+   split::SplitVector<hash_pair<K, K>> out;
+   split::tools::copy_if_loop(in, out, IsOdd<K>());
+   CHECK(out.size() == m);
+   for (size_t i = 0; i < m; ++i) CHECK(out[i] == expect[i]);
+   split::SplitVector<K> kout;
+   split::tools::copy_if_keys_loop(in, kout, IsOdd<K>());
+   CHECK(kout.size() == m);
+   for (size_t i = 0; i < m; ++i) CHECK(kout[i] == expect[i].first);
    return true;
 }
 

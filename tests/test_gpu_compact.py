@@ -94,3 +94,8 @@ def test_extract_pattern_on_map_snapshot(gpu, bits):
     lt = hn_pred(_lib.HN_PRED_LT, 0, 0, 1 << (bits - 1))
     got = to_numpy(hm.extractKeysByPattern(lt), bits)
     assert np.array_equal(got, O.copy_keys_if_pairs(snap, O.make_pred(O.PRED_LT, 0, 0, 1 << (bits - 1), bits), bits))
+    # the *Loop variants return the same elements in the same order
+    assert np.array_equal(to_numpy(hm.extractLoop(), bits).reshape(-1, 2), exp)
+    assert np.array_equal(to_numpy(hm.extractPatternLoop(), bits).reshape(-1, 2), exp)
+    assert np.array_equal(to_numpy(hm.extractAllKeysLoop(), bits), exp[:, 0])
+    assert np.array_equal(to_numpy(hm.extractKeysByPatternLoop(lt), bits), got)
