@@ -25,7 +25,7 @@
  *
  * Not provided here (the reference's device-only API and host-side
  * mutation): warpInsert/warpFind/warpErase, device_insert/device_find/
- * device_erase and device iterators, upload/download handles, and the host
+ * device_erase and device iterators, the upload()/get_device_pointer() handles, and the host
  * insertion path (_at, operator[], erase(key) on the host).
  */
 #pragma once
@@ -342,6 +342,19 @@ public:
       std::swap(_dinfo, other._dinfo);
       std::swap(_info, other._info);
       _view_ok = other._view_ok = false;
+   }
+
+   /* download() (hashinator.h:1457-1467), called after device-side work on the
+    * map: read the Info back; grow one size up when the overflow exceeds
+    * BUCKET_OVERFLOW, else remove tombstones if there are any.  The reference
+    * grows with its host rehash; the device rehash here gives the same
+    * key/value set. */
+   void download(hipStream_t s = 0) {
+      pull(s);
+      if (_info.currentMaxBucketOverflow > static_cast<size_t>(defaults::BUCKET_OVERFLOW))
+         device_rehash(_info.sizePower + 1, s);
+      else if (_info.tombstoneCounter > 0)
+         clean_tombstones(s);
    }
 
    /* GPU performCleanupTasks (hashinator.h:642-650). */

@@ -189,6 +189,17 @@ bool resize_and_grow() {
    hmap.resize(17, targets::host);
    CHECK(hmap.getSizePower() >= 17 && hmap.size() == n);
    {
+      // download() (hashinator.h:1457-1467) after device-side work: tombstones go, the set stays
+      Map hd(14);
+      hd.insert(keys.data(), vals.data(), n, 1.0f);
+      hd.erase(keys.data(), n / 4);
+      CHECK(hd.tombstone_count() == n / 4);
+      hd.download();
+      CHECK(hd.tombstone_count() == 0 && hd.size() == n - n / 4 && hd.getSizePower() == 14);
+      hd.optimizeCPU();
+      for (size_t i = 0; i < n; ++i) CHECK((hd.find(keys[i]) != hd.end()) == (i >= n / 4));
+   }
+   {
       // growing a map of many keys runs the rehash as one bulk build (the key threshold lowered here)
       Map hb(14);
       hb.setAutoBuild(1.0f / 16, 0);
