@@ -23,9 +23,10 @@
  * Instantiations: Hashmap<uint32_t,uint32_t> and Hashmap<uint64_t,uint64_t>
  * with the default sentinels and hash.
  *
- * Not provided here (the reference's device-only API and host-side
- * mutation): warpInsert/warpFind/warpErase, device_insert/device_find/
- * device_erase and device iterators, the upload()/get_device_pointer() handles, and the host
+ * Of the reference's device-only API, upload()/get_device_pointer() and the
+ * warp members (warpInsert, warpInsert_V, warpFind, warpErase) are provided;
+ * not provided: device_insert/device_find/device_erase,
+ * set_element/insert_element/read_element, device iterators, and the host
  * insertion path (_at, operator[], erase(key) on the host).
  */
 #pragma once
@@ -66,6 +67,7 @@ class Hashmap {
    mutable MapInfo _info;      // host mirror of *_dinfo
    mutable std::vector<pair_t> _view;
    mutable bool _view_ok = false;
+   Hashmap* _device_map = nullptr; // upload(): this object's copy in device memory
    hn_tune _tune{0, 0, 0, 0};
    bool _has_tune = false;
    float _auto_reorder = 1.0f / 16; // reorder after inserts of at least this fraction of the buckets
@@ -106,8 +108,10 @@ class Hashmap {
    void release() {
       if (_buckets) SPLIT_CHECK_ERR(hipFree(_buckets));
       if (_dinfo) SPLIT_CHECK_ERR(hipFree(_dinfo));
+      if (_device_map) SPLIT_CHECK_ERR(hipFree(_device_map));
       _buckets = nullptr;
       _dinfo = nullptr;
+      _device_map = nullptr;
       _nbuckets = 0;
    }
    const std::vector<pair_t>& view() const {
@@ -134,9 +138,10 @@ public:
    }
    Hashmap(Hashmap&& other) noexcept
        : _buckets(other._buckets), _nbuckets(other._nbuckets), _dinfo(other._dinfo), _info(other._info),
-         _tune(other._tune), _has_tune(other._has_tune) {
+         _device_map(other._device_map), _tune(other._tune), _has_tune(other._has_tune) {
       other._buckets = nullptr;
       other._dinfo = nullptr;
+      other._device_map = nullptr;
       other._nbuckets = 0;
    }
    Hashmap& operator=(const Hashmap& other) {
@@ -151,6 +156,7 @@ public:
       std::swap(_buckets, other._buckets);
       std::swap(_dinfo, other._dinfo);
       std::swap(_nbuckets, other._nbuckets);
+      std::swap(_device_map, other._device_map);
       _info = other._info;
       _view_ok = false;
       return *this;
@@ -340,17 +346,156 @@ public:
       std::swap(_buckets, other._buckets);
       std::swap(_nbuckets, other._nbuckets);
       std::swap(_dinfo, other._dinfo);
+      std::swap(_device_map, other._device_map);
       std::swap(_info, other._info);
       _view_ok = other._view_ok = false;
    }
 
+   /* ------------------------------------------------------------ device-side API (hashinator.h:853-1126)
+    *
+    * upload() / get_device_pointer() (hashinator.h:1416-1438) give user
+    * kernels this map object in device memory; its warp* members run there,
+    * one call per wave: all 64 lanes call together with the same key, w_tid =
+    * the lane's index in the wave (0..63), and the lanes probe 64 consecutive
+    * slots per step from the key's home.  Upload again after anything that
+    * replaces the bucket array (an insert that grows, rehash, resize,
+    * clear(host)); call download() after the kernels, before host-side use.
+    * Loops are bounded by the table size.  Differences from the reference's
+    * device code: a lookup takes the first match before the first EMPTY slot
+    * in probe order (the reference lets a match later in the same 64-slot
+    * window win; identical on tables these operations produce); the overflow
+    * is the cyclic probe distance (the reference takes the whole table size
+    * for a probe that wraps past the end); a new key clears the Robin Hood
+    * order flag, which the library's host-side lookups use. */
+   template <bool prefetches = true>
+   Hashmap* upload(hipStream_t s = 0) {
+      if (!_device_map) SPLIT_CHECK_ERR(hipMalloc(&_device_map, sizeof(Hashmap)));
+      SPLIT_CHECK_ERR(hipMemcpyAsync(_device_map, this, sizeof(Hashmap), hipMemcpyHostToDevice, s));
+      SPLIT_CHECK_ERR(hipStreamSynchronize(s)); // the source is this host object
+      return _device_map;
+   }
+   Hashmap* get_device_pointer() { return _device_map; }
+
+   /* warpInsert (hashinator.h:853-936): insert or overwrite candidateKey.
+    * A key already present within the window wins over EMPTY slots; EMPTY
+    * slots are claimed by CAS in lane order; tombstones are not reused. */
+   template <bool skipOverWrites = false>
+   HASHINATOR_DEVICEONLY void warpInsert(const KEY_TYPE& candidateKey, const VAL_TYPE& candidateVal,
+                                         const size_t w_tid) noexcept {
+      (void)warp_insert<skipOverWrites>(candidateKey, candidateVal, w_tid);
+   }
+   /* warpInsert_V (938-1025): the same, true when this call added the key. */
+   template <bool skipOverWrites = false>
+   HASHINATOR_DEVICEONLY bool warpInsert_V(const KEY_TYPE& candidateKey, const VAL_TYPE& candidateVal,
+                                           const size_t w_tid) noexcept {
+      return warp_insert<skipOverWrites>(candidateKey, candidateVal, w_tid);
+   }
+   /* warpFind (1027-1075): every lane's candidateVal becomes the key's value
+    * when found, and is left as it was otherwise. */
+   HASHINATOR_DEVICEONLY void warpFind(const KEY_TYPE& candidateKey, VAL_TYPE& candidateVal,
+                                       const size_t w_tid) const noexcept {
+      const int sp = _dinfo->sizePower;
+      const uint64_t N = uint64_t(1) << sp, mask = N - 1;
+      const uint64_t home = static_cast<uint64_t>(HashFunction::_hash(candidateKey, sp)) & mask;
+      for (uint64_t i = 0; i < N; i += defaults::WARPSIZE) {
+         const pair_t e = _buckets[(home + i + w_tid) & mask];
+         const uint64_t hit = first_before_empty(__ballot(e.first == candidateKey), __ballot(e.first == EMPTYBUCKET));
+         if (hit) {
+            candidateVal = __shfl(e.second, __ffsll(static_cast<unsigned long long>(hit)) - 1, defaults::WARPSIZE);
+            return;
+         }
+         if (__ballot(e.first == EMPTYBUCKET)) return;
+      }
+   }
+   /* warpErase (1077-1126): the key's slot becomes TOMBSTONE (CAS, so a key
+    * erased twice at once counts once); fill -1, tombstones +1. */
+   HASHINATOR_DEVICEONLY void warpErase(const KEY_TYPE& candidateKey, const size_t w_tid) noexcept {
+      const int sp = _dinfo->sizePower;
+      const uint64_t N = uint64_t(1) << sp, mask = N - 1;
+      const uint64_t home = static_cast<uint64_t>(HashFunction::_hash(candidateKey, sp)) & mask;
+      for (uint64_t i = 0; i < N; i += defaults::WARPSIZE) {
+         const uint64_t p = (home + i + w_tid) & mask;
+         const KEY_TYPE k = _buckets[p].first;
+         const uint64_t empty = __ballot(k == EMPTYBUCKET);
+         const uint64_t hit = first_before_empty(__ballot(k == candidateKey), empty);
+         if (hit) {
+            if (static_cast<int>(w_tid) == __ffsll(static_cast<unsigned long long>(hit)) - 1) {
+               KEY_TYPE expected = candidateKey;
+               __hip_atomic_compare_exchange_strong(&_buckets[p].first, &expected, TOMBSTONE, __ATOMIC_RELAXED,
+                                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+               if (expected == candidateKey) {
+                  __hip_atomic_fetch_sub(&_dinfo->fill, size_t(1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                  __hip_atomic_fetch_add(&_dinfo->tombstoneCounter, size_t(1), __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+               }
+            }
+            return;
+         }
+         if (empty) return;
+      }
+   }
+
+private:
+   /* Bits of `match` below the lowest bit of `empty` (all of them when no EMPTY). */
+   HASHINATOR_DEVICEONLY static uint64_t first_before_empty(uint64_t match, uint64_t empty) noexcept {
+      return match & (empty ? ((empty & (0 - empty)) - 1) : ~uint64_t(0));
+   }
+   template <bool skipOverWrites>
+   HASHINATOR_DEVICEONLY bool warp_insert(KEY_TYPE key, VAL_TYPE val, size_t w_tid) noexcept {
+      MapInfo* info = _dinfo;
+      const int sp = info->sizePower;
+      const uint64_t N = uint64_t(1) << sp, mask = N - 1;
+      const uint64_t home = static_cast<uint64_t>(HashFunction::_hash(key, sp)) & mask;
+      const int lane = static_cast<int>(w_tid);
+      for (uint64_t i = 0; i < N; i += defaults::WARPSIZE) {
+         const uint64_t p = (home + i + w_tid) & mask;
+         const KEY_TYPE k = _buckets[p].first;
+         const uint64_t exists = __ballot(k == key);
+         uint64_t empty = __ballot(k == EMPTYBUCKET);
+         if (exists) {
+            if (!skipOverWrites && lane == __ffsll(static_cast<unsigned long long>(exists)) - 1)
+               __hip_atomic_exchange(&_buckets[p].second, val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+         }
+         while (empty) {
+            const int winner = __ffsll(static_cast<unsigned long long>(empty)) - 1;
+            int done = 0; // 1: placed here, 2: the same key was placed here meanwhile
+            if (lane == winner) {
+               KEY_TYPE expected = EMPTYBUCKET;
+               __hip_atomic_compare_exchange_strong(&_buckets[p].first, &expected, key, __ATOMIC_RELAXED,
+                                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+               if (expected == EMPTYBUCKET) {
+                  __hip_atomic_exchange(&_buckets[p].second, val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                  __hip_atomic_fetch_add(&info->fill, size_t(1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                  const size_t ovf = static_cast<size_t>((p - home) & mask) + 1;
+                  if (ovf > info->currentMaxBucketOverflow)
+                     __hip_atomic_fetch_max(&info->currentMaxBucketOverflow, nextOverflow(ovf, defaults::WARPSIZE),
+                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                  __hip_atomic_fetch_and(&info->reserved1, ~int32_t(HN_INFO_ORDERED), __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+                  done = 1;
+               } else if (expected == key) {
+                  if (!skipOverWrites)
+                     __hip_atomic_exchange(&_buckets[p].second, val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                  done = 2;
+               }
+            }
+            if (__ballot(done == 1)) return true;
+            if (__ballot(done == 2)) return false;
+            empty &= ~(uint64_t(1) << winner);
+         }
+      }
+      return false; // no EMPTY slot anywhere: not placed
+   }
+
+public:
    /* download() (hashinator.h:1457-1467), called after device-side work on the
     * map: read the Info back; grow one size up when the overflow exceeds
     * BUCKET_OVERFLOW, else remove tombstones if there are any.  The reference
     * grows with its host rehash; the device rehash here gives the same
     * key/value set. */
    void download(hipStream_t s = 0) {
-      pull(s);
+      changed(s); // device-side work may have written the buckets
       if (_info.currentMaxBucketOverflow > static_cast<size_t>(defaults::BUCKET_OVERFLOW))
          device_rehash(_info.sizePower + 1, s);
       else if (_info.tombstoneCounter > 0)

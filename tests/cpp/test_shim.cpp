@@ -238,6 +238,111 @@ bool resize_and_grow() {
    return true;
 }
 
+/* User kernel of the device-side API: one wave per key, w_tid = lane. */
+template <typename Map, typename K>
+__global__ void k_warp_ops(Map* m, const K* keys, const K* vals, K* out, size_t n, int op) {
+   const size_t wid = (static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x) / 64;
+   const size_t w_tid = threadIdx.x % 64;
+   if (wid >= n) return; // whole waves
+   if (op == 0) {
+      m->warpInsert(keys[wid], vals[wid], w_tid);
+   } else if (op == 1) {
+      K v = out[wid];
+      m->warpFind(keys[wid], v, w_tid);
+      if (w_tid == 0) out[wid] = v;
+   } else if (op == 2) {
+      m->warpErase(keys[wid], w_tid);
+   } else {
+      const bool added = m->warpInsert_V(keys[wid], vals[wid], w_tid);
+      if (w_tid == 0) out[wid] = added ? K(1) : K(0);
+   }
+}
+template <typename Map, typename K>
+void warp_ops(Map* dm, split::SplitVector<K>& keys, split::SplitVector<K>& vals, split::SplitVector<K>& out, size_t n,
+              int op) {
+   const unsigned blocks = static_cast<unsigned>((n * 64 + 255) / 256);
+   hipLaunchKernelGGL((k_warp_ops<Map, K>), dim3(blocks), dim3(256), 0, 0, dm, keys.data(), vals.data(), out.data(),
+                      n, op);
+   SPLIT_CHECK_ERR(hipGetLastError());
+   SPLIT_CHECK_ERR(hipDeviceSynchronize());
+}
+
+/* upload() + warpInsert / warpInsert_V / warpFind / warpErase from a user
+ * kernel (hashinator.h:853-1126, 1416-1467), then the host API on the same map. */
+template <typename K>
+bool device_api() {
+   using Map = Hashmap<K, K>;
+   const size_t n = 3000, m = 1000;
+   split::SplitVector<K> keys(n + m), vals(n + m), out(n + m);
+   for (size_t i = 0; i < n + m; ++i) {
+      keys[i] = static_cast<K>(7 * i + 3);
+      vals[i] = static_cast<K>(i ^ 0x55);
+   }
+   Map hmap(14);
+   Map* dm = hmap.upload();
+   CHECK(dm != nullptr && hmap.get_device_pointer() == dm);
+   warp_ops(dm, keys, vals, out, n, 0); // insert the first n keys
+   hmap.download();
+   CHECK(hmap.size() == n && hmap.tombstone_count() == 0);
+   {
+      // linear probing's occupied-slot set does not depend on the insertion
+      // order: it must equal a serial insert's on the host
+      std::vector<hash_pair<K, K>> b(hmap.bucket_count());
+      SPLIT_CHECK_ERR(hipMemcpy(b.data(), hmap.template expose_bucketdata<false>(), b.size() * sizeof(b[0]),
+                                hipMemcpyDeviceToHost));
+      const size_t N = b.size();
+      std::vector<char> occ(N, 0);
+      for (size_t i = 0; i < n; ++i) {
+         size_t p = static_cast<size_t>(Hashinator::HashFunctions::Fibonacci<K>::_hash(keys[i], 14)) & (N - 1);
+         while (occ[p]) p = (p + 1) & (N - 1);
+         occ[p] = 1;
+      }
+      for (size_t j = 0; j < N; ++j) CHECK(occ[j] == (b[j].first != hmap.get_emptybucket()));
+   }
+   dm = hmap.upload(); // download() may have replaced the bucket array (a rehash)
+   // host-side retrieve sees them
+   split::SplitVector<K> got(n + m);
+   for (size_t i = 0; i < n + m; ++i) got[i] = K(9);
+   hmap.retrieve(keys.data(), got.data(), n + m);
+   for (size_t i = 0; i < n + m; ++i) CHECK(got[i] == (i < n ? vals[i] : K(9)));
+   // warpFind: values of present keys, absent keys untouched
+   for (size_t i = 0; i < n + m; ++i) out[i] = K(5);
+   warp_ops(dm, keys, vals, out, n + m, 1);
+   for (size_t i = 0; i < n + m; ++i) CHECK(out[i] == (i < n ? vals[i] : K(5)));
+   // warpInsert_V: false for present keys, true for new ones
+   warp_ops(dm, keys, vals, out, n + m, 3);
+   for (size_t i = 0; i < n + m; ++i) CHECK(out[i] == (i < n ? K(0) : K(1)));
+   hmap.download();
+   CHECK(hmap.size() == n + m);
+   dm = hmap.upload();
+   // warpErase of the first half; download() removes the tombstones
+   warp_ops(dm, keys, vals, out, (n + m) / 2, 2);
+   SPLIT_CHECK_ERR(hipDeviceSynchronize());
+   hmap.download();
+   CHECK(hmap.size() == (n + m) - (n + m) / 2 && hmap.tombstone_count() == 0);
+   hmap.optimizeCPU();
+   for (size_t i = 0; i < n + m; ++i) CHECK((hmap.find(keys[i]) != hmap.end()) == (i >= (n + m) / 2));
+   // a Robin Hood ordered map (bulk build) takes new keys from the device and
+   // the host lookups still find every key
+   Map hb(13);
+   hb.setAutoBuild(1.0f / 16, 0);
+   hb.insert(keys.data(), vals.data(), n, 1.0f);
+   CHECK(hb.ordered());
+   Map* db = hb.upload();
+   split::SplitVector<K> nk(m), nv(m), no(m);
+   for (size_t i = 0; i < m; ++i) {
+      nk[i] = keys[n + i];
+      nv[i] = vals[n + i];
+   }
+   warp_ops(db, nk, nv, no, m, 0);
+   hb.download();
+   CHECK(hb.size() == n + m && !hb.ordered());
+   for (size_t i = 0; i < n + m; ++i) got[i] = K(9);
+   hb.retrieve(keys.data(), got.data(), n + m);
+   for (size_t i = 0; i < n + m; ++i) CHECK(got[i] == vals[i]);
+   return true;
+}
+
 bool compaction_and_scan() {
    const size_t n = (1 << 20) + 333;
    split::SplitVector<uint32_t> in(n), out(n);
@@ -304,6 +409,8 @@ int main() {
        {"erase_clean_extract<u64> sp=14", [] { return erase_clean_extract<uint64_t>(14); }},
        {"resize_and_grow<u32>", [] { return resize_and_grow<uint32_t>(); }},
        {"resize_and_grow<u64>", [] { return resize_and_grow<uint64_t>(); }},
+       {"device_api<u32>", [] { return device_api<uint32_t>(); }},
+       {"device_api<u64>", [] { return device_api<uint64_t>(); }},
        {"compaction_and_scan", compaction_and_scan},
        {"splitvector_host_semantics", splitvector_host_semantics},
    };
