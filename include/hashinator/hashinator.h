@@ -23,11 +23,10 @@
  * Instantiations: Hashmap<uint32_t,uint32_t> and Hashmap<uint64_t,uint64_t>
  * with the default sentinels and hash.
  *
- * Of the reference's device-only API, upload()/get_device_pointer() and the
- * warp members (warpInsert, warpInsert_V, warpFind, warpErase) are provided;
- * not provided: device_insert/device_find/device_erase,
- * set_element/insert_element/read_element, device iterators, and the host
- * insertion path (_at, operator[], erase(key) on the host).
+ * The reference's device-only API is provided (upload, the warp members,
+ * device_find/device_insert/device_erase/set_element/read_element and device
+ * iterators); not provided: the host insertion path (_at, operator[],
+ * erase(key) on the host).
  */
 #pragma once
 #include <cmath>
@@ -435,7 +434,152 @@ public:
       }
    }
 
+   /* Single-thread device API (hashinator.h:1470-1769): iterators over the
+    * live slots, device_find / device_count / device_erase, set_element /
+    * device_insert (one thread per key, CAS from the home slot on; tombstones
+    * are not reused), read_element.  A key absent from the map is undefined
+    * for read_element, as in the reference (which asserts). */
+   class device_iterator {
+      size_t index;
+      Hashmap* map;
+
+   public:
+      HASHINATOR_DEVICEONLY device_iterator(Hashmap& m, size_t i) : index(i), map(&m) {}
+      HASHINATOR_DEVICEONLY size_t getIndex() const { return index; }
+      HASHINATOR_DEVICEONLY device_iterator& operator++() {
+         index = map->next_live(index + 1);
+         return *this;
+      }
+      HASHINATOR_DEVICEONLY device_iterator operator++(int) {
+         device_iterator t = *this;
+         ++(*this);
+         return t;
+      }
+      HASHINATOR_DEVICEONLY bool operator==(const device_iterator& o) const { return map == o.map && index == o.index; }
+      HASHINATOR_DEVICEONLY bool operator!=(const device_iterator& o) const { return !(*this == o); }
+      HASHINATOR_DEVICEONLY pair_t& operator*() const { return map->_buckets[index]; }
+      HASHINATOR_DEVICEONLY pair_t* operator->() const { return &map->_buckets[index]; }
+   };
+   class const_device_iterator {
+      size_t index;
+      const Hashmap* map;
+
+   public:
+      HASHINATOR_DEVICEONLY explicit const_device_iterator(const Hashmap& m, size_t i) : index(i), map(&m) {}
+      HASHINATOR_DEVICEONLY size_t getIndex() const { return index; }
+      HASHINATOR_DEVICEONLY const_device_iterator& operator++() {
+         index = map->next_live(index + 1);
+         return *this;
+      }
+      HASHINATOR_DEVICEONLY const_device_iterator operator++(int) {
+         const_device_iterator t = *this;
+         ++(*this);
+         return t;
+      }
+      HASHINATOR_DEVICEONLY bool operator==(const const_device_iterator& o) const {
+         return map == o.map && index == o.index;
+      }
+      HASHINATOR_DEVICEONLY bool operator!=(const const_device_iterator& o) const { return !(*this == o); }
+      HASHINATOR_DEVICEONLY const pair_t& operator*() const { return map->_buckets[index]; }
+      HASHINATOR_DEVICEONLY const pair_t* operator->() const { return &map->_buckets[index]; }
+   };
+
+   HASHINATOR_DEVICEONLY size_t device_size() const { return _nbuckets; }
+   HASHINATOR_DEVICEONLY device_iterator device_end() { return device_iterator(*this, _nbuckets); }
+   HASHINATOR_DEVICEONLY const_device_iterator device_end() const { return const_device_iterator(*this, _nbuckets); }
+   HASHINATOR_DEVICEONLY device_iterator device_begin() { return device_iterator(*this, next_live(0)); }
+   HASHINATOR_DEVICEONLY const_device_iterator device_begin() const {
+      return const_device_iterator(*this, next_live(0));
+   }
+   HASHINATOR_DEVICEONLY device_iterator device_find(KEY_TYPE key) { return device_iterator(*this, find_slot(key)); }
+   HASHINATOR_DEVICEONLY const_device_iterator device_find(KEY_TYPE key) const {
+      return const_device_iterator(*this, find_slot(key));
+   }
+   HASHINATOR_DEVICEONLY size_t device_count(const KEY_TYPE& key) const { return find_slot(key) != _nbuckets; }
+   /* The slot becomes TOMBSTONE (CAS: erased once); fill -1, tombstones +1. */
+   HASHINATOR_DEVICEONLY device_iterator device_erase(device_iterator pos) {
+      const size_t i = pos.getIndex();
+      if (i < _nbuckets) {
+         KEY_TYPE item = _buckets[i].first;
+         if (item != EMPTYBUCKET && item != TOMBSTONE) {
+            KEY_TYPE expected = item;
+            __hip_atomic_compare_exchange_strong(&_buckets[i].first, &expected, TOMBSTONE, __ATOMIC_RELAXED,
+                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (expected == item) {
+               __hip_atomic_fetch_sub(&_dinfo->fill, size_t(1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+               __hip_atomic_fetch_add(&_dinfo->tombstoneCounter, size_t(1), __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
+            }
+         }
+      }
+      return ++pos;
+   }
+   HASHINATOR_DEVICEONLY size_t device_erase(const KEY_TYPE& key) {
+      const size_t i = find_slot(key);
+      if (i == _nbuckets) return 0;
+      device_erase(device_iterator(*this, i));
+      return 1;
+   }
+   /* Insert or (unless skipOverWrites) overwrite; true for a new key.  The
+    * overflow takes the probe length (rounded up to 64). */
+   template <bool skipOverWrites = false>
+   HASHINATOR_DEVICEONLY bool set_element(const KEY_TYPE& key, VAL_TYPE val) {
+      const uint64_t mask = _nbuckets - 1;
+      const uint64_t home = static_cast<uint64_t>(HashFunction::_hash(key, _dinfo->sizePower)) & mask;
+      for (uint64_t i = 0; i < _nbuckets; ++i) {
+         const uint64_t p = (home + i) & mask;
+         KEY_TYPE old = EMPTYBUCKET;
+         __hip_atomic_compare_exchange_strong(&_buckets[p].first, &old, key, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+         if (old == EMPTYBUCKET || old == key) {
+            const bool added = old == EMPTYBUCKET;
+            if (added || !skipOverWrites)
+               __hip_atomic_exchange(&_buckets[p].second, val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (added) {
+               __hip_atomic_fetch_add(&_dinfo->fill, size_t(1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+               __hip_atomic_fetch_and(&_dinfo->reserved1, ~int32_t(HN_INFO_ORDERED), __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
+            }
+            __hip_atomic_fetch_max(&_dinfo->currentMaxBucketOverflow,
+                                   nextOverflow(static_cast<size_t>(i + 1), defaults::WARPSIZE), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            return added;
+         }
+      }
+      return false; // no EMPTY slot anywhere: not placed
+   }
+   template <bool skipOverWrites = false>
+   HASHINATOR_DEVICEONLY hash_pair<device_iterator, bool> device_insert(pair_t newEntry) {
+      const bool added = set_element<skipOverWrites>(newEntry.first, newEntry.second);
+      return hash_pair<device_iterator, bool>(device_find(newEntry.first), added);
+   }
+   HASHINATOR_DEVICEONLY const VAL_TYPE& read_element(const KEY_TYPE& key) const {
+      const size_t i = find_slot(key);
+      return _buckets[i < _nbuckets ? i : 0].second;
+   }
+
 private:
+   /* First live slot at or after i (the bucket count when none). */
+   HASHINATOR_DEVICEONLY size_t next_live(size_t i) const {
+      for (; i < _nbuckets; ++i) {
+         const KEY_TYPE k = _buckets[i].first;
+         if (k != EMPTYBUCKET && k != TOMBSTONE) return i;
+      }
+      return _nbuckets;
+   }
+   /* Slot of `key` (linear probe from its home, past tombstones, ending at
+    * EMPTY), or the bucket count. */
+   HASHINATOR_DEVICEONLY size_t find_slot(const KEY_TYPE& key) const {
+      const uint64_t mask = _nbuckets - 1;
+      const uint64_t home = static_cast<uint64_t>(HashFunction::_hash(key, _dinfo->sizePower)) & mask;
+      for (uint64_t i = 0; i < _nbuckets; ++i) {
+         const uint64_t p = (home + i) & mask;
+         const KEY_TYPE k = _buckets[p].first;
+         if (k == key) return static_cast<size_t>(p);
+         if (k == EMPTYBUCKET) break;
+      }
+      return _nbuckets;
+   }
    /* Bits of `match` below the lowest bit of `empty` (all of them when no EMPTY). */
    HASHINATOR_DEVICEONLY static uint64_t first_before_empty(uint64_t match, uint64_t empty) noexcept {
       return match & (empty ? ((empty & (0 - empty)) - 1) : ~uint64_t(0));

@@ -343,6 +343,101 @@ bool device_api() {
    return true;
 }
 
+/* User kernels of the single-thread device API: one thread per key. */
+template <typename Map, typename K>
+__global__ void k_thread_ops(Map* m, const K* keys, const K* vals, K* out, size_t n, int op) {
+   const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+   if (i >= n) return;
+   if (op == 0) {
+      out[i] = m->set_element(keys[i], vals[i]) ? K(1) : K(0);
+   } else if (op == 1) {
+      auto it = m->device_find(keys[i]);
+      out[i] = it != m->device_end() ? it->second : K(5);
+   } else if (op == 2) {
+      out[i] = static_cast<K>(m->device_erase(keys[i]));
+   } else if (op == 3) {
+      out[i] = static_cast<K>(m->device_count(keys[i]));
+   } else if (op == 4) {
+      auto r = m->device_insert(hash_pair<K, K>(keys[i], vals[i]));
+      out[i] = (r.second ? K(1) : K(0)) + (r.first != m->device_end() && r.first->second == vals[i] ? K(2) : K(0));
+   } else {
+      out[i] = m->read_element(keys[i]);
+   }
+}
+template <typename Map, typename K>
+__global__ void k_iterate(Map* m, K* out) {
+   if (threadIdx.x != 0 || blockIdx.x != 0) return;
+   size_t c = 0;
+   K sum = 0;
+   for (auto it = m->device_begin(); it != m->device_end(); ++it) {
+      ++c;
+      sum += it->first;
+   }
+   out[0] = static_cast<K>(c);
+   out[1] = sum;
+}
+template <typename Map, typename K>
+void thread_ops(Map* dm, split::SplitVector<K>& keys, split::SplitVector<K>& vals, split::SplitVector<K>& out,
+                size_t n, int op) {
+   hipLaunchKernelGGL((k_thread_ops<Map, K>), dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, 0, dm,
+                      keys.data(), vals.data(), out.data(), n, op);
+   SPLIT_CHECK_ERR(hipGetLastError());
+   SPLIT_CHECK_ERR(hipDeviceSynchronize());
+}
+
+/* set_element / device_insert / device_find / device_count / device_erase /
+ * read_element and device iterators (hashinator.h:1470-1769). */
+template <typename K>
+bool thread_api() {
+   using Map = Hashmap<K, K>;
+   const size_t n = 3000, m = 1000;
+   split::SplitVector<K> keys(n + m), vals(n + m), out(n + m), it(2);
+   K ksum = 0;
+   for (size_t i = 0; i < n + m; ++i) {
+      keys[i] = static_cast<K>(11 * i + 5);
+      vals[i] = static_cast<K>(i * 3 + 1);
+      if (i < n) ksum += keys[i];
+   }
+   Map hmap(14);
+   Map* dm = hmap.upload();
+   thread_ops(dm, keys, vals, out, n, 0);
+   for (size_t i = 0; i < n; ++i) CHECK(out[i] == K(1));
+   thread_ops(dm, keys, vals, out, n, 0); // again: no new keys
+   for (size_t i = 0; i < n; ++i) CHECK(out[i] == K(0));
+   hmap.download();
+   CHECK(hmap.size() == n);
+   dm = hmap.upload();
+   thread_ops(dm, keys, vals, out, n + m, 1);
+   for (size_t i = 0; i < n + m; ++i) CHECK(out[i] == (i < n ? vals[i] : K(5)));
+   thread_ops(dm, keys, vals, out, n + m, 3);
+   for (size_t i = 0; i < n + m; ++i) CHECK(out[i] == (i < n ? K(1) : K(0)));
+   thread_ops(dm, keys, vals, out, n, 5);
+   for (size_t i = 0; i < n; ++i) CHECK(out[i] == vals[i]);
+   hipLaunchKernelGGL((k_iterate<Map, K>), dim3(1), dim3(64), 0, 0, dm, it.data());
+   SPLIT_CHECK_ERR(hipDeviceSynchronize());
+   CHECK(it[0] == static_cast<K>(n) && it[1] == ksum);
+   // host retrieve agrees with the device-side inserts
+   split::SplitVector<K> got(n + m);
+   for (size_t i = 0; i < n + m; ++i) got[i] = K(9);
+   hmap.retrieve(keys.data(), got.data(), n + m);
+   for (size_t i = 0; i < n + m; ++i) CHECK(got[i] == (i < n ? vals[i] : K(9)));
+   // device_erase of the first half (1 each), then device_insert of the rest
+   thread_ops(dm, keys, vals, out, n / 2, 2);
+   for (size_t i = 0; i < n / 2; ++i) CHECK(out[i] == K(1));
+   split::SplitVector<K> nk(m), nv(m), no(m);
+   for (size_t i = 0; i < m; ++i) {
+      nk[i] = keys[n + i];
+      nv[i] = vals[n + i];
+   }
+   thread_ops(dm, nk, nv, no, m, 4);
+   for (size_t i = 0; i < m; ++i) CHECK(no[i] == K(3));
+   hmap.download();
+   CHECK(hmap.size() == n - n / 2 + m && hmap.tombstone_count() == 0);
+   hmap.optimizeCPU();
+   for (size_t i = 0; i < n + m; ++i) CHECK((hmap.find(keys[i]) != hmap.end()) == (i >= n / 2));
+   return true;
+}
+
 bool compaction_and_scan() {
    const size_t n = (1 << 20) + 333;
    split::SplitVector<uint32_t> in(n), out(n);
@@ -411,6 +506,8 @@ int main() {
        {"resize_and_grow<u64>", [] { return resize_and_grow<uint64_t>(); }},
        {"device_api<u32>", [] { return device_api<uint32_t>(); }},
        {"device_api<u64>", [] { return device_api<uint64_t>(); }},
+       {"thread_api<u32>", [] { return thread_api<uint32_t>(); }},
+       {"thread_api<u64>", [] { return thread_api<uint64_t>(); }},
        {"compaction_and_scan", compaction_and_scan},
        {"splitvector_host_semantics", splitvector_host_semantics},
    };
