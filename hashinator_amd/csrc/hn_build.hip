@@ -728,7 +728,7 @@ __device__ void probe_to_header(RegionLds<K>& L, uint32_t probe, BuildHeader* h)
    }
 }
 
-// One workgroup walks kRegionChunk consecutive regions in order: sort a
+// One workgroup walks `chunk` consecutive regions in order: sort a
 // region, record its carry function, place it with the carry handed on from
 // the region before, and hand on max(a, c + b).  The walk starts from the
 // carry out of the region before the chunk taken with no carry in (that
@@ -736,19 +736,28 @@ __device__ void probe_to_header(RegionLds<K>& L, uint32_t probe, BuildHeader* h)
 // >= N are skipped.  Nothing waits on another workgroup; the carry kernels
 // finds the regions whose carry was wrong (a spill crossing a whole region
 // into a chunk, or the wrap past the table's end) and k_region_fix places
-// them again.
+// them again.  The carries do not depend on the chunk: a shorter one only
+// means more walks and more regions sorted twice.
 constexpr uint32_t kRegionChunk = 16;
+
+// Regions per walk: kRegionChunk, or fewer when that would leave the device
+// short of workgroups (tables below 2^24 slots): a 2^20-slot table has 512
+// regions, 32 walks of 16 but 512 of 1.
+inline uint32_t region_chunk(uint64_t NR) {
+   const uint64_t c = NR / (static_cast<uint64_t>(cu_count()) * 2);
+   return static_cast<uint32_t>(c < 1 ? 1 : (c > kRegionChunk ? kRegionChunk : c));
+}
 
 template <typename K>
 __global__ __launch_bounds__(kRegionBS) void k_region_build(const Slot<K>* S, const uint64_t* start, RegionLB lb,
                                                             Slot<K>* b, int sp, int rlog, uint64_t NR,
-                                                            BuildHeader* h) {
+                                                            uint32_t chunk, BuildHeader* h) {
    __shared__ RegionLds<K> L;
    if (!h->go) return;
    const uint64_t R = 1ull << rlog;
    const int64_t Ri = static_cast<int64_t>(R);
-   const uint64_t r0 = static_cast<uint64_t>(blockIdx.x) * kRegionChunk;
-   const uint64_t r1 = std::min<uint64_t>(r0 + kRegionChunk, NR);
+   const uint64_t r0 = static_cast<uint64_t>(blockIdx.x) * chunk;
+   const uint64_t r1 = std::min<uint64_t>(r0 + chunk, NR);
    K kk[kRegionIPT], vv[kRegionIPT];
    uint32_t uu[kRegionIPT];
    int32_t ff[kRegionIPT];
@@ -1067,8 +1076,9 @@ int build_core(const BuildPlan& P, int mode, int src0, PartIn<K> in0, uint64_t n
    }
    hipLaunchKernelGGL(k_region_starts<K>, dim3(stream_blocks(P.NR + 1, 256)), dim3(256), 0, s, S, start, lb, P.NR,
                       P.sp, P.rlog, h);
-   hipLaunchKernelGGL(k_region_build<K>, dim3(static_cast<uint32_t>((P.NR + kRegionChunk - 1) / kRegionChunk)),
-                      dim3(kRegionBS), 0, s, S, start, lb, T, P.sp, P.rlog, P.NR, h);
+   const uint32_t chunk = region_chunk(P.NR);
+   hipLaunchKernelGGL(k_region_build<K>, dim3(static_cast<uint32_t>((P.NR + chunk - 1) / chunk)), dim3(kRegionBS), 0,
+                      s, S, start, lb, T, P.sp, P.rlog, P.NR, chunk, h);
    RegionSum* blk = reinterpret_cast<RegionSum*>(base + P.off_blk);
    const uint64_t nblk = (P.NR + kCarryTile - 1) / kCarryTile;
    const int64_t Ri = static_cast<int64_t>(P.R), Ni = static_cast<int64_t>(P.N);
