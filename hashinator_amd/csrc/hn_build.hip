@@ -55,7 +55,7 @@ constexpr int kRegionLog = 11;   // 2048 slots per region
 constexpr int kPartBS = 256;     // partition workgroup
 constexpr int kPartIPT = 16;     // pairs per thread and tile (4096-pair tiles: ~8 pairs per 9-bit digit run)
 constexpr int kPartTile = kPartBS * kPartIPT;
-constexpr int kPartSub = 8;      // tiles per partition block
+constexpr int kPartSub = 8;      // tiles per partition block at most (fewer for small inputs)
 constexpr int kScatBS = 256;     // threads of a scatter block (512 measured the same)
 constexpr int kPartSuper = kPartTile * kPartSub;
 constexpr int kMaxDigitBits = 9; // 512 digits per pass at most
@@ -93,7 +93,8 @@ struct BuildPlan {
    int rbits;           // log2 NR
    int passes;          // partition passes
    int dbits[3];        // digit bits per pass (low digit first)
-   uint32_t nblocks;    // partition tiles
+   uint32_t nblocks;    // partition blocks
+   uint32_t sub;        // tiles per partition block
    size_t off_pairs, off_mat, off_scan, off_start, off_word, off_spec, off_carry, off_mark, off_list, off_blk, total;
    size_t off_pairs2;   // a second pair array for the passes that cannot use the table's memory (0: none)
 };
@@ -127,7 +128,15 @@ bool build_plan(int sp, uint64_t n, int dcap, bool rebuild, BuildPlan* P, bool t
       P->dbits[i] = left / (P->passes - i) + ((left % (P->passes - i)) ? 1 : 0);
       left -= P->dbits[i];
    }
-   const uint64_t nb = (n + kPartSuper - 1) / kPartSuper;
+   // Tiles per block: kPartSub, or as few as keep the blocks within one
+   // resident round (2 per CU) for inputs below ~2^22 pairs: more blocks
+   // than that would leave a partial second round.
+   const uint64_t tiles = (n + kPartTile - 1) / kPartTile;
+   const uint64_t slots = static_cast<uint64_t>(cu_count()) * 2;
+   const uint64_t per = (tiles + slots - 1) / slots;
+   P->sub = static_cast<uint32_t>(per < 1 ? 1 : (per > kPartSub ? kPartSub : per));
+   const uint64_t super = static_cast<uint64_t>(kPartTile) * P->sub;
+   const uint64_t nb = (n + super - 1) / super;
    if (nb > 0xFFFFFFFFull) return false;
    P->nblocks = static_cast<uint32_t>(nb);
    // Sized for the default digit width, so any narrower one fits the same scratch.
@@ -229,12 +238,12 @@ __device__ __forceinline__ bool live_key(K k) {
 }
 
 // Per-block digit counts, written digit-major: mat[d * nblocks + block].  A
-// block covers kPartSub consecutive tiles (kPartSuper pairs), which keeps the
+// block covers `sub` consecutive tiles (up to kPartSuper pairs), which keeps the
 // count matrix -- and its scan -- small.
 template <typename K, int SRC, bool FILTER>
 __global__ __launch_bounds__(kPartBS) void k_part_count(PartIn<K> in, uint64_t n_in, const uint64_t* d_n, uint32_t* mat,
-                                                        uint32_t nblocks, int sp, int rlog, int dshift, int dbits,
-                                                        BuildHeader* h) {
+                                                        uint32_t nblocks, uint32_t nsub, int sp, int rlog, int dshift,
+                                                        int dbits, BuildHeader* h) {
    __shared__ uint32_t hist[1 << kMaxDigitBits];
    __shared__ uint32_t live_s;
    if (!h->go) return;
@@ -243,8 +252,9 @@ __global__ __launch_bounds__(kPartBS) void k_part_count(PartIn<K> in, uint64_t n
    for (uint32_t d = threadIdx.x; d < D; d += kPartBS) hist[d] = 0;
    if (threadIdx.x == 0) live_s = 0;
    __syncthreads();
-   const uint64_t t0 = static_cast<uint64_t>(blockIdx.x) * kPartSuper;
-   const uint64_t t1 = std::min<uint64_t>(t0 + kPartSuper, n);
+   const uint64_t super = static_cast<uint64_t>(kPartTile) * nsub;
+   const uint64_t t0 = static_cast<uint64_t>(blockIdx.x) * super;
+   const uint64_t t1 = std::min<uint64_t>(t0 + super, n);
    uint32_t live = 0;
    for (uint64_t e = t0 + threadIdx.x; e < t1; e += kPartBS) {
       const K key = part_key<K, SRC>(in, e);
@@ -261,7 +271,7 @@ __global__ __launch_bounds__(kPartBS) void k_part_count(PartIn<K> in, uint64_t n
    if (FILTER && threadIdx.x == 0 && live_s) atomic_add_relaxed(&h->n, static_cast<uint64_t>(live_s));
 }
 
-// Stable multisplit.  A block walks its kPartSub tiles in order, keeping the
+// Stable multisplit.  A block walks its `nsub` tiles in order, keeping the
 // running output offset of every digit in LDS.  In a tile, wave w ranks the
 // pairs [1024w, 1024w + 1024) in sixteen rounds of 64: lanes with equal digits are
 // found with one ballot per digit bit, and the wave's own LDS counters carry
@@ -270,8 +280,9 @@ __global__ __launch_bounds__(kPartBS) void k_part_count(PartIn<K> in, uint64_t n
 // LDS in digit order and leaves in runs.
 template <typename K, int SRC, bool FILTER>
 __global__ __launch_bounds__(kScatBS) void k_part_scatter(PartIn<K> in, uint64_t n_in, const uint64_t* d_n,
-                                                          Slot<K>* out, const uint32_t* offs, uint32_t nblocks, int sp,
-                                                          int rlog, int dshift, int dbits, const BuildHeader* h) {
+                                                          Slot<K>* out, const uint32_t* offs, uint32_t nblocks,
+                                                          uint32_t nsub, int sp, int rlog, int dshift, int dbits,
+                                                          const BuildHeader* h) {
    constexpr int NW = kScatBS / kWave;
    constexpr int ROUNDS = kPartTile / kScatBS;
    constexpr uint32_t DMAX = 1u << kMaxDigitBits;
@@ -289,7 +300,7 @@ __global__ __launch_bounds__(kScatBS) void k_part_scatter(PartIn<K> in, uint64_t
    const uint64_t lt = (1ull << lane) - 1ull;
    const uint32_t d0 = threadIdx.x * DPT; // this thread's digits in the digit loops
    for (uint32_t d = d0; d < d0 + DPT && d < D; ++d) goff_s[d] = offs[static_cast<uint64_t>(d) * nblocks + blockIdx.x];
-   const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kPartSuper;
+   const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kPartTile * nsub;
    // The pairs of tile `sub` are loaded while tile sub-1 is written out.
    Slot<K> el[ROUNDS];
    auto load_tile = [&](int sub) {
@@ -302,7 +313,7 @@ __global__ __launch_bounds__(kScatBS) void k_part_scatter(PartIn<K> in, uint64_t
       }
    };
    load_tile(0);
-   for (int sub = 0; sub < kPartSub; ++sub) {
+   for (int sub = 0; sub < static_cast<int>(nsub); ++sub) {
       const uint64_t t0 = base + static_cast<uint64_t>(sub) * kPartTile;
       if (t0 >= n) break;
       for (uint32_t d = d0; d < d0 + DPT && d < D; ++d) {
@@ -381,7 +392,7 @@ __global__ __launch_bounds__(kScatBS) void k_part_scatter(PartIn<K> in, uint64_t
             tile_s[p] = el[r];
          }
       }
-      if (sub + 1 < kPartSub) load_tile(sub + 1);
+      if (sub + 1 < static_cast<int>(nsub)) load_tile(sub + 1);
       __syncthreads();
       uint32_t cnt = 0; // pairs of this tile that are written: the digit scan's total
       for (int w = 0; w < NW; ++w) cnt += wsum_s[w];
@@ -1007,11 +1018,11 @@ int launch_part(const BuildPlan& P, int pass, PartIn<K> in, uint64_t n_in, const
    const int dbits = P.dbits[pass];
    const uint64_t D = 1ull << dbits;
    hipLaunchKernelGGL((k_part_count<K, SRC, FILTER>), dim3(P.nblocks), dim3(kPartBS), 0, s, in, n_in, d_n, mat,
-                      P.nblocks, P.sp, P.rlog, dshift, dbits, h);
+                      P.nblocks, P.sub, P.sp, P.rlog, dshift, dbits, h);
    int rc = hn_exclusive_scan_u32(mat, mat, D * P.nblocks, scan_scratch, scan_bytes, reinterpret_cast<hn_stream_t>(s));
    if (rc) return rc;
    hipLaunchKernelGGL((k_part_scatter<K, SRC, FILTER>), dim3(P.nblocks), dim3(kScatBS), 0, s, in, n_in, d_n, out, mat,
-                      P.nblocks, P.sp, P.rlog, dshift, dbits, h);
+                      P.nblocks, P.sub, P.sp, P.rlog, dshift, dbits, h);
    return static_cast<int>(hipGetLastError());
 }
 

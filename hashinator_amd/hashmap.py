@@ -70,7 +70,7 @@ class Hashmap:
     """Hashinator::Hashmap<K,V> with K = V = uint32_t or uint64_t, accelerated API."""
 
     def __init__(self, sizePower: int = 5, key_bits: int = 32, device="cuda", tune: hn_tune | None = None,
-                 auto_reorder: float = 1.0 / 16, auto_build: float = 1.0 / 16, build_min_keys: int = 1 << 19):
+                 auto_reorder: float = 1.0 / 16, auto_build: float = 1.0 / 16, build_min_keys: int = 1 << 17):
         """``auto_reorder``: after an insert of at least this fraction of the
         bucket count, and after clean_tombstones and device_rehash, run the
         Robin Hood reorder pass (``reorder()``), which makes later misses
@@ -78,10 +78,13 @@ class Hashmap:
         this fraction of the bucket count into an empty table takes the bulk
         path (hn_build_*: partition + per-region sort, no probing), which
         writes the reordered image directly; 0 turns it off.
-        ``build_min_keys``: and holds at least this many keys -- below about
-        2^19 keys the build's dozen small launches cost more than the probing
-        insert and reorder (one MI355X: 0.23 ms against 0.09 ms at 131,072
-        keys, break-even near 524,288, 0.53 ms against 1.5 ms at 15 M)."""
+        ``build_min_keys``: and holds at least this many keys -- the build's
+        dozen small launches take about 0.09 ms however few the keys, so
+        below about 2^17 keys at low load factors the probing insert and
+        reorder are faster (one MI355X, u32: 0.088 against 0.090 ms at
+        131,072 keys and LF 0.5, 0.10 against 0.19 ms at 235,929 keys and LF
+        0.9, 0.10 against 0.26 ms at 524,288 keys and LF 0.5).  Growing a
+        map of at least this many keys rehashes it by the build too."""
         if key_bits not in (32, 64):
             raise ValueError("key_bits must be 32 or 64")
         self.kb = key_bits

@@ -90,7 +90,7 @@ class _Map:
 
 class ServingEngine:
     def __init__(self, sps, key_bits=64, n_streams=8, hbm_budget_bytes=32 << 30, grow_lf=0.85, device="cuda",
-                 tune=None, build_rehash_min=1 << 19):
+                 tune=None, build_rehash_min=None):
         self.kb = key_bits
         self.sfx = "u32" if key_bits == 32 else "u64"
         self.slot = 8 if key_bits == 32 else 16
@@ -98,7 +98,9 @@ class ServingEngine:
         self.L = _lib.load()
         self.dev = torch.device(device)
         self.grow_lf = grow_lf
-        self.build_rehash_min = build_rehash_min  # grow maps of at least this many keys by the bulk rehash
+        # grow maps of at least this many keys by the bulk rehash (None: never;
+        # its scratch, ~16 B per old slot, is an allocation on the request path)
+        self.build_rehash_min = build_rehash_min
         self.budget = hbm_budget_bytes
         self.tune = C.byref(tune) if tune is not None else None
         self.maps = []
@@ -188,7 +190,7 @@ class ServingEngine:
     def _rehash_scratch(self, m: _Map, new_sp):
         """(bytes, bulk) of a growth's scratch: a map of many keys grows by the
         bulk rehash (one build from its live slots), a small one by reinsertion."""
-        if m.fill_upper >= self.build_rehash_min:
+        if self.build_rehash_min is not None and m.fill_upper >= self.build_rehash_min:
             sb = getattr(self.L, f"hn_rehash_build_scratch_bytes_{self.sfx}")(m.sp, new_sp)
             if sb:
                 return sb, True

@@ -71,7 +71,7 @@ class Hashmap {
    bool _has_tune = false;
    float _auto_reorder = 1.0f / 16; // reorder after inserts of at least this fraction of the buckets
    float _auto_build = 1.0f / 16;   // bulk-build inserts of at least this fraction into an empty table
-   size_t _build_min_keys = size_t(1) << 19; // ... and of at least this many keys (below, launches dominate)
+   size_t _build_min_keys = size_t(1) << 17; // ... and of at least this many keys (below, launches dominate)
 
    uint32_t hash(KEY_TYPE in) const { return static_cast<uint32_t>(HashFunction::_hash(in, _info.sizePower)); }
    const hn_tune* tune() const { return _has_tune ? &_tune : nullptr; }
@@ -813,9 +813,11 @@ public:
    /* Fraction of the bucket count an insert into an empty table must reach
     * to take the bulk path (hn_build_*: partition + per-region sort, no
     * probing; writes the reordered image directly), and the fewest keys it
-    * must hold (below about 2^19 keys the build's launches cost more than the
-    * probing insert and reorder); fraction 0 turns it off. */
-   void setAutoBuild(float fraction, size_t min_keys = size_t(1) << 19) noexcept {
+    * must hold (below about 2^17 keys at low load factors the build's
+    * launches cost more than the probing insert and reorder); fraction 0
+    * turns it off.  A device_rehash that grows a map of at least min_keys
+    * keys also takes the build. */
+   void setAutoBuild(float fraction, size_t min_keys = size_t(1) << 17) noexcept {
       _auto_build = fraction;
       _build_min_keys = min_keys;
    }

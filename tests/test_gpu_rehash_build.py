@@ -109,7 +109,7 @@ def test_map_device_rehash_uses_build(gpu, bits):
     _, v, h = hm.retrieve(live[:, 0].copy())
     assert h.cpu().numpy().all() and np.array_equal(to_numpy(v, bits), live[:, 1])
     # a map below the key threshold grows by the probing reinsert (then reordered)
-    hm2 = Hashmap(sp, bits)  # build_min_keys = 2^19
+    hm2 = Hashmap(sp, bits)  # build_min_keys = 2^17
     hm2.insert(keys, vals, targetLF=1.0)
     hm2.erase(keys[::5])
     hm2.device_rehash(sp + 2)
