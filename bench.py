@@ -135,6 +135,9 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the many-maps (batched vs per-map) comparison")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop measurement")
     ap.add_argument("--no-compaction", action="store_true", help="skip the 2^28-element compaction/scan measurement")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="only the many-maps batched call, 3 runs with probe width 8 and 1024 threads per map "
+                         "(for a kernel profile of that call)")
     ap.add_argument("--calib-sweep", action="store_true",
                     help="only measure the random-gather ceiling (16..256 B reads, 1/4/16 dependent reads) and exit")
     ap.add_argument("--calib-copy", action="store_true",
@@ -161,6 +164,9 @@ def main():
 
     if args.calib_sweep:
         print(json.dumps(calib_sweep(torch, L, dev, S, P)), flush=True)
+        return
+    if args.batched_only:
+        print(json.dumps(batched_only(torch)), flush=True)
         return
 
     tune = C.byref(hn_tune(args.probe_width, 0, 0, 0)) if args.probe_width else None
@@ -562,6 +568,31 @@ def run_small_map(torch, L, dev, S, P, with_cpu=True, reps=20):
                       "retrieve_ms": round((t2 - t1) * 1e3, 2),
                       "insert_retrieve_Mops": round(3 * n / (t2 - t0) / 1e6, 2)}
     return res
+
+
+def batched_only(torch):
+    """The batched call alone on the many-maps workload (see run_batched):
+    fresh tables, then insert -> retrieve -> erase -> clean in one call, 3
+    times; HIP-event time of each call."""
+    from hashinator_amd import _lib, hn_tune
+    from hashinator_amd import workloads as Wl
+    from hashinator_amd.batch import BatchedMaps
+
+    bm = BatchedMaps(*Wl.batched_workload(), key_bits=32)
+    ops = _lib.HN_BATCH_INSERT | _lib.HN_BATCH_RETRIEVE | _lib.HN_BATCH_ERASE | _lib.HN_BATCH_CLEAN
+    tune = hn_tune(8, 1024, 0, 0)
+    stream = torch.cuda.current_stream()
+    times = []
+    for _ in range(3):
+        bm.init()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        bm.run(ops=ops, tune=tune)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        times.append(round(e0.elapsed_time(e1), 3))
+    return {"workload": "4096 Hashmap<u32,u32>, sizes 2^8..2^16, LF 0.6: insert + retrieve + erase 20% + clean",
+            "tune": {"probe_width": 8, "block_size": 1024}, "batched_ms": times}
 
 
 def run_batched(torch):
