@@ -132,7 +132,7 @@ SIGNATURES = {
     "hn_copy_keys_if_flags_u64": (I, [P, P, S, P, P, P, S, P]),
     "hn_exclusive_scan_u32": (I, [P, P, S, P, S, P]),
     "hn_exclusive_scan_u64": (I, [P, P, S, P, S, P]),
-    "hn_batch_scratch_bytes": (S, []),
+    "hn_batch_scratch_bytes": (S, [C.c_uint32]),
     "hn_batch_init_u32": (I, [P, C.c_uint32, P, P]),
     "hn_batch_init_u64": (I, [P, C.c_uint32, P, P]),
     "hn_batch_run_u32": (I, [P, C.c_uint32, C.c_uint32, P, P, P, P, P, P, P, P, P, TUNE]),

@@ -76,7 +76,7 @@ class BatchedMaps:
             np.zeros(0, np.uint64)
         self.clean_scratch = torch.empty((max(1, int(self.ins_len.sum())), 2), dtype=_TDT[key_bits],
                                          device=self.dev)
-        self.scratch = torch.zeros(int(self.L.hn_batch_scratch_bytes()), dtype=torch.uint8, device=self.dev)
+        self.scratch = torch.zeros(int(self.L.hn_batch_scratch_bytes(self.n)), dtype=torch.uint8, device=self.dev)
         self.order = np.argsort(-self.sps, kind="stable") if largest_first else np.arange(self.n)
         slot_bytes = 8 if key_bits == 32 else 16
         descs = (hn_batch_map * self.n)()

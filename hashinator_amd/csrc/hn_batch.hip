@@ -36,7 +36,7 @@ struct BatchArgs {
    uint8_t* qry_hit;
    const K* ers_keys;
    Slot<K>* clean_scratch;
-   uint32_t* ticket;
+   uint64_t* ticket;
 };
 
 __device__ __forceinline__ void phase_barrier() {
@@ -47,7 +47,7 @@ __device__ __forceinline__ void phase_barrier() {
 
 template <typename K, int G>
 __global__ __launch_bounds__(1024) void k_batch(BatchArgs<K> a) {
-   __shared__ uint32_t s_map;
+   __shared__ uint64_t s_map;
    __shared__ unsigned long long s_placed, s_max, s_erased, s_lifted;
    __shared__ int s_failed;
    const uint64_t groups = blockDim.x / G;
@@ -74,11 +74,11 @@ __global__ __launch_bounds__(1024) void k_batch(BatchArgs<K> a) {
 
    for (;;) {
       if (threadIdx.x == 0) {
-         s_map = __hip_atomic_fetch_add(a.ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+         s_map = __hip_atomic_fetch_add(a.ticket, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
          reset_shared();
       }
       __syncthreads();
-      const uint32_t mi = s_map;
+      const uint64_t mi = s_map;
       if (mi >= a.n_maps) break;
       const hn_batch_map M = a.maps[mi];
       hn_info* info = M.info;
@@ -197,54 +197,82 @@ __global__ __launch_bounds__(1024) void k_batch(BatchArgs<K> a) {
    }
 }
 
-__global__ void k_zero_u32(uint32_t* p) { *p = 0; }
-
-// The next work unit of a wave, wave-uniform (a scalar register).  Every
-// lane adds 1, so a wave takes kWave tickets at once (one atomic after the
-// compiler folds the wave's adds) and its unit is the first ticket / kWave.
-// No lane-0-only branch: with one, and the unit taken from a shuffle, the
-// compiler treats the unit as divergent and restructures the work loop
-// around the whole-wave probe loops so that a skipped unit does not take a
-// new ticket (the loop never ends).
-__device__ __forceinline__ uint32_t next_unit(uint32_t* ticket) {
-   const uint32_t t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-   return __builtin_amdgcn_readfirstlane(t) / kWave;
-}
-
-// Longest query slice of any map; the unit ticket starts at 0.
-__global__ __launch_bounds__(256) void k_batch_prep(const hn_batch_map* maps, uint32_t n_maps, uint64_t* max_qry,
-                                                    uint32_t* t_qry) {
-   uint64_t mq = 0;
-   for (uint32_t m = threadIdx.x; m < n_maps; m += blockDim.x) mq = maps[m].qry_len > mq ? maps[m].qry_len : mq;
-   mq = wave_max(mq);
-   __shared__ uint64_t wq[256 / kWave];
-   if (__lane_id() == 0) wq[threadIdx.x / kWave] = mq;
-   __syncthreads();
-   if (threadIdx.x == 0) {
-      uint64_t b = 0;
-      for (int w = 0; w < 256 / kWave; ++w) b = wq[w] > b ? wq[w] : b;
-      *max_qry = b;
-      *t_qry = 0;
-   }
-}
+__global__ void k_zero_u64(uint64_t* p) { *p = 0; }
 
 // ---------------------------------------------------------- retrieve phase
 //
 // The retrieve phase runs on its own, one key per lane (lane_lookup): every
-// map's queries are cut into units of kBatchUnit; unit u is chunk u % umax of
-// map u / umax (umax: the most units any map has), so units come in map order
-// (largest maps first) and a wave takes the next one by an atomic ticket.
-// Units past the end of a map's queries are skipped.  Results per map are
-// those of the plain retrieve: maps are independent, and the phases still run
-// insert -> retrieve -> erase -> clean for each of them.
+// map's queries are cut into units of kBatchUnit, numbered map after map
+// (map m owns units [unit_start[m], unit_start[m+1]), so units come in the
+// caller's map order, largest maps first), and a wave takes the next unit by
+// a 64-bit ticket.  No unit is empty and no ticket is wasted.  Results per
+// map are those of the plain retrieve: maps are independent, and the phases
+// still run insert -> retrieve -> erase -> clean for each of them.
 constexpr uint64_t kBatchUnit = 4096;
 
 struct BatchScratch {
-   uint32_t ticket;     // map ticket of k_batch
-   uint32_t qry_ticket; // unit ticket of the retrieve phase (kWave tickets per unit)
-   uint64_t max_qry;    // longest query slice of any map
+   uint64_t ticket;     // map ticket of k_batch
+   uint64_t qry_ticket; // unit ticket of the retrieve phase
+   uint64_t units;      // units of all maps
+   uint64_t reserved[5];
+   // followed by unit_start[n_maps + 1]
 };
-static_assert(sizeof(BatchScratch) <= 256, "fits hn_batch_scratch_bytes");
+static_assert(sizeof(BatchScratch) == 64, "header of hn_batch_scratch_bytes");
+
+__host__ __device__ inline uint64_t* unit_start(BatchScratch* sc) { return reinterpret_cast<uint64_t*>(sc + 1); }
+
+size_t batch_scratch_bytes(uint32_t n_maps) {
+   const size_t b = sizeof(BatchScratch) + (static_cast<size_t>(n_maps) + 1) * sizeof(uint64_t);
+   return (b + 255) & ~static_cast<size_t>(255);
+}
+
+// unit_start = exclusive prefix sum of ceil(qry_len / kBatchUnit) over the
+// maps, units = the total; the unit ticket starts at 0.  One workgroup of
+// 1024 threads, 1024 maps per step.
+__global__ __launch_bounds__(1024) void k_batch_prep(const hn_batch_map* maps, uint32_t n_maps, BatchScratch* sc) {
+   __shared__ uint64_t wsum[1024 / kWave];
+   __shared__ uint64_t carry;
+   const int lane = __lane_id();
+   const int w = threadIdx.x / kWave;
+   uint64_t* us = unit_start(sc);
+   if (threadIdx.x == 0) carry = 0;
+   __syncthreads();
+   for (uint64_t base = 0; base < n_maps; base += 1024) {
+      const uint64_t m = base + threadIdx.x;
+      const uint64_t u = m < n_maps ? (maps[m].qry_len + kBatchUnit - 1) / kBatchUnit : 0;
+      uint64_t x = u; // inclusive scan over the wave
+      for (int o = 1; o < kWave; o <<= 1) {
+         const uint64_t y = __shfl_up(x, o, kWave);
+         if (lane >= o) x += y;
+      }
+      if (lane == kWave - 1) wsum[w] = x;
+      __syncthreads();
+      uint64_t off = carry;
+      for (int i = 0; i < w; ++i) off += wsum[i];
+      if (m < n_maps) us[m] = off + x - u;
+      __syncthreads();
+      if (threadIdx.x == 1023) carry = off + x;
+      __syncthreads();
+   }
+   if (threadIdx.x == 0) {
+      us[n_maps] = carry;
+      sc->units = carry;
+      sc->qry_ticket = 0;
+   }
+}
+
+// The next work unit of a wave.  Lane 0 draws one ticket and readfirstlane
+// hands it to the wave as a scalar, so the unit -- and every branch on it --
+// is wave-uniform by construction and each wave takes exactly one ticket per
+// unit.  Must be called with the whole wave active (lane 0 is then the first
+// active lane).
+__device__ __forceinline__ uint64_t next_unit(uint64_t* ticket) {
+   uint64_t t = 0;
+   if (__lane_id() == 0) t = __hip_atomic_fetch_add(ticket, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+   const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(t));
+   const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(t >> 32));
+   return (static_cast<uint64_t>(hi) << 32) | lo;
+}
 
 template <typename K>
 __global__ __launch_bounds__(kWave) void k_batch_retrieve_lane(BatchArgs<K> a, BatchScratch* sc) {
@@ -252,17 +280,25 @@ __global__ __launch_bounds__(kWave) void k_batch_retrieve_lane(BatchArgs<K> a, B
    __shared__ uint32_t wb_s[kWave];
    __shared__ K ring_v[kLaneRing];
    __shared__ uint8_t ring_s[kLaneRing];
-   const uint64_t umax = (sc->max_qry + kBatchUnit - 1) / kBatchUnit;
-   const uint64_t units = umax * a.n_maps;
+   const uint64_t units = sc->units;
+   const uint64_t* us = unit_start(sc);
    for (;;) {
-      const uint32_t u = next_unit(&sc->qry_ticket);
+      const uint64_t u = next_unit(&sc->qry_ticket);
       if (u >= units) break;
-      const uint32_t m = static_cast<uint32_t>(u / umax);
-      const uint64_t q0 = (u % umax) * kBatchUnit;
-      const hn_batch_map M = a.maps[m];
+      // the map holding unit u: us[lo] <= u < us[hi] throughout, so map lo has units
+      uint32_t lo = 0, hi = a.n_maps;
+      while (hi - lo > 1) {
+         const uint32_t mid = lo + (hi - lo) / 2;
+         if (us[mid] <= u)
+            lo = mid;
+         else
+            hi = mid;
+      }
+      const hn_batch_map M = a.maps[lo];
+      const uint64_t q0 = (u - us[lo]) * kBatchUnit;
       hn_info* info = M.info;
       const int sp = info->size_power;
-      if (q0 < M.qry_len && sp >= 1 && sp <= max_size_power<K>()) {
+      if (sp >= 1 && sp <= max_size_power<K>()) {
          const uint64_t q1 = q0 + kBatchUnit < M.qry_len ? q0 + kBatchUnit : M.qry_len;
          const uint64_t ovf = info->current_max_bucket_overflow;
          const bool ordered = (info->reserved1 & kInfoOrdered) != 0;
@@ -339,25 +375,25 @@ int batch_run(const hn_batch_map* maps, uint32_t n_maps, uint32_t ops, const K* 
    if (blocks > n_maps) blocks = n_maps;
    BatchKernel<K> k = pick_batch<K>(G);
    BatchScratch* sc = static_cast<BatchScratch*>(scratch);
-   uint32_t* ticket = &sc->ticket;
+   uint64_t* ticket = &sc->ticket;
    BatchArgs<K> a{maps, n_maps, ops, ik, iv, qk, qv, qh, ek, static_cast<Slot<K>*>(clean_scratch), ticket};
    // insert; retrieve (one key per lane, its own launch); erase and clean
    if (ops & HN_BATCH_INSERT) {
       a.ops = HN_BATCH_INSERT;
-      hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, ticket);
+      hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, ticket);
       hipLaunchKernelGGL(k, dim3(blocks), dim3(bs), 0, s, a);
    }
    if (ops & HN_BATCH_RETRIEVE) {
       static std::atomic<int> occ{0};
       const int per_cu = resident_blocks_per_cu(k_batch_retrieve_lane<K>, kWave, occ);
       a.ops = HN_BATCH_RETRIEVE;
-      hipLaunchKernelGGL(k_batch_prep, dim3(1), dim3(256), 0, s, maps, n_maps, &sc->max_qry, &sc->qry_ticket);
+      hipLaunchKernelGGL(k_batch_prep, dim3(1), dim3(1024), 0, s, maps, n_maps, sc);
       hipLaunchKernelGGL(k_batch_retrieve_lane<K>, dim3(static_cast<uint32_t>(cu_count()) * per_cu), dim3(kWave), 0, s,
                          a, sc);
    }
    if (ops & (HN_BATCH_ERASE | HN_BATCH_CLEAN)) {
       a.ops = ops & (HN_BATCH_ERASE | HN_BATCH_CLEAN);
-      hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, ticket);
+      hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, ticket);
       hipLaunchKernelGGL(k, dim3(blocks), dim3(bs), 0, s, a);
    }
    return static_cast<int>(hipGetLastError());
@@ -378,7 +414,7 @@ using namespace hn;
 
 extern "C" {
 
-size_t hn_batch_scratch_bytes(void) { return 256; }
+size_t hn_batch_scratch_bytes(uint32_t n_maps) { return batch_scratch_bytes(n_maps); }
 
 int hn_batch_init_u32(const hn_batch_map* maps, uint32_t n, const int32_t* sps, hn_stream_t s) {
    return batch_init<uint32_t>(maps, n, sps, s);

@@ -455,6 +455,7 @@ int reset_keys(const void* pairs, void* buckets, hn_info* info, size_t len, hipS
 template <typename K>
 int reset_all(void* buckets, hn_info* info, size_t len, hipStream_t s) {
    if (!buckets || !info) return HN_EINVAL;
+   if (len == 0) return 0; // the reference launches no block for len 0 (hashers.h:155-163): fill stays
    const uint32_t blocks = stream_blocks(len, 256);
    hipLaunchKernelGGL(k_reset_all<K>, dim3(blocks), dim3(256), 0, s, static_cast<Slot<K>*>(buckets), info,
                       static_cast<uint64_t>(len));

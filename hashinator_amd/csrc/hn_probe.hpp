@@ -121,8 +121,11 @@ __device__ __forceinline__ void insert_range(const InsertArgs<K>& a, const int s
          const uint64_t be = group_bits<G>(__ballot(valid && s.key == kEmpty), gbase);
          const int fm = bm ? first_bit(bm) : G;
          bool done = false;
-         // Only EMPTY slots ahead of an existing copy of the key may be claimed.
-         uint64_t em = fm >= 64 ? be : (be & ((1ull << fm) - 1ull));
+         // A copy of the key anywhere in the window wins over the window's
+         // EMPTY slots, as in insert_kernel (kernels_AMD.h:339-349 before the
+         // claim loop at 353): a table with EMPTY holes inside a probe path
+         // (left by reset_to_empty) gets no second copy from this window.
+         uint64_t em = fm < G ? 0 : be;
          while (em) {
             const int j = first_bit(em);
             K old = kEmpty;

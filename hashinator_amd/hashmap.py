@@ -12,7 +12,10 @@ int32/int64 (or uint32/uint64) carry their bit patterns.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
+import functools
+import inspect
 import math
 
 import numpy as np
@@ -34,6 +37,56 @@ def _ptr(t):
 def _stream(stream=None):
     s = stream if stream is not None else torch.cuda.current_stream()
     return C.c_void_p(s.cuda_stream)
+
+
+def on_stream(fn):
+    """Run a method with its ``stream`` argument as torch's current stream.
+
+    Then every allocation of the call comes from that stream's pool, every
+    table initialisation and Info write is enqueued on it, and every read
+    back (``.cpu()``, ``.item()``) waits for it: the kernels launched on
+    ``stream`` and the work around them are ordered by the stream itself.
+    """
+    idx = list(inspect.signature(fn).parameters).index("stream")
+
+    @functools.wraps(fn)
+    def call(*a, **kw):
+        s = kw["stream"] if "stream" in kw else (a[idx] if len(a) > idx else None)
+        ctx = torch.cuda.stream(s) if s is not None else contextlib.nullcontext()
+        with ctx:
+            return fn(*a, **kw)
+
+    return call
+
+
+def _same_device(a: torch.device, b: torch.device) -> bool:
+    if a.type != b.type:
+        return False
+    if a.type != "cuda":
+        return True
+    ia = a.index if a.index is not None else torch.cuda.current_device()
+    ib = b.index if b.index is not None else torch.cuda.current_device()
+    return ia == ib
+
+
+def out_tensor(t, n, dtype, device, what):
+    """Check a caller-provided output tensor before a kernel writes to it:
+    on ``device``, of ``dtype`` (or its unsigned twin), contiguous, with at
+    least ``n`` elements.  Returns it viewed as ``dtype``."""
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{what} must be a torch tensor, got {type(t).__name__}")
+    twin = {torch.int32: torch.uint32, torch.int64: torch.uint64}.get(dtype)
+    if t.dtype == twin:
+        t = t.view(dtype)
+    if t.dtype != dtype:
+        raise TypeError(f"{what} must be {dtype}, got {t.dtype}")
+    if not _same_device(t.device, torch.device(device)):
+        raise ValueError(f"{what} is on {t.device}, the map is on {device}")
+    if not t.is_contiguous():
+        raise ValueError(f"{what} must be contiguous")
+    if t.numel() < n:
+        raise ValueError(f"{what} holds {t.numel()} elements, {n} needed")
+    return t
 
 
 def as_device(a, key_bits, device="cuda"):
@@ -108,9 +161,10 @@ class Hashmap:
     def _tune(self):
         return C.byref(self.tune) if self.tune is not None else None
 
-    def _new_table(self, sp):
+    def _new_table(self, sp, init=True):
         t = torch.empty((1 << sp, 2), dtype=_TDT[self.kb], device=self.device)
-        check(self._fn("table_init")(_ptr(t), 1 << sp, _stream()), "hn_table_init")
+        if init:
+            check(self._fn("table_init")(_ptr(t), 1 << sp, _stream()), "hn_table_init")
         return t
 
     def _sync(self, stream=None):
@@ -182,6 +236,7 @@ class Hashmap:
         if needed > self._sp:
             self.resize(needed, targets.device, stream)
 
+    @on_stream
     def insert(self, keys, vals=None, targetLF: float = 0.5, stream=None):
         """insert(keys, vals, len, targetLF, s) (hashinator.h:1318-1335), or
         insert(pairs, len, targetLF, s) (1358-1374) when ``vals`` is None and
@@ -239,6 +294,7 @@ class Hashmap:
             self._reorder(stream)
         self._sync(stream)
 
+    @on_stream
     def reorder(self, stream=None) -> bool:
         """Robin Hood reorder of every cluster (no counterpart in the
         reference).  Same live set and occupancy; retrieve and erase of absent
@@ -259,6 +315,7 @@ class Hashmap:
         """True when the table carries the Robin Hood order flag."""
         return bool(self.info().reserved1 & _lib.HN_INFO_ORDERED)
 
+    @on_stream
     def insertIndex(self, keys, targetLF: float = 0.5, stream=None):
         """insertIndex (hashinator.h:1338-1355): value = position in ``keys``."""
         k = self._dev(keys)
@@ -273,6 +330,7 @@ class Hashmap:
                                        self._tune()), "insertIndex")
         self._after_insert(n, stream)
 
+    @on_stream
     def retrieve(self, keys, vals=None, hit=None, stream=None):
         """retrieve(keys, vals, len, s) (hashinator.h:1377-1384).  ``vals`` is
         updated in place on hits and left untouched on misses; with ``vals``
@@ -282,12 +340,18 @@ class Hashmap:
         k = self._dev(keys)
         if k.dim() == 2 and vals is None:
             n = k.shape[0]
+            if hit is not None:
+                hit = out_tensor(hit, n, torch.uint8, self.device, "hit")
             h = hit if hit is not None else torch.empty(n, dtype=torch.uint8, device=self.device)
             check(self._fn("retrieve_pairs")(_ptr(k), _ptr(h), _ptr(self.buckets), _ptr(self._info), n,
                                              _stream(stream), self._tune()), "retrieve(pairs)")
             self._sync(stream)
             return k, h
         n = k.numel()
+        if vals is not None:
+            vals = out_tensor(vals, n, _TDT[self.kb], self.device, "vals")
+        if hit is not None:
+            hit = out_tensor(hit, n, torch.uint8, self.device, "hit")
         v = vals if vals is not None else torch.zeros(n, dtype=_TDT[self.kb], device=self.device)
         h = hit if hit is not None else torch.empty(n, dtype=torch.uint8, device=self.device)
         check(self._fn("retrieve")(_ptr(k), _ptr(v), _ptr(h), _ptr(self.buckets), _ptr(self._info), n,
@@ -295,6 +359,7 @@ class Hashmap:
         self._sync(stream)
         return k, v, h
 
+    @on_stream
     def erase(self, keys, stream=None):
         """erase(keys, len, s) (hashinator.h:1397-1409)."""
         k = self._dev(keys)
@@ -302,6 +367,7 @@ class Hashmap:
                                 self._tune()), "erase")
         self._sync(stream)
 
+    @on_stream
     def clean_tombstones(self, stream=None, scratch_budget: int = 0):
         """clean_tombstones(s) (hashinator.h:1261-1315).  ``scratch_budget``
         bounds the scratch in bytes (0: enough for the whole table)."""
@@ -319,6 +385,7 @@ class Hashmap:
             self._reorder(stream)  # the bounded-scratch path reinserts in arrival order
             self._sync(stream)
 
+    @on_stream
     def device_rehash(self, newSizePower: int, stream=None):
         """device_rehash(p, s) (hashinator.h:284-336)."""
         if newSizePower > 32:
@@ -327,7 +394,7 @@ class Hashmap:
         # insert(valid, n, targetLF=1) grows further when the elements do not fit (hashinator.h:332, 1329-1332)
         if fill > 0:
             newSizePower = max(newSizePower, math.ceil(math.log2(fill)))
-        nb = self._new_table(newSizePower)
+        nb = self._new_table(newSizePower, init=False)  # both rehash paths write every slot of it
         # Growing a map of many keys: one bulk build from the old table's live
         # slots (Robin Hood order directly); else reinsert them by probing.
         sb = 0
@@ -341,6 +408,7 @@ class Hashmap:
             scratch = torch.empty(self._L.hn_rehash_scratch_bytes(), dtype=torch.uint8, device=self.device)
             check(self._fn("rehash")(_ptr(self.buckets), self._sp, _ptr(nb), newSizePower, _ptr(self._info),
                                      _ptr(scratch), _stream(stream)), "device_rehash")
+        old = self.buckets  # read by the rehash until the stream is synchronised
         self.buckets = nb
         self._sp = newSizePower
         if sb:
@@ -348,18 +416,22 @@ class Hashmap:
         if self.auto_reorder and not (sb and self.ordered()):
             self._reorder(stream)
         self._sync(stream)
+        del old, scratch
 
+    @on_stream
     def resize(self, newSizePower: int, t: int = targets.device, stream=None):
         """resize(p, targets, s) (hashinator.h:552-566).  Every target runs
         the device rehash here; the host rehash of the reference is its
         CPU-only path, not part of the accelerated API."""
         self.device_rehash(newSizePower, stream)
 
+    @on_stream
     def resize_to_lf(self, targetLF: float = 0.5, t: int = targets.device, stream=None):
         """resize_to_lf (hashinator.h:530-546)."""
         while self.load_factor() > targetLF:
             self.device_rehash(self._sp + 1, stream)
 
+    @on_stream
     def clear(self, t: int = targets.device, stream=None, length: int = 0):
         """clear(targets, s, len) (hashinator.h:491-518).  targets.device:
         keys -> EMPTY and fill = 0 only; otherwise a fresh table and Info."""
@@ -371,6 +443,7 @@ class Hashmap:
             check(self._L.hn_info_init(_ptr(self._info), self._sp, _stream(stream)), "clear")
         self._sync(stream)
 
+    @on_stream
     def performCleanupTasks(self, stream=None):
         """GPU performCleanupTasks (hashinator.h:642-650)."""
         if self.tombstone_ratio() > 0.025:
@@ -378,6 +451,7 @@ class Hashmap:
         while self.info().current_max_bucket_overflow > _lib.HN_BUCKET_OVERFLOW:
             self.device_rehash(self._sp + 1, stream)
 
+    @on_stream
     def download(self, stream=None):
         """download() (hashinator.h:1457-1467): rehash when the overflow grew
         beyond BUCKET_OVERFLOW, else clean tombstones if any."""
@@ -387,6 +461,7 @@ class Hashmap:
             self.clean_tombstones(stream)
 
     # ----------------------------------------------------------- extraction (hashinator.h:1154-1259)
+    @on_stream
     def extractPattern(self, pred: hn_pred | None = None, stream=None):
         """Pairs of the bucket array selected by ``pred`` (default: valid keys),
         in bucket order, as an (n, 2) device tensor."""
@@ -394,26 +469,32 @@ class Hashmap:
 
         return copy_if_pairs(self.buckets, pred if pred is not None else valid_pred(), self.kb, stream)
 
+    @on_stream
     def extractKeysByPattern(self, pred: hn_pred | None = None, stream=None):
         from .tools import copy_keys_if_pairs
 
         return copy_keys_if_pairs(self.buckets, pred if pred is not None else valid_pred(), self.kb, stream)
 
+    @on_stream
     def extractAllKeys(self, stream=None):
         return self.extractKeysByPattern(valid_pred(), stream)
 
     # The *Loop variants (hashinator.h:1186-1259): the reference runs them as
     # one block looping over the buckets; the same elements and order come
     # from the single-pass compaction here.
+    @on_stream
     def extractPatternLoop(self, pred: hn_pred | None = None, stream=None):
         return self.extractPattern(pred, stream)
 
+    @on_stream
     def extractLoop(self, stream=None):
         return self.extractPattern(valid_pred(), stream)
 
+    @on_stream
     def extractKeysByPatternLoop(self, pred: hn_pred | None = None, stream=None):
         return self.extractKeysByPattern(pred, stream)
 
+    @on_stream
     def extractAllKeysLoop(self, stream=None):
         return self.extractAllKeys(stream)
 

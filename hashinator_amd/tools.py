@@ -15,7 +15,7 @@ import torch
 
 from . import _lib
 from ._lib import check, hn_pred
-from .hashmap import _ptr, _stream
+from .hashmap import _ptr, _stream, on_stream
 
 _TDT = {32: torch.int32, 64: torch.int64}
 
@@ -38,6 +38,7 @@ def pred(kind=_lib.HN_PRED_VALID, field=0, mask=0, value=0) -> hn_pred:
     return hn_pred(kind, field, mask, value)
 
 
+@on_stream
 def copy_if_pairs(pairs: torch.Tensor, p: hn_pred, key_bits: int, stream=None, out=None, scratch=None,
                   sync=True):
     """Stable copy_if over an (n, 2) tensor of hash_pair<K,K>.  Returns the
@@ -54,6 +55,7 @@ def copy_if_pairs(pairs: torch.Tensor, p: hn_pred, key_bits: int, stream=None, o
     return out[: int(cnt.item())]
 
 
+@on_stream
 def copy_keys_if_pairs(pairs: torch.Tensor, p: hn_pred, key_bits: int, stream=None, out=None, scratch=None,
                        sync=True):
     """copy_keys_if: the .first of every selected pair, in input order."""
@@ -69,6 +71,7 @@ def copy_keys_if_pairs(pairs: torch.Tensor, p: hn_pred, key_bits: int, stream=No
     return out[: int(cnt.item())]
 
 
+@on_stream
 def copy_if_flags(elems: torch.Tensor, flags: torch.Tensor, stream=None, out=None, scratch=None, sync=True):
     """copy_if with the rule pre-evaluated: keep elems[i] where flags[i] != 0.
     ``elems`` is (n,) of 4- or 8-byte items or (n, 2) of 8-byte pairs."""
@@ -86,6 +89,7 @@ def copy_if_flags(elems: torch.Tensor, flags: torch.Tensor, stream=None, out=Non
     return out[: int(cnt.item())]
 
 
+@on_stream
 def split_prefix_scan(inp: torch.Tensor, out: torch.Tensor | None = None, stream=None, scratch=None, sync=True):
     """Exclusive prefix sum with wraparound (uint32 via int32 bit patterns, or uint64 via int64)."""
     L = _lib.load()
@@ -100,7 +104,7 @@ def split_prefix_scan(inp: torch.Tensor, out: torch.Tensor | None = None, stream
         raise TypeError("scan supports 4- and 8-byte elements")
     check(fn(_ptr(inp), _ptr(out), n, _ptr(sc), sb, _stream(stream)), "split_prefix_scan")
     if sync:
-        (stream or torch.cuda.current_stream()).synchronize()
+        torch.cuda.current_stream().synchronize()
     return out
 
 

@@ -152,16 +152,19 @@ def batched_workload(n_maps=4096, sp_lo=8, sp_hi=16, lf=0.6, grid=200, sigma=20.
     sps = rng.integers(sp_lo, sp_hi + 1, size=n_maps)
     ins, vals, qry, ers = [], [], [], []
     cells = grid ** 3
+    member = np.zeros(cells, dtype=bool)  # the current map's keys, cleared after each map
     for m in range(n_maps):
         n = int(lf * (1 << int(sps[m])))
         center = rng.integers(0, grid, size=3)
         k = velocity_grid_keys(n, center, sigma, rng, grid)
         v = values(k, int(seed) + m)
+        member[k] = True
         a = np.empty(0, dtype=np.uint32)
         while a.size < n:
             cand = rng.integers(0, cells, size=n - a.size + n // 4 + 16).astype(np.uint32)
-            cand = cand[~np.isin(cand, k)]
+            cand = cand[~member[cand]]
             a = np.unique(np.concatenate([a, cand]))
+        member[k] = False
         a = rng.permutation(a)[:n]
         q = rng.permutation(np.concatenate([k, a]))
         e = rng.choice(k, int(erase_frac * n), replace=False) if n else np.zeros(0, np.uint32)

@@ -68,8 +68,8 @@ enum { HN_STATUS_SUCCESS = 0, HN_STATUS_FAIL = 1, HN_STATUS_INVALID = 2 };
 /* Launch tuning.  NULL means "use the tuned defaults".  probe_width is the
  * number of lanes that probe one key together (1, 2, 4, 8, 16, 32 or 64; a
  * window of probe_width consecutive, aligned slots is read per step).
- * probe_width 0 (and NULL) selects the default: for inserts 8 lanes (u64) or
- * 16 (u32); for retrieve and erase the lane kernel, one key per lane, whose
+ * probe_width 0 (and NULL) selects the default: for inserts 4 lanes; for
+ * reset 8 (u64) or 16 (u32); for retrieve and erase the lane kernel, one key per lane, whose
  * 128-byte windows are loaded by 8 lanes together and handed to their lanes
  * through LDS (it runs one wave per block and ignores block_size).
  * block_size is threads per workgroup (64..1024, multiple of 64); grid_blocks
@@ -172,14 +172,19 @@ HN_API int hn_erase_u64(const uint64_t* keys, void* buckets, hn_info* info, size
 /* clear(targets::device) (hashinator.h:491-518) -> Hasher::reset_all
  * (hashers.h:155-163) -> reset_all_to_empty (kernels_AMD.h:26-43).  Keys of
  * the first `len` slots become EMPTYBUCKET, fill = 0; values, tombstone
- * counter and overflow are left as they are. */
+ * counter and overflow are left as they are.  len == 0 changes nothing (the
+ * reference launches no block then). */
 HN_API int hn_reset_all_u32(void* buckets, hn_info* info, size_t len, hn_stream_t s);
 HN_API int hn_reset_all_u64(void* buckets, hn_info* info, size_t len, hn_stream_t s);
 
 /* Hasher::reset(src, dst, info, len, s) (hashers.h:144-152) -> reset_to_empty
- * (kernels_AMD.h:65-134): for each src pair, the first slot holding its key
- * (probing the whole table; EMPTY does not end the probe) becomes EMPTY;
- * fill -= len. */
+ * (kernels_AMD.h:65-134): for each src pair, the first slot in probe order
+ * holding its key (probing the whole table; EMPTY does not end the probe)
+ * becomes EMPTY, its value kept; fill -= len, found or not.  The EMPTY holes
+ * this leaves inside probe paths are what clean_tombstones' reinsert fills;
+ * an insert of a key that still has a copy behind a hole updates that copy
+ * when it lies in the same probe window as the hole (as the reference's
+ * 64-slot windows do) and otherwise claims the hole. */
 HN_API int hn_reset_u32(const void* src_pairs, void* buckets, hn_info* info, size_t len, hn_stream_t s);
 HN_API int hn_reset_u64(const void* src_pairs, void* buckets, hn_info* info, size_t len, hn_stream_t s);
 
@@ -373,8 +378,9 @@ typedef struct hn_batch_map {
 
 enum { HN_BATCH_INSERT = 1, HN_BATCH_RETRIEVE = 2, HN_BATCH_ERASE = 4, HN_BATCH_CLEAN = 8 };
 
-/* Device scratch of hn_batch_run (the map ticket). */
-HN_API size_t hn_batch_scratch_bytes(void);
+/* Device scratch of hn_batch_run for a batch of n_maps maps: the map and
+ * unit tickets and each map's first retrieve unit (8 bytes per map). */
+HN_API size_t hn_batch_scratch_bytes(uint32_t n_maps);
 
 /* Fresh tables ({EMPTYBUCKET, 0}) and Info(sps[m]) for all maps, one launch.
  * `maps` and `sps` are device arrays. */

@@ -129,6 +129,7 @@ def _declare(L):
         h("retrieve").argtypes = [P, IP, P, P, P, C.c_size_t]
         h("erase").argtypes = [P, IP, P, C.c_size_t]
         h("reset_all").argtypes = [P, IP, C.c_size_t]
+        h("reset").argtypes = [P, IP, P, C.c_size_t]
         h("clean_tombstones").restype, h("clean_tombstones").argtypes = C.c_size_t, [P, IP]
         h("rehash").restype, h("rehash").argtypes = C.c_int, [P, C.c_int, P, C.c_int, IP]
         h("max_probe").restype, h("max_probe").argtypes = C.c_uint64, [P, C.c_int]
@@ -266,8 +267,13 @@ class GpuModel:
         k = self._k(keys)
         self._g("erase")(_p(self.b), C.byref(self.info), _p(k), k.size)
 
-    def reset_all(self):
-        self._g("reset_all")(_p(self.b), C.byref(self.info), self.b.size // 2)
+    def reset_all(self, len=None):
+        n = self.b.size // 2 if len is None else len
+        self._g("reset_all")(_p(self.b), C.byref(self.info), n)
+
+    def reset(self, pairs):
+        pr = self._k(pairs)
+        self._g("reset")(_p(self.b), C.byref(self.info), _p(pr), pr.size // 2)
 
     def clean_tombstones(self):
         return self._g("clean_tombstones")(_p(self.b), C.byref(self.info))

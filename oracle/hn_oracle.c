@@ -345,11 +345,36 @@ static inline uint32_t or_hash_idx64(uint64_t k, int sp) { return (uint32_t)or_f
       }                                                                                                        \
    }                                                                                                           \
                                                                                                                \
-   /* reset_all_to_empty (kernels_AMD.h:26-43): keys -> EMPTY, fill = 0; values, tombstone count and         \
-    * overflow are left as they are. */                                                                       \
+   /* reset_all_to_empty (kernels_AMD.h:26-43): keys of slots [0, len) -> EMPTY, fill = 0 (written by thread  \
+    * 0, 38-40); values, tombstone count and overflow are left as they are.  Hasher::reset_all launches       \
+    * ceil(len / MAX_BLOCKSIZE) blocks (hashers.h:155-163): len == 0 launches nothing and changes nothing. */ \
    void or_gpu##W##_reset_all(K* b, or_info* info, size_t len) {                                               \
+      if (len == 0) return;                                                                                    \
       for (size_t i = 0; i < len; ++i) b[2 * i] = OR_EMPTY##W;                                                 \
       info->fill = 0;                                                                                          \
+   }                                                                                                           \
+                                                                                                               \
+   /* reset_to_empty (kernels_AMD.h:65-134) through Hasher::reset (hashers.h:144-152): for each src pair in   \
+    * turn, the first slot in probe order from its home that holds its key becomes EMPTY; the probe covers   \
+    * the whole table (the loop bound is 1 << sizePower, 100) and EMPTY slots do not end it (only a match is \
+    * voted on, 112); the slot's value is kept (only .first is written, 119).  fill -= len whether or not a  \
+    * key was found (76-78).  len == 0 launches nothing. */                                                   \
+   void or_gpu##W##_reset(K* b, or_info* info, const K* pairs, size_t len) {                                   \
+      if (len == 0) return;                                                                                    \
+      const int sp = info->size_power;                                                                         \
+      const size_t n = (size_t)1 << sp, mask = n - 1;                                                          \
+      for (size_t i = 0; i < len; ++i) {                                                                       \
+         const K key = pairs[2 * i];                                                                           \
+         const uint64_t h = (uint64_t)or_hash_idx##W(key, sp);                                                 \
+         for (size_t off = 0; off < n; ++off) {                                                                \
+            const size_t s = (size_t)(h + off) & mask;                                                         \
+            if (b[2 * s] == key) {                                                                             \
+               b[2 * s] = OR_EMPTY##W;                                                                         \
+               break;                                                                                          \
+            }                                                                                                  \
+         }                                                                                                     \
+      }                                                                                                        \
+      info->fill -= len;                                                                                       \
    }                                                                                                           \
                                                                                                                \
    /* clean_tombstones (hashinator.h:1261-1315): TOMBSTONE -> EMPTY, extract the live elements that are not   \

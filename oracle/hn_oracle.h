@@ -95,6 +95,7 @@ void or_gpu32_retrieve(const uint32_t* buckets, const or_info* info, const uint3
                        uint8_t* hit, size_t n);
 void or_gpu32_erase(uint32_t* buckets, or_info* info, const uint32_t* keys, size_t n);
 void or_gpu32_reset_all(uint32_t* buckets, or_info* info, size_t len);
+void or_gpu32_reset(uint32_t* buckets, or_info* info, const uint32_t* pairs, size_t len);
 size_t or_gpu32_clean_tombstones(uint32_t* buckets, or_info* info);
 int or_gpu32_rehash(const uint32_t* old_buckets, int old_sp, uint32_t* new_buckets, int new_sp, or_info* info);
 uint64_t or_gpu32_max_probe(const uint32_t* buckets, int size_power);
@@ -108,6 +109,7 @@ void or_gpu64_retrieve(const uint64_t* buckets, const or_info* info, const uint6
                        uint8_t* hit, size_t n);
 void or_gpu64_erase(uint64_t* buckets, or_info* info, const uint64_t* keys, size_t n);
 void or_gpu64_reset_all(uint64_t* buckets, or_info* info, size_t len);
+void or_gpu64_reset(uint64_t* buckets, or_info* info, const uint64_t* pairs, size_t len);
 size_t or_gpu64_clean_tombstones(uint64_t* buckets, or_info* info);
 int or_gpu64_rehash(const uint64_t* old_buckets, int old_sp, uint64_t* new_buckets, int new_sp, or_info* info);
 uint64_t or_gpu64_max_probe(const uint64_t* buckets, int size_power);

@@ -56,6 +56,41 @@ def check_against_oracle(bm, wl, key_bits, ops=_lib.HN_BATCH_ALL):
         assert info.current_max_bucket_overflow >= max(64, next_overflow(mp)), m
 
 
+def check_bulk(bm, wl, key_bits, ops=_lib.HN_BATCH_ALL):
+    """check_against_oracle with one download of every device array (many maps)."""
+    sps, ins, vals, qry, ers = wl
+    nd = np.uint32 if key_bits == 32 else np.uint64
+    arena = bm.arena.cpu().numpy().view(nd)
+    qv = bm.qry_vals.cpu().numpy().view(nd)
+    qh = bm.qry_hit.cpu().numpy().astype(bool)
+    infos = bm.all_infos()
+    mp_fn = getattr(O.lib(), f"or_gpu{key_bits}_max_probe")
+    for m, sp in enumerate(sps):
+        sp = int(sp)
+        model = O.GpuModel(sp, key_bits)
+        if ops & _lib.HN_BATCH_INSERT:
+            model.insert(ins[m], vals[m])
+        if ops & _lib.HN_BATCH_RETRIEVE:
+            ev, eh = model.retrieve(qry[m])
+            lo, n = int(bm.qry_off[m]), int(bm.qry_len[m])
+            h, v = qh[lo:lo + n], qv[lo:lo + n]
+            assert np.array_equal(h, eh), m
+            assert np.array_equal(v[h], ev[eh]), m
+        if ops & _lib.HN_BATCH_ERASE:
+            model.erase(ers[m])
+        if ops & _lib.HN_BATCH_CLEAN:
+            model.clean_tombstones()
+        lo = int(bm.slot_off[m])
+        img = np.ascontiguousarray(arena[lo:lo + (1 << sp)])
+        fill, ovf, tomb = int(infos[m, 1]), int(infos[m, 2]), int(infos[m, 3])
+        err = int(infos[m, 4]) & 0xFFFFFFFF
+        assert np.array_equal(live_pairs(img, key_bits), live_pairs(model.b, key_bits)), m
+        assert np.array_equal(occupancy(img, key_bits), occupancy(model.b, key_bits)), m
+        assert fill == model.info.fill and tomb == model.info.tombstones and err == 0, m
+        assert reachable_all(img, key_bits, sp), m
+        assert ovf >= max(64, next_overflow(mp_fn(img.reshape(-1).ctypes.data, sp))), m
+
+
 @pytest.mark.parametrize("key_bits", [32, 64])
 @pytest.mark.parametrize("width,block", [(8, 512), (4, 256), (16, 1024), (1, 64)])
 def test_batch_all_ops_vs_oracle(gpu, key_bits, width, block):
@@ -93,6 +128,50 @@ def test_batch_velocity_grid_workload_small(gpu):
     bm.run()
     torch.cuda.synchronize()
     check_against_oracle(bm, wl, 32)
+
+
+def test_batch_velocity_grid_workload_full_size(gpu):
+    """The full batched workload: 4,096 u32 maps of 2^8..2^16 slots at LF 0.6
+    (35.7 M keys), every map's retrieve results, live set, occupancy and Info
+    against the oracle's sequential insert -> retrieve -> erase -> clean."""
+    wl = W.batched_workload()
+    bm = BatchedMaps(*wl, key_bits=32)
+    bm.init()
+    bm.run()
+    torch.cuda.synchronize()
+    assert len(wl[0]) == 4096 and sum(len(k) for k in wl[1]) > 35_000_000
+    check_bulk(bm, wl, 32)
+
+
+@pytest.mark.parametrize("key_bits", [32, 64])
+def test_batch_skewed_query_units(gpu, key_bits):
+    """One map with 2 M queries (489 retrieve units) among 3,000 maps, most
+    with no queries at all: every unit is served once, every map's results
+    equal the oracle's."""
+    rng = np.random.default_rng(17)
+    n_maps = 3000
+    sps = np.concatenate([[18], rng.integers(2, 9, size=n_maps - 1)])
+    ins, vals, qry, ers = [], [], [], []
+    for m, sp in enumerate(sps):
+        n = int(0.7 * (1 << int(sp)))
+        k = W.unique_keys(n, key_bits, 300 + m)
+        ins.append(k)
+        vals.append(W.values(k, m).astype(k.dtype))
+        if m == 0:
+            a = W.unique_keys(n, key_bits, 300 + m, start=n + 2)
+            q = np.concatenate([np.tile(k, 8), a])[:2_000_000]
+            qry.append(rng.permutation(q))
+        elif m % 3 == 0:
+            qry.append(rng.permutation(np.concatenate([k, W.unique_keys(5, key_bits, 300 + m, start=n + 2)])))
+        else:
+            qry.append(k[:0])
+        ers.append(k[: n // 5])
+    wl = (sps, ins, vals, qry, ers)
+    bm = BatchedMaps(*wl, key_bits=key_bits)
+    bm.init()
+    bm.run()
+    torch.cuda.synchronize()
+    check_bulk(bm, wl, key_bits)
 
 
 def test_batch_graph_replays_match_eager(gpu):

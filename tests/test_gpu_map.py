@@ -7,6 +7,8 @@ mathematics fixes: the key/value set, the occupied-slot set (order
 independent under linear probing), reachability of every key, fill, and the
 overflow bound derived from the device's own image.
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 import torch
@@ -187,6 +189,130 @@ def test_reset_all_exact(gpu, bits):
     assert info_tuple(hm)[:4] == model.info.as_tuple()[:4]
 
 
+def _tombstoned(bits, sp, seed):
+    keys = keyset(int(0.95 * (1 << sp)), bits, seed=seed)
+    model = fresh_model(sp, bits)
+    model.insert(keys, (keys ^ NDT[bits](0x5A5A)).astype(NDT[bits]))
+    model.erase(keys[::5])
+    model.info.err = 0
+    return model
+
+
+def _call(hm, name, *args):
+    rc = hm._fn(name)(*args, C.c_void_p(torch.cuda.current_stream().cuda_stream))
+    assert rc == 0, (name, rc)
+    torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+@pytest.mark.parametrize("n_len", [0, 1, 333, 1024])
+def test_reset_all_lengths_exact(gpu, bits, n_len):
+    """reset_all over the first len slots of a tombstoned table with values
+    and a raised overflow: image and Info as the oracle's (len 0: no change)."""
+    sp = 10
+    model = _tombstoned(bits, sp, 71)
+    hm = Hashmap(sp, bits, auto_build=0)
+    upload(hm, model)
+    _call(hm, "reset_all", C.c_void_p(hm.bucket_ptr()), C.c_void_p(hm.info_ptr()), n_len)
+    model.reset_all(n_len)
+    assert np.array_equal(hm.host_buckets().reshape(-1), model.b)
+    assert info_tuple(hm) == model.info.as_tuple()
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+@pytest.mark.parametrize("sp", [6, 11, 16])
+def test_reset_exact(gpu, bits, sp):
+    """Hasher::reset (hashers.h:144-152): present keys, absent keys and
+    repeats on a tombstoned table -- image and Info equal the oracle's."""
+    model = _tombstoned(bits, sp, 73 + sp)
+    hm = Hashmap(sp, bits, auto_build=0)
+    upload(hm, model)
+    img0 = model.b.reshape(-1, 2)
+    present = img0[(img0[:, 0] != EMPTY[bits]) & (img0[:, 0] != TOMB[bits])]
+    rng = np.random.default_rng(sp)
+    pick = present[rng.choice(present.shape[0], present.shape[0] // 3, replace=False)]
+    absent = keyset(40, bits, seed=sp, start=10**6)
+    src = np.concatenate([pick, np.stack([absent, absent], axis=1), pick[:7]]).astype(NDT[bits])
+    rng.shuffle(src)
+    d = torch.from_numpy(src.view(np.int32 if bits == 32 else np.int64).copy()).cuda()
+    _call(hm, "reset", C.c_void_p(d.data_ptr()), C.c_void_p(hm.bucket_ptr()), C.c_void_p(hm.info_ptr()),
+          src.shape[0])
+    model.reset(src)
+    assert np.array_equal(hm.host_buckets().reshape(-1), model.b)
+    assert info_tuple(hm) == model.info.as_tuple()
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_reset_first_copy_in_probe_order(gpu, bits):
+    """Two copies of a key, the first in probe order behind an EMPTY home slot
+    and across the table's end: the device empties the same slots as the
+    oracle, one call at a time."""
+    from tests.test_oracle import _key_with_home
+
+    sp = 3
+    E = EMPTY[bits]
+    a = _key_with_home(bits, sp, 6)
+    img = np.array([[E, 0], [a, 11], [E, 0], [E, 0], [E, 0], [E, 0], [E, 0], [a, 77]], dtype=NDT[bits])
+    model = model_from(img, (sp, 2, 64, 0, 0), bits)
+    hm = Hashmap(sp, bits, auto_build=0)
+    upload(hm, model)
+    src = np.array([[a, 0]], dtype=NDT[bits])
+    d = torch.from_numpy(src.view(np.int32 if bits == 32 else np.int64).copy()).cuda()
+    for _ in range(2):
+        _call(hm, "reset", C.c_void_p(d.data_ptr()), C.c_void_p(hm.bucket_ptr()), C.c_void_p(hm.info_ptr()), 1)
+        model.reset(src)
+        assert np.array_equal(hm.host_buckets().reshape(-1), model.b)
+        assert info_tuple(hm) == model.info.as_tuple()
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_reset_then_reinsert(gpu, bits):
+    """clean_tombstones' own use of reset (hashinator.h:1299-1311): the
+    displaced live elements are reset and inserted again; live set, occupied
+    slots and Info equal the oracle's."""
+    sp = 12
+    model = _tombstoned(bits, sp, 79)
+    hm = Hashmap(sp, bits, auto_build=0, auto_reorder=0)
+    upload(hm, model)
+    img0 = model.b.reshape(-1, 2)
+    live = (img0[:, 0] != EMPTY[bits]) & (img0[:, 0] != TOMB[bits])
+    home = O.fibhash(img0[:, 0], sp, bits).astype(np.int64) & ((1 << sp) - 1)
+    disp = img0[live & (home != np.arange(1 << sp))].copy()
+    assert disp.shape[0] > 100
+    d = torch.from_numpy(disp.view(np.int32 if bits == 32 else np.int64).copy()).cuda()
+    _call(hm, "reset", C.c_void_p(d.data_ptr()), C.c_void_p(hm.bucket_ptr()), C.c_void_p(hm.info_ptr()),
+          disp.shape[0])
+    model.reset(disp)
+    assert np.array_equal(hm.host_buckets().reshape(-1), model.b)
+    hm.insert(d, targetLF=1.0)
+    model.insert_pairs(disp)
+    img = hm.host_buckets()
+    assert np.array_equal(live_pairs(img, bits), live_pairs(model.b, bits))
+    assert np.array_equal(occupancy(img, bits), occupancy(model.b, bits))
+    sp_, fill, ovf, tomb, err = info_tuple(hm)
+    assert (fill, tomb, err) == (model.info.fill, model.info.tombstones, 0)
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_insert_copy_in_window_wins_over_hole(gpu, bits):
+    """A key whose copy sits behind an EMPTY hole in the same probe window is
+    updated in place, not stored twice (insert_kernel, kernels_AMD.h:339-349)."""
+    from tests.test_oracle import _key_with_home
+
+    sp = 3
+    E = EMPTY[bits]
+    a = _key_with_home(bits, sp, 6)
+    img = np.array([[E, 0], [E, 0], [E, 0], [E, 0], [E, 0], [E, 0], [E, 0], [a, 77]], dtype=NDT[bits])
+    model = model_from(img, (sp, 1, 64, 0, 0), bits)
+    hm = Hashmap(sp, bits, auto_build=0, auto_reorder=0, tune=hn_tune(8, 256, 0, 0))
+    upload(hm, model)
+    hm.insert(np.array([a], dtype=NDT[bits]), np.array([5], dtype=NDT[bits]), targetLF=1.0)
+    exp = img.copy()
+    exp[7, 1] = 5
+    assert np.array_equal(hm.host_buckets(), exp)
+    assert info_tuple(hm)[1] == 1
+
+
 @pytest.mark.parametrize("bits", [32, 64])
 @pytest.mark.parametrize("budget", [0, 1 << 16])
 def test_clean_tombstones(gpu, bits, budget):
@@ -290,3 +416,33 @@ def test_empty_batches(gpu, bits):
     _, v, h = hm.retrieve(np.zeros(0, NDT[bits]))
     hm.erase(np.zeros(0, NDT[bits]))
     assert hm.size() == 0
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_side_stream_growth_rehash_and_compaction(gpu, bits):
+    """Every call on a non-default stream: an insert that grows the map (new
+    table and scratch on that stream), retrieve, device_rehash, clean and
+    compaction read back on it -- results as on the default stream."""
+    from hashinator_amd import tools
+
+    s = torch.cuda.Stream()
+    keys = keyset(5000, bits, seed=91)
+    vals = (keys ^ NDT[bits](3)).astype(NDT[bits])
+    hm = Hashmap(8, bits, auto_build=0)
+    hm.insert(keys, vals, targetLF=0.5, stream=s)  # 2^8 -> 2^14 slots
+    assert hm.getSizePower() == 14 and hm.size() == keys.size
+    _, v, h = hm.retrieve(keys, stream=s)
+    assert h.cpu().numpy().all() and np.array_equal(to_numpy(v, bits), vals)
+    hm.erase(keys[::3], stream=s)
+    hm.device_rehash(15, stream=s)
+    hm.clean_tombstones(stream=s)
+    live = np.stack([np.delete(keys, np.s_[::3]), np.delete(vals, np.s_[::3])], axis=1)
+    live = live[np.argsort(live[:, 0])]
+    assert np.array_equal(live_pairs(hm.host_buckets(), bits), live)
+    got = to_numpy(hm.extractPattern(stream=s), bits).reshape(-1, 2)
+    assert np.array_equal(got[np.argsort(got[:, 0])], live)
+    flags = torch.from_numpy((np.arange(hm.bucket_count()) % 3 == 0).astype(np.uint8)).cuda()
+    torch.cuda.synchronize()  # the flags are copied on the default stream
+    out = tools.copy_if_flags(hm.buckets, flags, stream=s)
+    snap = hm.host_buckets()
+    assert np.array_equal(to_numpy(out, bits).reshape(-1, 2), snap[::3])

@@ -104,3 +104,26 @@ def test_algorithmic_bytes_model():
     assert abs(bench.alg_bytes_insert(1, 0.9) - 216) < 1e-9
     n = int(0.9 * (1 << 28))
     assert abs(bench.alg_bytes_retrieve(n, n, 0.9) / 1e9 - 247.39) < 0.01
+
+
+def test_output_tensors_are_checked_before_a_launch():
+    """retrieve's caller-provided vals/hit are rejected unless they are device
+    tensors of the right dtype, contiguous and long enough."""
+    import torch
+
+    from hashinator_amd.hashmap import out_tensor
+
+    dev = "cuda:0"
+    with pytest.raises(ValueError, match="is on cpu"):
+        out_tensor(torch.zeros(8, dtype=torch.int64), 8, torch.int64, dev, "vals")
+    with pytest.raises(TypeError, match="must be torch.int64"):
+        out_tensor(torch.zeros(8, dtype=torch.int32), 8, torch.int64, dev, "vals")
+    with pytest.raises(TypeError, match="must be a torch tensor"):
+        out_tensor(np.zeros(8, dtype=np.uint64), 8, torch.int64, dev, "vals")
+    # on the CPU the remaining checks are reached with a CPU "map device"
+    with pytest.raises(ValueError, match="contiguous"):
+        out_tensor(torch.zeros(16, dtype=torch.uint8)[::2], 8, torch.uint8, "cpu", "hit")
+    with pytest.raises(ValueError, match="8 needed"):
+        out_tensor(torch.zeros(7, dtype=torch.uint8), 8, torch.uint8, "cpu", "hit")
+    t = torch.zeros(9, dtype=torch.uint64)
+    assert out_tensor(t, 8, torch.int64, "cpu", "vals").dtype == torch.int64

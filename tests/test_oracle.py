@@ -178,6 +178,143 @@ def test_gpu_model_clean_and_rehash_postconditions(bits):
     assert reachable_all(m.b, bits, sp + 1)
 
 
+# ------------------------------------------------------------------ clear(device) and reset_to_empty
+def _tombstoned_model(bits, sp=10, seed=61):
+    """A table with live keys, TOMBSTONEs, non-zero values and an overflow above 64."""
+    keys = W.unique_keys(int(0.95 * (1 << sp)), bits, seed)
+    m = O.GpuModel(sp, bits)
+    m.insert(keys, (keys ^ NDT[bits](0x5A5A)).astype(NDT[bits]))
+    m.erase(keys[::5])
+    assert m.info.tombstones > 0 and m.info.overflow > 64
+    m.info.err = 0
+    return m, keys
+
+
+def reset_all_closed_form(b0, info0, n_len, bits):
+    """kernels_AMD.h:26-43 as the closed form: slots [0, len) get key EMPTY,
+    every value stays, fill = 0; sizePower, overflow, tombstones and err stay.
+    len == 0 launches no block (hashers.h:155-163), so nothing changes."""
+    img = np.asarray(b0).reshape(-1, 2).copy()
+    t = info0.as_tuple()
+    if n_len == 0:
+        return img, t
+    img[:n_len, 0] = EMPTY[bits]
+    return img, (t[0], 0, t[2], t[3], t[4])
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_reset_all_closed_form(bits):
+    m0, _ = _tombstoned_model(bits)
+    n = 1 << m0.info.size_power
+    for n_len in (0, 1, 333, n - 1, n):
+        m = O.GpuModel(m0.info.size_power, bits, buckets=m0.b, info=m0.info)
+        m.reset_all(n_len)
+        exp_img, exp_info = reset_all_closed_form(m0.b, m0.info, n_len, bits)
+        assert np.array_equal(m.b.reshape(-1, 2), exp_img), n_len
+        assert m.info.as_tuple() == exp_info, n_len
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_reset_all_check_catches_planted_mistakes(bits):
+    """clear(host) semantics (hashinator.h:494-498: fresh buckets {EMPTY, V()}
+    and Info(sizePower)) and other near misses fail the closed form."""
+    m0, _ = _tombstoned_model(bits)
+    n = 1 << m0.info.size_power
+    m = O.GpuModel(m0.info.size_power, bits, buckets=m0.b, info=m0.info)
+    m.reset_all(n)
+    exp_img, exp_info = reset_all_closed_form(m0.b, m0.info, n, bits)
+    got_img, got_info = m.b.reshape(-1, 2), m.info.as_tuple()
+    assert np.array_equal(got_img, exp_img) and got_info == exp_info
+    # tombstone counter reset, as clear(host) does
+    assert (got_info[0], 0, got_info[2], 0, got_info[4]) != exp_info
+    # overflow back to BUCKET_OVERFLOW, as Info(sizePower) has it
+    assert (got_info[0], 0, 64, got_info[3], got_info[4]) != exp_info
+    # values reset to V()
+    bad = got_img.copy()
+    bad[:, 1] = 0
+    assert not np.array_equal(bad, exp_img)
+    # only live keys reset (TOMBSTONE slots left as they were)
+    bad = got_img.copy()
+    tomb = np.asarray(m0.b).reshape(-1, 2)[:, 0] == TOMB[bits]
+    bad[tomb, 0] = TOMB[bits]
+    assert tomb.any() and not np.array_equal(bad, exp_img)
+    # an off-by-one in the slot range
+    m1 = O.GpuModel(m0.info.size_power, bits, buckets=m0.b, info=m0.info)
+    m1.reset_all(334)
+    assert not np.array_equal(m1.b.reshape(-1, 2), reset_all_closed_form(m0.b, m0.info, 333, bits)[0])
+
+
+def _key_with_home(bits, sp, home, skip=0):
+    """The smallest keys (after `skip` of them) whose home slot is `home`."""
+    k = 0
+    found = 0
+    while True:
+        if int(O.fibhash(np.array([k], dtype=NDT[bits]), sp, bits)[0]) & ((1 << sp) - 1) == home:
+            if found == skip:
+                return NDT[bits](k)
+            found += 1
+        k += 1
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_reset_known_answers(bits):
+    """Hand-built 8-slot images with the outcome read off kernels_AMD.h:65-134:
+    the first copy in probe order from the home slot is emptied, across EMPTY
+    slots and around the end of the table; the value stays; fill -= len."""
+    sp = 3
+    E = EMPTY[bits]
+    a = _key_with_home(bits, sp, 6)
+    b = _key_with_home(bits, sp, 2)
+    c = _key_with_home(bits, sp, 6, skip=1)  # never stored: an absent key
+    img = np.array([[E, 0], [a, 11], [b, 22], [E, 0], [E, 0], [E, 0], [E, 0], [a, 77]], dtype=NDT[bits])
+    # a's home is slot 6 (EMPTY); probing 6, 7, 0, 1, ... finds the copy in slot 7 first
+    info = O.Info(sp, 0, 3, 64, 0, 0, 0)
+    m = O.GpuModel(sp, bits, buckets=img.reshape(-1), info=info)
+    m.reset(np.array([[a, 999]], dtype=NDT[bits]))
+    exp = img.copy()
+    exp[7, 0] = E  # slot 7, not slot 1 (the lower index) and not nothing (EMPTY at 6 does not stop)
+    assert np.array_equal(m.b.reshape(-1, 2), exp)
+    assert m.info.fill == 2 and m.info.as_tuple()[2:] == (64, 0, 0)
+    # the second reset of a finds the copy in slot 1, past EMPTY slots 6 and 7 and the wrap
+    m.reset(np.array([[a, 0]], dtype=NDT[bits]))
+    exp[1, 0] = E
+    assert np.array_equal(m.b.reshape(-1, 2), exp) and m.info.fill == 1
+    # an absent key changes no slot but still counts in fill -= len
+    m.reset(np.array([[c, 0], [b, 0]], dtype=NDT[bits]))
+    exp[2, 0] = E
+    assert np.array_equal(m.b.reshape(-1, 2), exp)
+    assert m.info.fill == np.uint64(2**64 - 1)  # 1 - 2 wraps, as size_t does
+    # len == 0 changes nothing
+    before = m.b.copy(), m.info.as_tuple()
+    m.reset(np.empty((0, 2), dtype=NDT[bits]))
+    assert np.array_equal(m.b, before[0]) and m.info.as_tuple() == before[1]
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_reset_set_postconditions(bits):
+    """On a consistent table (each key at most once): exactly the slots of the
+    reset keys that are present become EMPTY, values and every other slot stay,
+    fill drops by len (absent keys and repeats included)."""
+    m0, keys = _tombstoned_model(bits, sp=11, seed=67)
+    img0 = m0.b.reshape(-1, 2).copy()
+    rng = np.random.default_rng(bits)
+    present = img0[(img0[:, 0] != EMPTY[bits]) & (img0[:, 0] != TOMB[bits]), 0]
+    pick = rng.choice(present, present.size // 3, replace=False)
+    absent = W.unique_keys(50, bits, 68, start=10**6)
+    src_keys = np.concatenate([pick, absent, pick[:5]])  # five repeats: found once, counted twice
+    rng.shuffle(src_keys)
+    src = np.stack([src_keys, np.zeros_like(src_keys)], axis=1)
+    m = O.GpuModel(m0.info.size_power, bits, buckets=m0.b, info=m0.info)
+    m.reset(src)
+    img = m.b.reshape(-1, 2)
+    changed = np.nonzero(img[:, 0] != img0[:, 0])[0]
+    assert np.array_equal(np.sort(img0[changed, 0]), np.sort(pick))
+    assert np.all(img[changed, 0] == EMPTY[bits])
+    assert np.array_equal(img[:, 1], img0[:, 1])
+    assert m.info.fill == m0.info.fill - src_keys.size
+    assert m.info.as_tuple()[2:] == m0.info.as_tuple()[2:]
+
+
 def test_checks_catch_planted_mistakes():
     """The invariants used on the GPU results fail on a broken image."""
     bits, sp = 64, 10
