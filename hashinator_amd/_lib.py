@@ -122,6 +122,7 @@ SIGNATURES = {
     "hn_compact_scratch_bytes": (S, [S]),
     "hn_copy_if_pairs_u32": (I, [P, S, PRED, P, P, P, S, P]),
     "hn_copy_if_pairs_u64": (I, [P, S, PRED, P, P, P, S, P]),
+    "hn_copy_if_pairs_noalloc_u64": (I, [P, S, PRED, P, P, P]),
     "hn_copy_keys_if_pairs_u32": (I, [P, S, PRED, P, P, P, S, P]),
     "hn_copy_keys_if_pairs_u64": (I, [P, S, PRED, P, P, P, S, P]),
     "hn_copy_if_flags": (I, [P, P, S, S, P, P, P, S, P]),

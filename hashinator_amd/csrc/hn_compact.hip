@@ -13,7 +13,9 @@
 // ranks follow input order.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
+#include <mutex>
 
 #include "hn_device.hpp"
 #include "hn_host.hpp"
@@ -44,10 +46,10 @@ struct Lookback {
    uint64_t* hi; // [tiles]
 };
 
-inline size_t lookback_bytes(uint64_t tiles) { return 256 + tiles * 16 + 64; }
+constexpr size_t lookback_bytes(uint64_t tiles) { return 256 + tiles * 16 + 64; }
 
-inline Lookback lookback_layout(void* scratch, uint64_t tiles) {
-   char* p = static_cast<char*>(scratch);
+inline Lookback lookback_layout(void* scr, uint64_t tiles) {
+   char* p = static_cast<char*>(scr);
    Lookback L;
    L.ticket = reinterpret_cast<uint32_t*>(p);
    L.lo = reinterpret_cast<uint64_t*>(p + 256);
@@ -307,10 +309,10 @@ __global__ __launch_bounds__(kBS) void k_scan(const T* in, T* out, uint64_t n, L
 
 // ------------------------------------------------------------ host side
 
-inline uint64_t compact_tiles(uint64_t n) { return n ? (n + kCompactTile - 1) / kCompactTile : 0; }
+constexpr uint64_t compact_tiles(uint64_t n) { return n ? (n + kCompactTile - 1) / kCompactTile : 0; }
 
 template <typename In, typename Out, class Sel, class Proj>
-int run_compact(const In* in, uint64_t n, Out* out, uint64_t* d_count, void* scratch, size_t bytes, hipStream_t s,
+int run_compact(const In* in, uint64_t n, Out* out, uint64_t* d_count, void* scr, size_t bytes, hipStream_t s,
                 Sel sel, Proj proj) {
    if ((n && (!in || !out)) || !d_count) return HN_EINVAL;
    if (n == 0) {
@@ -318,8 +320,8 @@ int run_compact(const In* in, uint64_t n, Out* out, uint64_t* d_count, void* scr
       return static_cast<int>(hipGetLastError());
    }
    const uint64_t tiles = compact_tiles(n);
-   if (!scratch || bytes < lookback_bytes(tiles)) return HN_ESCRATCH;
-   Lookback L = lookback_layout(scratch, tiles);
+   if (!scr || bytes < lookback_bytes(tiles)) return HN_ESCRATCH;
+   Lookback L = lookback_layout(scr, tiles);
    hipLaunchKernelGGL(k_lookback_init, dim3(stream_blocks(tiles, 256)), dim3(256), 0, s, L, tiles);
    hipLaunchKernelGGL((k_compact<In, Out, Sel, Proj>), dim3(static_cast<uint32_t>(tiles)), dim3(kBS), 0, s, in, n,
                       out, d_count, L, tiles, sel, proj);
@@ -327,13 +329,13 @@ int run_compact(const In* in, uint64_t n, Out* out, uint64_t* d_count, void* scr
 }
 
 template <typename T>
-int run_scan(const T* in, T* out, uint64_t n, void* scratch, size_t bytes, hipStream_t s) {
+int run_scan(const T* in, T* out, uint64_t n, void* scr, size_t bytes, hipStream_t s) {
    if (n && (!in || !out)) return HN_EINVAL;
    if (n == 0) return 0;
    constexpr uint64_t TILE = static_cast<uint64_t>(kBS) * (kScanBytes / sizeof(T));
    const uint64_t tiles = (n + TILE - 1) / TILE;
-   if (!scratch || bytes < lookback_bytes(tiles)) return HN_ESCRATCH;
-   Lookback L = lookback_layout(scratch, tiles);
+   if (!scr || bytes < lookback_bytes(tiles)) return HN_ESCRATCH;
+   Lookback L = lookback_layout(scr, tiles);
    hipLaunchKernelGGL(k_lookback_init, dim3(stream_blocks(tiles, 256)), dim3(256), 0, s, L, tiles);
    const bool vec = (reinterpret_cast<uintptr_t>(in) % 16 == 0) && (reinterpret_cast<uintptr_t>(out) % 16 == 0);
    if (vec)
@@ -341,6 +343,51 @@ int run_scan(const T* in, T* out, uint64_t n, void* scratch, size_t bytes, hipSt
    else
       hipLaunchKernelGGL((k_scan<T, false>), dim3(static_cast<uint32_t>(tiles)), dim3(kBS), 0, s, in, out, n, L, tiles);
    return static_cast<int>(hipGetLastError());
+}
+
+// ------------------------------------------------------------ compaction without allocation
+//
+// hn_copy_if_pairs_noalloc_u64 keeps its look-back state (ticket and tile
+// words) in `scratch`, a file-scope device array sized for the largest
+// input it accepts, HN_NOALLOC_MAX_ELEMS elements (a 2^32-slot table):
+// 2^19 + 1 tiles of 16 bytes, 8 MiB in all, held by every device the
+// library runs on until the process ends.  Host code never names the array
+// in a launch or a copy: its host-side name is not a device address.  The
+// device address is looked up once per device (with that device current)
+// and kept in noalloc_ptr[].
+constexpr uint64_t kNoAllocMaxElems = HN_NOALLOC_MAX_ELEMS;
+constexpr size_t kNoAllocBytes = lookback_bytes(compact_tiles(kNoAllocMaxElems) + 1);
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::atomic<void*> noalloc_ptr[kMaxDevices];
+std::mutex noalloc_mu;
+
+// The current device's address of `scratch`, or a null pointer with *rc set.
+void* noalloc_scratch(int* rc) {
+   int dev = 0;
+   hipError_t e = hipGetDevice(&dev);
+   if (e != hipSuccess) {
+      *rc = static_cast<int>(e);
+      return nullptr;
+   }
+   if (dev < 0 || dev >= kMaxDevices) {
+      *rc = HN_EINVAL;
+      return nullptr;
+   }
+   void* p = noalloc_ptr[dev].load(std::memory_order_acquire);
+   if (p) return p;
+   std::lock_guard<std::mutex> lock(noalloc_mu);
+   p = noalloc_ptr[dev].load(std::memory_order_acquire);
+   if (!p) {
+      e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !p) {
+         *rc = e != hipSuccess ? static_cast<int>(e) : HN_ENOLIB;
+         return nullptr;
+      }
+      noalloc_ptr[dev].store(p, std::memory_order_release);
+   }
+   return p;
 }
 
 } // namespace hn
@@ -365,6 +412,15 @@ int hn_copy_if_pairs_u64(const void* in, size_t n, const hn_pred* p, void* out, 
    if (!p) return HN_EINVAL;
    return run_compact(static_cast<const Slot<uint64_t>*>(in), n, static_cast<Slot<uint64_t>*>(out), c, sc, sb, s,
                       PairSel<uint64_t>{*p}, Whole<Slot<uint64_t>>{});
+}
+int hn_copy_if_pairs_noalloc_u64(const void* in, size_t n, const hn_pred* p, void* out, uint64_t* c, hn_stream_t s) {
+   if (!p) return HN_EINVAL;
+   if (n > kNoAllocMaxElems) return HN_ESCRATCH;
+   int rc = 0;
+   void* sc = noalloc_scratch(&rc);
+   if (!sc) return rc;
+   return run_compact(static_cast<const Slot<uint64_t>*>(in), n, static_cast<Slot<uint64_t>*>(out), c, sc,
+                      kNoAllocBytes, s, PairSel<uint64_t>{*p}, Whole<Slot<uint64_t>>{});
 }
 int hn_copy_keys_if_pairs_u32(const void* in, size_t n, const hn_pred* p, uint32_t* out, uint64_t* c, void* sc,
                               size_t sb, hn_stream_t s) {

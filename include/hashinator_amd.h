@@ -322,6 +322,19 @@ HN_API int hn_copy_keys_if_pairs_u32(const void* in, size_t n, const hn_pred* pr
 HN_API int hn_copy_keys_if_pairs_u64(const void* in, size_t n, const hn_pred* pred, uint64_t* out, uint64_t* d_count,
                               void* scratch, size_t scratch_bytes, hn_stream_t s);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  hn_copy_if_pairs_u64 without the scratch argument: the
+ * look-back state lives in a static device array of the library (8 MiB per
+ * device, held until the process ends; no hipMalloc, hipFree or allocating
+ * library call).  Accepts n <= HN_NOALLOC_MAX_ELEMS (else HN_ESCRATCH, before
+ * any launch).  Every call on a device shares that device's array: calls that
+ * overlap on one device (other streams, other threads) race and give wrong
+ * results; order them.  The state is re-initialised by each call. */
+#define HN_NOALLOC_MAX_ELEMS (1ull << 32)
+HN_API int hn_copy_if_pairs_noalloc_u64(const void* in, size_t n, const hn_pred* pred, void* out, uint64_t* d_count,
+                                        hn_stream_t s);
+
 /* copy_if with the rule already evaluated into flags (one byte per element,
  * nonzero = keep): the path the header shim takes for user functors.
  * elem_bytes must be 4, 8 or 16. */
