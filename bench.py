@@ -692,6 +692,15 @@ def run_serving(torch, rates=(1000.0, 4000.0), n_requests=300):
         rep = latency_report(reqs, wall)
         rep.update({"rate_hz": rate, "rehashes": eng.rehashes, "launches": eng.launches,
                     "merged_requests": n_requests - eng.launches})
+        # every retrieve and every final fill against a sequential host replay (not timed)
+        t2 = time.perf_counter()
+        infos = eng.infos()
+        replay = Wl.ServingReplay([(k.size, 7000 + m, 1) for m, k in enumerate(pre)])
+        bad, checked = replay.check(reqs, infos[:, 1])
+        bad += [f"map {m}: err {int(e)}" for m, e in enumerate(infos[:, 4] & 0xFFFFFFFF) if e]
+        rep.update({"checked": not bad, "checked_keys": checked, "check_s": round(time.perf_counter() - t2, 1)})
+        if bad:
+            rep["mismatches"] = bad[:5]
         out["runs"].append(rep)
         del eng
         torch.cuda.empty_cache()

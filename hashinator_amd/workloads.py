@@ -71,8 +71,10 @@ def unique_keys(n, bits, seed, start=0):
     else:
         off = mix64(np.uint64(seed))
         with np.errstate(over="ignore"):
-            k = mix64(idx + off)
-        k = k[k < np.uint64(0xFFFFFFFFFFFFFFFE)]
+            idx += off
+        k = mix64(idx)
+        if (k >= np.uint64(0xFFFFFFFFFFFFFFFE)).any():
+            k = k[k < np.uint64(0xFFFFFFFFFFFFFFFE)]
     return k[:n]
 
 
@@ -173,3 +175,138 @@ def batched_workload(n_maps=4096, sp_lo=8, sp_hi=16, lf=0.6, grid=200, sigma=20.
         qry.append(q)
         ers.append(e.astype(np.uint32))
     return sps, ins, vals, qry, ers
+
+
+_MIX_C1 = 0xBF58476D1CE4E5B9
+_MIX_C2 = 0x94D049BB133111EB
+
+
+def inv_mix64(y):
+    """The inverse of mix64: each xor-shift and odd multiply undone in reverse order."""
+    y = _u64(y).copy()
+    i1 = np.uint64(pow(_MIX_C1, -1, 1 << 64))
+    i2 = np.uint64(pow(_MIX_C2, -1, 1 << 64))
+    with np.errstate(over="ignore"):
+        y ^= (y >> np.uint64(31)) ^ (y >> np.uint64(62))
+        y *= i2
+        y ^= (y >> np.uint64(27)) ^ (y >> np.uint64(54))
+        y *= i1
+        y ^= (y >> np.uint64(30)) ^ (y >> np.uint64(60))
+    return y
+
+
+def in_unique_keys64(keys, n, seed):
+    """Membership of `keys` in unique_keys(n, 64, seed) (start 0), without
+    generating the set: the key's index under the inverse bijection is below
+    n, plus one for each sentinel the generator skipped before that point."""
+    off = mix64(np.uint64(seed))
+    with np.errstate(over="ignore"):
+        idx = inv_mix64(keys) - off
+        skipped = np.sort(inv_mix64(np.array([0xFFFFFFFFFFFFFFFE, 0xFFFFFFFFFFFFFFFF], dtype=np.uint64)) - off)
+    cut = n
+    for s in skipped.tolist():
+        if s < cut:
+            cut += 1
+    k = _u64(keys)
+    return (idx < np.uint64(cut)) & (k < np.uint64(0xFFFFFFFFFFFFFFFE))
+
+
+class ServingReplay:
+    """Sequential replay, in set semantics, of a serving request stream over
+    u64 maps, for checking the loop's results at full size.
+
+    Map m starts with the keys unique_keys(n0, 64, key_seed) holding
+    values(key, value_seed); that set is never materialised (membership by
+    in_unique_keys64).  What the requests change is kept per map as sorted
+    arrays: the key, its last inserted value and whether it is live."""
+
+    def __init__(self, preloads):
+        """preloads: one (n0, key_seed, value_seed) per map."""
+        self.pre = [(int(n), int(ks), int(vs)) for n, ks, vs in preloads]
+        self.dk = [np.empty(0, np.uint64) for _ in self.pre]
+        self.dv = [np.empty(0, np.uint64) for _ in self.pre]
+        self.live = [np.empty(0, bool) for _ in self.pre]
+
+    def preloaded(self, m, keys):
+        n, ks, _ = self.pre[m]
+        return in_unique_keys64(keys, n, ks)
+
+    def _find(self, m, skeys):
+        """Positions of sorted keys in the map's change arrays, and which are there."""
+        dk = self.dk[m]
+        idx = np.searchsorted(dk, skeys)
+        ok = idx < dk.size
+        found = np.zeros(skeys.size, dtype=bool)
+        found[ok] = dk[idx[ok]] == skeys[ok]
+        return found, idx
+
+    def _add(self, m, skeys, vals, idx, live):
+        """Insert sorted keys absent from the change arrays at their positions."""
+        self.dk[m] = np.insert(self.dk[m], idx, skeys)
+        self.dv[m] = np.insert(self.dv[m], idx, vals)
+        self.live[m] = np.insert(self.live[m], idx, np.full(skeys.size, live))
+
+    def insert(self, m, keys, vals):
+        o = np.argsort(_u64(keys))
+        sk, sv = _u64(keys)[o], _u64(vals)[o]
+        found, idx = self._find(m, sk)
+        self.dv[m][idx[found]] = sv[found]
+        self.live[m][idx[found]] = True
+        self._add(m, sk[~found], sv[~found], idx[~found], True)
+
+    def erase(self, m, keys):
+        sk = np.sort(_u64(keys))
+        found, idx = self._find(m, sk)
+        self.live[m][idx[found]] = False
+        new = ~found
+        new[new] = self.preloaded(m, sk[new])
+        self._add(m, sk[new], np.zeros(int(new.sum()), np.uint64), idx[new], False)
+
+    def retrieve(self, m, keys):
+        """(values, hit) the map gives for `keys` now; values where hit."""
+        keys = _u64(keys)
+        o = np.argsort(keys)
+        sk = keys[o]
+        found, idx = self._find(m, sk)
+        hit_s = np.empty(sk.size, dtype=bool)
+        val_s = np.empty(sk.size, dtype=np.uint64)
+        hit_s[found] = self.live[m][idx[found]]
+        val_s[found] = self.dv[m][idx[found]]
+        rest = ~found
+        hit_s[rest] = self.preloaded(m, sk[rest])
+        val_s[rest] = values(sk[rest], self.pre[m][2])
+        hit = np.empty_like(hit_s)
+        val = np.empty_like(val_s)
+        hit[o] = hit_s
+        val[o] = val_s
+        return val, hit
+
+    def check(self, requests, fills):
+        """Replay served `requests` (objects with rid, map_id, kind, keys, vals,
+        arrival, and out_vals/out_hit for retrieves) in arrival order and
+        compare every retrieve's hit flags and hit values, then each map's
+        final fill against `fills`.  Returns the mismatches found (empty when
+        everything agrees) and the number of retrieved keys checked."""
+        bad, checked = [], 0
+        for r in sorted(requests, key=lambda r: r.arrival):
+            if r.kind == "insert":
+                self.insert(r.map_id, r.keys, r.vals)
+            elif r.kind == "erase":
+                self.erase(r.map_id, r.keys)
+            else:
+                ev, eh = self.retrieve(r.map_id, r.keys)
+                h = np.asarray(r.out_hit, dtype=bool)
+                v = np.asarray(r.out_vals).view(np.uint64)
+                checked += r.keys.size
+                if not np.array_equal(h, eh):
+                    bad.append(f"request {r.rid}: {int(np.count_nonzero(h != eh))} hit flags differ")
+                elif not np.array_equal(v[h], ev[eh]):
+                    bad.append(f"request {r.rid}: {int(np.count_nonzero(v[h] != ev[eh]))} values differ")
+        for m in range(len(self.pre)):
+            if int(fills[m]) != self.fill(m):
+                bad.append(f"map {m}: fill {int(fills[m])}, replay {self.fill(m)}")
+        return bad, checked
+
+    def fill(self, m):
+        pre = self.preloaded(m, self.dk[m])
+        return self.pre[m][0] + int(np.count_nonzero(self.live[m] & ~pre)) - int(np.count_nonzero(~self.live[m] & pre))

@@ -105,3 +105,29 @@ def test_fullsize_extract_pattern_and_scan(big):
     ones = torch.ones(1 << SP, dtype=torch.int32, device="cuda")
     ramp = tools.split_prefix_scan(ones)
     assert torch.equal(ramp, torch.arange(1 << SP, dtype=torch.int32, device="cuda"))
+
+
+@pytest.mark.parametrize("lf,build", [(0.5, True), (0.75, True), (0.5, False), (0.75, False)])
+def test_fullsize_other_load_factors(gpu, lf, build):
+    """2^28 slots at load factors 0.5 and 0.75, through the bulk build and
+    through the probing insert (+ reorder): every present key retrieves its
+    value, no absent key touches its output, Info is exact."""
+    n = int(lf * (1 << SP))
+    seed = W.SEED_BASE + 2
+    keys = W.mixed64(n, seed)
+    dk = torch.from_numpy(keys.view(np.int64)).cuda()
+    dv = torch.from_numpy(W.values(keys, seed).view(np.int64)).cuda()
+    da = torch.from_numpy(W.absent64(n, seed, n).view(np.int64)).cuda()
+    del keys
+    hm = Hashmap(SP, 64, auto_build=1.0 / 16 if build else 0)
+    hm.insert(dk, dv, targetLF=1.0)
+    i = hm.info()
+    assert (hm.getSizePower(), i.fill, i.tombstone_counter, i.err) == (SP, n, 0, 0)
+    assert hm.ordered()  # the build writes Robin Hood order; the probing path reorders after a large insert
+    g = torch.Generator(device="cuda").manual_seed(int(lf * 100))
+    perm = torch.randperm(2 * n, device="cuda", generator=g)
+    q = torch.cat([dk, da])[perm]
+    expect = torch.cat([dv, torch.full((n,), MISS, dtype=torch.int64, device="cuda")])[perm]
+    check_retrieve(hm, q, expect, perm < n)
+    del hm, q, expect, perm, dk, dv, da
+    torch.cuda.empty_cache()

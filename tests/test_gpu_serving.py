@@ -80,3 +80,30 @@ def test_mix_equals_alone_equals_oracle(gpu, realtime, budget, bulk_rehash_min):
         assert i1[m][1] == i2[m][1] == models[m].info.fill
         assert (i1[m][4] & 0xFFFFFFFF) == 0
     assert e1.rehashes >= 1  # the stream crosses the 0.85 ceiling somewhere
+
+
+def test_serving_full_size(gpu):
+    """The serving workload at full size: 64 resident u64 maps of 2^20..2^26
+    slots preloaded to LF 0.4 (500 M keys), 300 Poisson requests of 1k..4M
+    keys at 4,000 requests/s with growth at LF 0.85.  Every retrieve's hit
+    flags and values and every map's final fill equal a sequential host
+    replay of the stream in set semantics (workloads.ServingReplay)."""
+    import torch
+
+    rng = np.random.default_rng(5)
+    sps = [int(x) for x in rng.integers(20, 27, size=64)]
+    pre = [W.unique_keys(int(0.4 * (1 << sp)), 64, 7000 + m) for m, sp in enumerate(sps)]
+    eng = ServingEngine(sps, key_bits=64, n_streams=8, hbm_budget_bytes=64 << 30)
+    for m, k in enumerate(pre):
+        eng.preload(m, k, W.values(k, 1))
+    torch.cuda.synchronize()
+    reqs = make_requests(sps, 300, rate_hz=4000.0, seed=4000, live_keys=[[k] for k in pre])
+    eng.serve(reqs, realtime=True)
+    infos = eng.infos()
+    assert eng.rehashes >= 1
+    assert not np.any(infos[:, 4] & 0xFFFFFFFF)  # err == SUCCESS on every map
+    del pre
+    rp = W.ServingReplay([(int(0.4 * (1 << sp)), 7000 + m, 1) for m, sp in enumerate(sps)])
+    bad, checked = rp.check(reqs, infos[:, 1])
+    assert not bad, bad[:10]
+    assert checked > 10_000_000
