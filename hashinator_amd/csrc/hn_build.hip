@@ -38,11 +38,13 @@
 //
 // The table's memory is the output of the first partition pass when there
 // are two, so the scratch holds one pair array, not two.  A region holding
-// more than kRegionCap keys (only with strongly skewed keys) aborts the
-// build: the table is re-initialised empty, err = INVALID, and the caller
-// inserts with the probing kernel instead.  The same err = INVALID, with the
-// table untouched, answers a table that is not empty (fill or tombstones
-// nonzero).
+// more than kRegionCap keys (only with strongly skewed keys) abandons the
+// placement on the device: the table is emptied and the probing insert
+// places the partitioned pairs (k_build_fallback_prep, then k_insert), so the
+// call still ends with err = SUCCESS (or FAIL if a key found no slot), the
+// same live set in arrival order, and HN_INFO_ORDERED not set.  A table that
+// is not empty (fill or tombstones nonzero) is answered with err = INVALID
+// and left untouched.
 #include <algorithm>
 
 #include "hn_device.hpp"

@@ -266,8 +266,10 @@ class Hashmap:
 
     def _try_build(self, k, v, n, pairs, stream) -> bool:
         """Bulk path for a large insert into an empty table.  False when it
-        does not apply or the device declined it (err = INVALID: the table is
-        then empty and the probing insert runs instead)."""
+        does not apply or the device declined it (err = INVALID: the table was
+        not empty and is untouched; the probing insert runs instead).  Strongly
+        skewed keys are placed by the device's own probing fallback inside the
+        same call (err = SUCCESS, table not marked ordered)."""
         N = self.bucket_count()
         if not self.auto_build or n < self.auto_build * N or n < self.build_min_keys or n >= N:
             return False
