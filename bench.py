@@ -501,7 +501,29 @@ def run_compaction(torch, L, dev, S, P, buckets, fill):
                                    "GBps_alg": round(alg / (t * 1e-3) / 1e9, 1), "ok": int(cnt.item()) == fill}
     del out, sc
     torch.cuda.empty_cache()
+    res["dropin_copy_if"] = run_dropin_compaction()
     return res
+
+
+def run_dropin_compaction(log2n=28, p=0.3):
+    """The same Bernoulli(0.3) compaction of 2^28 hash_pair<u64,u64> through
+    the drop-in C++ API: split::tools::copy_if(SplitVector&, SplitVector&,
+    rule functor) on device-resident vectors (split_device_allocator), timed
+    as a whole call, next to the raw C-ABI call on the same input
+    (tests/cpp/dropin_compaction.cpp, built by __graft_entry__.build())."""
+    import json as _json
+    import subprocess
+
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "cpp", "_bin", "dropin_compaction")
+    if not os.path.exists(exe):
+        return {"error": "tests/cpp/_bin/dropin_compaction not built"}
+    r = subprocess.run([exe, str(log2n), str(p)], capture_output=True, text=True, timeout=120)
+    try:
+        out = _json.loads(r.stdout.strip().splitlines()[-1])
+    except (ValueError, IndexError):
+        return {"error": f"rc {r.returncode}: {r.stderr[-300:]}"}
+    out["ok"] = r.returncode == 0 and bool(out.get("same_as_abi"))
+    return out
 
 
 def run_small_map(torch, L, dev, S, P, with_cpu=True, reps=20):

@@ -714,8 +714,8 @@ public:
    void streamAttach(hipStream_t, uint32_t = hipMemAttachSingle) {}
 
    /* ------------------------------------------------------------ extraction (hashinator.h:1154-1259) */
-   template <bool prefetches = true, typename Rule>
-   size_t extractPattern(split::SplitVector<pair_t>& elements, Rule rule, hipStream_t s = 0) {
+   template <bool prefetches = true, typename Rule, class A>
+   size_t extractPattern(split::SplitVector<pair_t, A>& elements, Rule rule, hipStream_t s = 0) {
       elements.resize(_info.fill + 1, true);
       split::tools::splitStackArena mPool(split::tools::estimateMemoryForCompaction(_nbuckets), s);
       const size_t len = split::tools::copy_if_raw(_buckets, elements.data(), _nbuckets, rule, mPool, s);
@@ -727,16 +727,16 @@ public:
       split::tools::splitStackArena mPool(split::tools::estimateMemoryForCompaction(_nbuckets), s);
       return split::tools::copy_if_raw(_buckets, elements, _nbuckets, rule, mPool, s);
    }
-   template <bool prefetches = true, typename Rule>
-   size_t extractKeysByPattern(split::SplitVector<KEY_TYPE>& elements, Rule rule, hipStream_t s = 0) {
+   template <bool prefetches = true, typename Rule, class A>
+   size_t extractKeysByPattern(split::SplitVector<KEY_TYPE, A>& elements, Rule rule, hipStream_t s = 0) {
       elements.resize(_info.fill + 1, true);
       split::tools::splitStackArena mPool(split::tools::estimateMemoryForCompaction(_nbuckets), s);
       const size_t len = split::tools::copy_keys_if_raw(_buckets, elements.data(), _nbuckets, rule, mPool, s);
       elements.erase(elements.data() + len, elements.end());
       return elements.size();
    }
-   template <bool prefetches = true, typename Rule>
-   size_t extractKeysByPattern(split::SplitVector<KEY_TYPE>& elements, Rule rule, void* stack, size_t max_size,
+   template <bool prefetches = true, typename Rule, class A>
+   size_t extractKeysByPattern(split::SplitVector<KEY_TYPE, A>& elements, Rule rule, void* stack, size_t max_size,
                                hipStream_t s = 0) {
       elements.resize(_info.fill + 1, true);
       split::tools::splitStackArena mPool(stack, max_size);
@@ -745,8 +745,8 @@ public:
       return elements.size();
    }
    /* All live keys, through the library's built-in predicate (no rule staging). */
-   template <bool prefetches = true>
-   size_t extractAllKeys(split::SplitVector<KEY_TYPE>& elements, hipStream_t s = 0) {
+   template <bool prefetches = true, class A>
+   size_t extractAllKeys(split::SplitVector<KEY_TYPE, A>& elements, hipStream_t s = 0) {
       elements.resize(_info.fill + 1, true);
       const size_t sb = hn_compact_scratch_bytes(_nbuckets);
       split::tools::splitStackArena mPool(sb + 512, s);
@@ -761,8 +761,8 @@ public:
       elements.erase(elements.data() + len, elements.end());
       return elements.size();
    }
-   template <bool prefetches = true>
-   size_t extractAllKeys(split::SplitVector<KEY_TYPE>& elements, void* stack, size_t max_size, hipStream_t s = 0) {
+   template <bool prefetches = true, class A>
+   size_t extractAllKeys(split::SplitVector<KEY_TYPE, A>& elements, void* stack, size_t max_size, hipStream_t s = 0) {
       auto rule = [] __host__ __device__(const pair_t& kval) -> bool {
          return kval.first != EMPTYBUCKET && kval.first != TOMBSTONE;
       };
@@ -771,23 +771,27 @@ public:
    /* The *Loop variants (hashinator.h:1186-1259): the reference runs them as
     * one block looping over the buckets (split::tools::copy_if_loop) and
     * resizes `elements` on the device.  Same elements and order here, through
-    * the single-pass compaction; `elements`' size is a host member, so the
-    * call ends with its stream synchronised. */
-   template <typename Rule>
-   void extractPatternLoop(split::SplitVector<pair_t>& elements, Rule rule, hipStream_t s = 0) {
+    * the single-pass compaction; the size is set from the host after the
+    * count is read, so the call ends with its stream synchronised. */
+   template <typename Rule, class A>
+   void extractPatternLoop(split::SplitVector<pair_t, A>& elements, Rule rule, hipStream_t s = 0) {
       extractPattern(elements, rule, s);
    }
-   void extractLoop(split::SplitVector<pair_t>& elements, hipStream_t s = 0) {
+   template <class A>
+   void extractLoop(split::SplitVector<pair_t, A>& elements, hipStream_t s = 0) {
       auto rule = [] __host__ __device__(const pair_t& kval) -> bool {
          return kval.first != EMPTYBUCKET && kval.first != TOMBSTONE;
       };
       extractPatternLoop(elements, rule, s);
    }
-   template <typename Rule>
-   void extractKeysByPatternLoop(split::SplitVector<KEY_TYPE>& elements, Rule rule, hipStream_t s = 0) {
+   template <typename Rule, class A>
+   void extractKeysByPatternLoop(split::SplitVector<KEY_TYPE, A>& elements, Rule rule, hipStream_t s = 0) {
       extractKeysByPattern(elements, rule, s);
    }
-   void extractAllKeysLoop(split::SplitVector<KEY_TYPE>& elements, hipStream_t s = 0) { extractAllKeys(elements, s); }
+   template <class A>
+   void extractAllKeysLoop(split::SplitVector<KEY_TYPE, A>& elements, hipStream_t s = 0) {
+      extractAllKeys(elements, s);
+   }
 
    /* ------------------------------------------------------------ Robin Hood order (an addition) */
 

@@ -1,12 +1,14 @@
 /* split_allocators.h -- allocators and the abort-on-error check of the
  * MI355X build.  kstppd/hashinator include/splitvector/split_allocators.h:
  * SPLIT_CHECK_ERR (31-50), split_unified_allocator (61-129),
- * split_host_allocator (142-201). */
+ * split_host_allocator (142-201); split_device_allocator is an addition. */
 #pragma once
 #include <cassert>
 #include <cstdlib>
 #include <iostream>
+#include <limits>
 #include <new>
+#include <type_traits>
 #include <utility>
 
 #include "archMacros.h"
@@ -14,12 +16,19 @@
 namespace split {
 
 #define SPLIT_CHECK_ERR(err) (split::hip_error((err), __FILE__, __LINE__))
+/* Any HIP failure ends the program, naming the call site (the reference's
+ * error convention for its containers). */
 inline void hip_error(hipError_t err, const char* file, int line) {
-   if (err != hipSuccess) {
-      std::cerr << "\n\n" << hipGetErrorString(err) << " in " << file << " at line " << line << "\n";
-      abort();
-   }
+   if (err == hipSuccess) return;
+   std::cerr << "split: HIP call failed at " << file << ":" << line << ": " << hipGetErrorName(err) << " ("
+             << hipGetErrorString(err) << ")" << std::endl;
+   abort();
 }
+
+/* Whether an allocator's memory is reachable by kernels only (the host then
+ * moves elements with copies). */
+template <class A>
+struct allocator_is_device_only : std::false_type {};
 
 /* Unified (managed) memory, the reference's default for SplitVector and the
  * map metadata.  Allocations are advised to live on the current device. */
@@ -62,10 +71,7 @@ public:
    static void deallocate(void* p, size_type n) {
       if (n != 0 && p != 0) SPLIT_CHECK_ERR(hipFree(p));
    }
-   size_type max_size() const throw() {
-      size_type max = static_cast<size_type>(-1) / sizeof(value_type);
-      return (max > 0 ? max : 1);
-   }
+   size_type max_size() const throw() { return std::numeric_limits<size_type>::max() / sizeof(value_type); }
    template <typename U, typename... Args>
    void construct(U* p, Args&&... args) {
       ::new (p) U(std::forward<Args>(args)...);
@@ -103,15 +109,53 @@ public:
    }
    void deallocate(pointer p, size_type) { free(p); }
    static void deallocate(void* p, size_type) { free(p); }
-   size_type max_size() const throw() {
-      size_type max = static_cast<size_type>(-1) / sizeof(value_type);
-      return (max > 0 ? max : 1);
-   }
+   size_type max_size() const throw() { return std::numeric_limits<size_type>::max() / sizeof(value_type); }
    template <typename U, typename... Args>
    void construct(U* p, Args&&... args) {
       ::new (p) U(std::forward<Args>(args)...);
    }
    void destroy(pointer p) { p->~value_type(); }
 };
+
+/* Device memory (hipMalloc): the storage kernels reach fastest on a device
+ * whose unified memory is host memory.  The host reaches the elements only
+ * through copies; SplitVector's host members make them itself. */
+template <class T>
+class split_device_allocator {
+public:
+   typedef T value_type;
+   typedef value_type* pointer;
+   typedef const value_type* const_pointer;
+   typedef value_type& reference;
+   typedef const value_type& const_reference;
+   typedef ptrdiff_t difference_type;
+   typedef size_t size_type;
+   template <class U>
+   struct rebind {
+      typedef split_device_allocator<U> other;
+   };
+   split_device_allocator() throw() {}
+   template <class U>
+   split_device_allocator(split_device_allocator<U> const&) throw() {}
+   static void* allocate_raw(size_type bytes, const void* = 0) {
+      void* ret = nullptr;
+      SPLIT_CHECK_ERR(hipMalloc(&ret, bytes));
+      if (ret == nullptr) throw std::bad_alloc();
+      return ret;
+   }
+   pointer allocate(size_type n, const void* = 0) {
+      assert(n && "allocate 0");
+      return static_cast<pointer>(allocate_raw(n * sizeof(value_type)));
+   }
+   void deallocate(pointer p, size_type n) {
+      if (n != 0 && p != 0) SPLIT_CHECK_ERR(hipFree(p));
+   }
+   static void deallocate(void* p, size_type n) {
+      if (n != 0 && p != 0) SPLIT_CHECK_ERR(hipFree(p));
+   }
+   size_type max_size() const throw() { return std::numeric_limits<size_type>::max() / sizeof(value_type); }
+};
+template <class T>
+struct allocator_is_device_only<split_device_allocator<T>> : std::true_type {};
 
 } // namespace split

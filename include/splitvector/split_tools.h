@@ -193,10 +193,10 @@ void copy_keys_if(SplitVector<T, A>& input, SplitVector<U, B>& output, Rule rule
  * the reference backs with one block looping over the input (loop_compact,
  * 718-881) for callers that avoid host round trips.  Same result as copy_if
  * (stable; output's size becomes the count), here through the single-pass
- * kernel.  The reference resizes the output on the device; this SplitVector
- * keeps its size on the host, so the call ends with its stream synchronised.
- * The output grows to the input's size first when it is smaller (the
- * reference requires that capacity and asserts it). */
+ * kernel; the size is set from the host after the count is read, so the call
+ * ends with its stream synchronised.  The output grows to the input's size
+ * first when it is smaller (the reference requires that capacity and
+ * asserts it). */
 template <typename T, typename Rule, class A>
 void copy_if_loop(SplitVector<T, A>& input, SplitVector<T, A>& output, Rule rule, hipStream_t s = 0) {
    if (output.size() < input.size()) output.resize(input.size(), true, s);

@@ -6,6 +6,7 @@
 #include <hashinator/hashinator.h>
 #include <splitvector/split_tools.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <random>
 #include <unordered_map>
@@ -493,6 +494,157 @@ bool splitvector_host_semantics() {
    return true;
 }
 
+// ---------------------------------------------------------------- SplitVector on the device
+// (splitvec.h:482-500 upload, 617-669 / 940-1040 HOSTDEVICE members,
+// 1069-1097 device_push_back, 901-912 device_resize, 1361-1462 device_insert)
+
+template <class Vec>
+__global__ void push_kernel(Vec* v, uint32_t n, uint8_t* ok) {
+   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+   if (i < n) ok[i] = v->device_push_back(static_cast<typename Vec::value_type>(3ull * i + 1)) ? 1 : 0;
+}
+
+// Many threads push at once; against a serial replay of the accepted pushes:
+// the same multiset after the initial elements, the size the reference's
+// rule gives (a push succeeds while size + 1 < capacity), and the host's
+// size() agreeing with the device's.
+template <class Alloc>
+bool splitvector_device_push_back() {
+   using Vec = split::SplitVector<uint64_t, Alloc>;
+   struct Case {
+      size_t init, cap, n;
+   } cases[] = {{0, 100000, 250000}, {0, 1 << 20, 300000}, {1000, 5000, 10000}, {0, 1, 64}, {0, 0, 64}};
+   for (const Case& c : cases) {
+      std::vector<uint64_t> first(c.init);
+      for (size_t i = 0; i < c.init; ++i) first[i] = 7 * i + 5;
+      Vec v(first);
+      if (c.cap) v.reserve(c.cap, true);
+      CHECK(v.capacity() == std::max(c.cap, c.init) && v.size() == c.init);
+      Vec* dv = v.upload();
+      uint8_t* ok = nullptr;
+      SPLIT_CHECK_ERR(hipMalloc(&ok, c.n));
+      hipLaunchKernelGGL(push_kernel<Vec>, dim3((c.n + 255) / 256), dim3(256), 0, 0, dv, static_cast<uint32_t>(c.n), ok);
+      SPLIT_CHECK_ERR(hipGetLastError());
+      SPLIT_CHECK_ERR(hipDeviceSynchronize());
+      std::vector<uint8_t> hok(c.n);
+      SPLIT_CHECK_ERR(hipMemcpy(hok.data(), ok, c.n, hipMemcpyDeviceToHost));
+      SPLIT_CHECK_ERR(hipFree(ok));
+      // serial replay of the accepted pushes
+      std::vector<uint64_t> replay(first);
+      size_t accepted = 0;
+      for (size_t i = 0; i < c.n; ++i)
+         if (hok[i]) {
+            replay.push_back(3ull * i + 1);
+            ++accepted;
+         }
+      const size_t expect = c.cap == 0 ? c.init : std::min(c.init + c.n, c.cap - 1);
+      CHECK(accepted + c.init == expect);
+      CHECK(v.size() == expect);
+      std::vector<uint64_t> got = v.to_host();
+      CHECK(got.size() == expect);
+      CHECK(std::equal(first.begin(), first.end(), got.begin()));
+      std::sort(got.begin() + c.init, got.end());
+      std::sort(replay.begin() + c.init, replay.end());
+      CHECK(got == replay);
+   }
+   return true;
+}
+
+template <class Vec>
+__global__ void script_kernel(Vec* v, long long* st) {
+   using T = typename Vec::value_type;
+   auto it = v->device_insert(v->begin() + 3, T(100));
+   st[0] = (it == v->begin() + 3) && (*it == T(100));
+   it = v->device_insert(v->begin(), 2, T(7));
+   st[1] = (it == v->begin()) && v->size() == 13;
+   const T src[3] = {T(41), T(42), T(43)};
+   it = v->device_insert(v->begin() + 5, src, src + 3);
+   st[2] = (it == v->begin() + 5) && (*it == T(41));
+   st[3] = v->device_resize(v->size() + 2) ? 1 : 0;
+   v->erase(v->begin() + 1);
+   v->erase(v->begin() + 2, v->begin() + 4);
+   v->pop_back();
+   const size_t before = v->size();
+   st[4] = (!v->device_resize(v->capacity() + 1) && v->size() == before) ? 1 : 0;
+   st[5] = (v->device_insert(v->begin(), v->capacity(), T(5)) == v->end() && v->size() == before) ? 1 : 0;
+   st[6] = static_cast<long long>(v->size());
+   // HOSTDEVICE accessors
+   long long sum = 0;
+   for (auto p = v->begin(); p != v->end(); ++p) sum += static_cast<long long>(*p);
+   long long sum2 = 0;
+   for (size_t i = 0; i < v->size(); ++i) sum2 += static_cast<long long>((*v)[i]) + static_cast<long long>(v->at(i));
+   st[7] = sum;
+   st[8] = sum2;
+   st[9] = static_cast<long long>(v->front()) * 1000 + static_cast<long long>(v->back());
+   st[10] = (v->data() == &(*v)[0] && !v->empty() && v->capacity() == 64) ? 1 : 0;
+}
+
+// One device thread runs a script of inserts, resizes and erases; a
+// std::vector runs the same script on the host.
+template <class Alloc>
+bool splitvector_device_script() {
+   using Vec = split::SplitVector<uint32_t, Alloc>;
+   std::vector<uint32_t> init(10);
+   for (uint32_t i = 0; i < 10; ++i) init[i] = i;
+   Vec v(init);
+   v.reserve(64, true);
+   Vec* dv = v.upload();
+   long long* st = nullptr;
+   SPLIT_CHECK_ERR(hipMalloc(&st, 16 * sizeof(long long)));
+   hipLaunchKernelGGL(script_kernel<Vec>, dim3(1), dim3(1), 0, 0, dv, st);
+   SPLIT_CHECK_ERR(hipGetLastError());
+   SPLIT_CHECK_ERR(hipDeviceSynchronize());
+   long long h[16];
+   SPLIT_CHECK_ERR(hipMemcpy(h, st, sizeof(h), hipMemcpyDeviceToHost));
+   SPLIT_CHECK_ERR(hipFree(st));
+   std::vector<uint32_t> e(init);
+   e.insert(e.begin() + 3, 100u);
+   e.insert(e.begin(), 2, 7u);
+   const uint32_t src[3] = {41, 42, 43};
+   e.insert(e.begin() + 5, src, src + 3);
+   e.resize(e.size() + 2);
+   e.erase(e.begin() + 1);
+   e.erase(e.begin() + 2, e.begin() + 4);
+   e.pop_back();
+   for (int i = 0; i < 6; ++i) CHECK(h[i] == 1);
+   CHECK(h[6] == static_cast<long long>(e.size()));
+   CHECK(v.size() == e.size());
+   CHECK(v.to_host() == e);
+   long long sum = 0;
+   for (uint32_t x : e) sum += x;
+   CHECK(h[7] == sum && h[8] == 2 * sum);
+   CHECK(h[9] == static_cast<long long>(e.front()) * 1000 + e.back());
+   CHECK(h[10] == 1);
+   // a host change after upload reaches the device header
+   v.push_back(99);
+   CHECK(v.size() == e.size() + 1);
+   split::SplitInfo info{0, 0};
+   v.copyMetadata(&info);
+   SPLIT_CHECK_ERR(hipDeviceSynchronize());
+   CHECK(info.size == e.size() + 1 && info.capacity == 64);
+   return true;
+}
+
+// The drop-in compaction on device-resident SplitVectors (the stream
+// compaction workload's path through split::tools::copy_if).
+bool splitvector_device_copy_if() {
+   using DVec = split::SplitVector<hash_pair<uint64_t, uint64_t>, split::split_device_allocator<hash_pair<uint64_t, uint64_t>>>;
+   const size_t n = (1 << 21) + 77;
+   std::vector<hash_pair<uint64_t, uint64_t>> h(n);
+   std::mt19937_64 rng(11);
+   for (size_t i = 0; i < n; ++i) h[i] = hash_pair<uint64_t, uint64_t>(i, rng());
+   DVec in(h), out(n);
+   split::tools::copy_if(in, out, IsOdd<uint64_t>());
+   std::vector<hash_pair<uint64_t, uint64_t>> expect;
+   for (auto& e : h)
+      if (e.second & 1) expect.push_back(e);
+   CHECK(out.size() == expect.size());
+   auto got = out.to_host();
+   CHECK(got.size() == expect.size());
+   for (size_t i = 0; i < expect.size(); ++i) CHECK(got[i].first == expect[i].first && got[i].second == expect[i].second);
+   return true;
+}
+
 int main() {
    struct Case {
       const char* name;
@@ -510,6 +662,11 @@ int main() {
        {"thread_api<u64>", [] { return thread_api<uint64_t>(); }},
        {"compaction_and_scan", compaction_and_scan},
        {"splitvector_host_semantics", splitvector_host_semantics},
+       {"splitvector_device_push_back<unified>", splitvector_device_push_back<split::split_unified_allocator<uint64_t>>},
+       {"splitvector_device_push_back<device>", splitvector_device_push_back<split::split_device_allocator<uint64_t>>},
+       {"splitvector_device_script<unified>", splitvector_device_script<split::split_unified_allocator<uint32_t>>},
+       {"splitvector_device_script<device>", splitvector_device_script<split::split_device_allocator<uint32_t>>},
+       {"splitvector_device_copy_if", splitvector_device_copy_if},
    };
    for (auto& c : cases) {
       printf("%s ...\n", c.name);

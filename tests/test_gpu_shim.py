@@ -25,3 +25,14 @@ def test_bucket_residency_results_agree(gpu):
     r = subprocess.run([exe, "18"], capture_output=True, text=True, timeout=300)
     print(r.stdout)
     assert r.returncode == 0 and "residency ok" in r.stdout, r.stdout + r.stderr
+
+
+def test_dropin_compaction_on_device_vectors(gpu):
+    """split::tools::copy_if on device-resident SplitVectors with a user rule
+    gives the raw C-ABI compaction's elements, order and count (2^22 pairs,
+    Bernoulli 0.3, all kept and none kept)."""
+    exe = os.path.join(os.path.dirname(BIN), "dropin_compaction")
+    for args in (["22", "0.3"], ["20", "1.0"], ["20", "0.0"]):
+        r = subprocess.run([exe, *args], capture_output=True, text=True, timeout=120)
+        print(r.stdout, r.stderr[-2000:])
+        assert r.returncode == 0 and '"same_as_abi": true' in r.stdout, args
