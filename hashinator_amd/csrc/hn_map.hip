@@ -29,6 +29,24 @@ size_t rebuild_scratch_bytes(int sp, bool two);
 template <typename K>
 int rebuild(void* buckets, hn_info* info, int sp, void* scratch, size_t bytes, hipStream_t s);
 
+// The segment pass of clean_tombstones (hn_clean.hip) and the reorder that
+// follows the region path when the segment pass hands a table to it
+// (hn_reorder.hip).
+size_t seg_clean_fixed_bytes(int sp);
+template <typename K>
+size_t seg_clean_default_pool(int sp);
+template <typename K>
+size_t seg_clean_full_pool(int sp);
+size_t seg_clean_min_pool();
+const uint32_t* seg_clean_gate(const void* scratch);
+template <typename K>
+int seg_clean(void* buckets, hn_info* info, int sp, void* scratch, void* pool, size_t pool_bytes, bool region_ok,
+              hipStream_t s);
+size_t reorder_scratch_bytes(int sp);
+template <typename K>
+int reorder_gated(void* buckets, hn_info* info, int sp, void* scratch, size_t bytes, const uint32_t* gate,
+                  hipStream_t s);
+
 // insert_kernel / insert_index_kernel (kernels_AMD.h:136-405, 507-641)
 template <typename K, int G, int SRC>
 __global__ __launch_bounds__(1024) void k_insert(InsertArgs<K> a) {
@@ -181,12 +199,13 @@ struct CleanHeader {
 
 constexpr uint64_t kNone = ~0ull;
 
-__global__ void k_clean_begin(const hn_info* info, CleanHeader* h, uint32_t n_regions) {
+// `gate` (nullable): run only when *gate is nonzero (the segment pass's flag).
+__global__ void k_clean_begin(const hn_info* info, CleanHeader* h, uint32_t n_regions, const uint32_t* gate) {
    for (uint32_t r = threadIdx.x; r < n_regions; r += blockDim.x) h->counters[r] = 0;
    if (threadIdx.x == 0) {
       h->start = kNone;
       h->total = 0;
-      h->skip = info->tombstone_counter == 0 ? 1 : 0;
+      h->skip = ((gate && *gate == 0) || info->tombstone_counter == 0) ? 1 : 0;
       h->fail = 0;
       h->no_empty = 0;
    }
@@ -496,17 +515,24 @@ bool clean_plan(uint64_t n, size_t bytes, uint64_t* region_slots, uint64_t* n_re
    return false;
 }
 
+// Scratch of clean_tombstones.  With room for the segment pass (its header
+// and lists, the reorder's, and a pool of at least seg_clean_min_pool()
+// bytes) it takes that path; the pool holds segments too long for LDS, and
+// when it also holds a whole-table region clean, a table with no EMPTY slot
+// is cleaned too.  With less, the region clean alone runs in what there is.
+// budget 0: the segment pass with a pool for long segments covering 1/32 of
+// the table.  budget SIZE_MAX: enough for any table -- a full one, or one
+// segment over the whole table.
 template <typename K>
 size_t clean_scratch_bytes(int sp, size_t budget) {
    if (sp < 1 || sp > max_size_power<K>()) return 0;
    const uint64_t n = 1ull << sp;
-   const size_t full = clean_header_bytes(1) + n * sizeof(Slot<K>);
-   const size_t rb = sp <= 32 ? rebuild_scratch_bytes<K>(sp, false) : 0;
-   const size_t rb2 = sp <= 32 ? rebuild_scratch_bytes<K>(sp, true) : 0;
-   if (budget == 0) return std::max(full, rb2);
-   if (rb2 && budget >= rb2) return rb2;
-   if (rb && budget >= rb) return rb;
-   if (budget >= full) return full;
+   const size_t fixed = seg_clean_fixed_bytes(sp) + reorder_scratch_bytes(sp);
+   const size_t whole = clean_header_bytes(1) + n * sizeof(Slot<K>);
+   const size_t pool = seg_clean_default_pool<K>(sp);
+   if (budget == 0) return fixed + pool;
+   if (budget == SIZE_MAX) return fixed + std::max(std::max(pool, whole), seg_clean_full_pool<K>(sp));
+   if (budget >= fixed + seg_clean_min_pool()) return budget;
    uint64_t rs, nr;
    size_t hb;
    for (size_t b = budget; b >= hn_clean_tombstones_min_scratch(); b /= 2)
@@ -527,12 +553,11 @@ template int launch_insert_pairs_dlen<uint32_t>(const Slot<uint32_t>*, Slot<uint
 template int launch_insert_pairs_dlen<uint64_t>(const Slot<uint64_t>*, Slot<uint64_t>*, hn_info*, uint64_t,
                                                 const uint64_t*, int32_t*, hipStream_t);
 
+// The region clean: TOMBSTONE -> EMPTY and displaced elements lifted out and
+// reinserted, region by region from an EMPTY slot; `gate` as in k_clean_begin.
 template <typename K>
-int clean_tombstones(void* buckets, hn_info* info, int sp, void* scratch, size_t bytes, hipStream_t s) {
-   if (!buckets || !info || !scratch) return HN_EINVAL;
-   if (sp < 1 || sp > max_size_power<K>()) return HN_EINVAL;
-   const size_t rb = sp <= 32 ? rebuild_scratch_bytes<K>(sp, false) : 0;
-   if (rb && bytes >= rb) return rebuild<K>(buckets, info, sp, scratch, bytes, s);
+int region_clean(void* buckets, hn_info* info, int sp, void* scratch, size_t bytes, const uint32_t* gate,
+                 hipStream_t s) {
    const uint64_t n = 1ull << sp;
    uint64_t rs = 0, nr = 0;
    size_t hb = 0;
@@ -540,7 +565,7 @@ int clean_tombstones(void* buckets, hn_info* info, int sp, void* scratch, size_t
    CleanHeader* h = static_cast<CleanHeader*>(scratch);
    Slot<K>* pairs = reinterpret_cast<Slot<K>*>(static_cast<char*>(scratch) + hb);
    Slot<K>* b = static_cast<Slot<K>*>(buckets);
-   hipLaunchKernelGGL(k_clean_begin, dim3(1), dim3(256), 0, s, info, h, static_cast<uint32_t>(nr));
+   hipLaunchKernelGGL(k_clean_begin, dim3(1), dim3(256), 0, s, info, h, static_cast<uint32_t>(nr), gate);
    const uint64_t chunk = 256 * 16;
    hipLaunchKernelGGL(k_find_empty<K>, dim3(static_cast<uint32_t>((n + chunk - 1) / chunk)), dim3(256), 0, s, b, n, h);
    hipLaunchKernelGGL(k_clean_commit, dim3(1), dim3(1), 0, s, info, h, n, rs);
@@ -553,6 +578,29 @@ int clean_tombstones(void* buckets, hn_info* info, int sp, void* scratch, size_t
    }
    hipLaunchKernelGGL(k_clean_end, dim3(1), dim3(1), 0, s, info, h);
    return static_cast<int>(hipGetLastError());
+}
+
+// Hashmap::clean_tombstones (hashinator.h:1261-1315).
+template <typename K>
+int clean_tombstones(void* buckets, hn_info* info, int sp, void* scratch, size_t bytes, hipStream_t s) {
+   if (!buckets || !info || !scratch) return HN_EINVAL;
+   if (sp < 1 || sp > max_size_power<K>()) return HN_EINVAL;
+   const uint64_t n = 1ull << sp;
+   const size_t seg = seg_clean_fixed_bytes(sp), ro = reorder_scratch_bytes(sp);
+   if (bytes < seg + ro + seg_clean_min_pool()) return region_clean<K>(buckets, info, sp, scratch, bytes, nullptr, s);
+   char* base = static_cast<char*>(scratch);
+   char* pool = base + seg + ro;
+   const size_t pool_bytes = bytes - seg - ro;
+   uint64_t rs = 0, nr = 0;
+   size_t hb = 0;
+   const bool region_ok = clean_plan<K>(n, pool_bytes, &rs, &nr, &hb) && nr == 1;
+   int rc = seg_clean<K>(buckets, info, sp, base, pool, pool_bytes, region_ok, s);
+   if (rc || !region_ok) return rc;
+   // a table with no EMPTY slot: one region over the whole table, then the reorder
+   const uint32_t* gate = seg_clean_gate(base);
+   rc = region_clean<K>(buckets, info, sp, pool, pool_bytes, gate, s);
+   if (rc) return rc;
+   return reorder_gated<K>(buckets, info, sp, base + seg, ro, gate, s);
 }
 
 template <typename K>

@@ -42,12 +42,13 @@ static_assert(sizeof(ReorderHeader) == 256, "header is one 256-byte line");
 
 inline uint64_t reorder_tile(int sp) { return std::min<uint64_t>(kReorderTile, 1ull << sp); }
 
-__global__ void k_reorder_begin(const hn_info* info, ReorderHeader* h, uint64_t n) {
+// `gate` (nullable): run only when *gate is nonzero (a flag another pass leaves).
+__global__ void k_reorder_begin(const hn_info* info, ReorderHeader* h, uint64_t n, const uint32_t* gate) {
    if (threadIdx.x == 0) {
       h->n_over = 0;
       h->fail = 0;
       h->max_probe = 0;
-      h->skip = (info->tombstone_counter != 0 || info->fill >= n) ? 1u : 0u;
+      h->skip = ((gate && *gate == 0) || info->tombstone_counter != 0 || info->fill >= n) ? 1u : 0u;
    }
 }
 
@@ -220,7 +221,8 @@ size_t reorder_scratch_bytes(int sp) {
 constexpr int kReorderBS1 = 256, kReorderBS2 = 1024, kReorderE = 8;
 
 template <typename K>
-int reorder(void* buckets, hn_info* info, int sp, void* scratch, size_t bytes, hipStream_t s) {
+int reorder(void* buckets, hn_info* info, int sp, void* scratch, size_t bytes, hipStream_t s,
+            const uint32_t* gate = nullptr) {
    if (!buckets || !info || !scratch) return HN_EINVAL;
    if (sp < 1 || sp > max_size_power<K>()) return HN_EINVAL;
    if (bytes < reorder_scratch_bytes(sp)) return HN_ESCRATCH;
@@ -229,7 +231,7 @@ int reorder(void* buckets, hn_info* info, int sp, void* scratch, size_t bytes, h
    ReorderHeader* h = static_cast<ReorderHeader*>(scratch);
    uint32_t* over = reinterpret_cast<uint32_t*>(h + 1);
    Slot<K>* b = static_cast<Slot<K>*>(buckets);
-   hipLaunchKernelGGL(k_reorder_begin, dim3(1), dim3(64), 0, s, info, h, n);
+   hipLaunchKernelGGL(k_reorder_begin, dim3(1), dim3(64), 0, s, info, h, n, gate);
    hipLaunchKernelGGL((k_reorder<K, kReorderBS1, kReorderE>), dim3(static_cast<uint32_t>(tiles)), dim3(kReorderBS1), 0,
                       s, b, sp, info, h, over, tile, static_cast<uint32_t>(tiles), 0);
    const uint32_t grid2 = static_cast<uint32_t>(std::min<uint64_t>(tiles, static_cast<uint64_t>(cu_count())));
@@ -238,6 +240,16 @@ int reorder(void* buckets, hn_info* info, int sp, void* scratch, size_t bytes, h
    hipLaunchKernelGGL(k_reorder_end, dim3(1), dim3(64), 0, s, info, h);
    return static_cast<int>(hipGetLastError());
 }
+
+// The reorder after the region clean of clean_tombstones, run only when the
+// segment pass of hn_clean.hip left the table to that path.
+template <typename K>
+int reorder_gated(void* buckets, hn_info* info, int sp, void* scratch, size_t bytes, const uint32_t* gate,
+                  hipStream_t s) {
+   return reorder<K>(buckets, info, sp, scratch, bytes, s, gate);
+}
+template int reorder_gated<uint32_t>(void*, hn_info*, int, void*, size_t, const uint32_t*, hipStream_t);
+template int reorder_gated<uint64_t>(void*, hn_info*, int, void*, size_t, const uint32_t*, hipStream_t);
 
 } // namespace hn
 

@@ -28,6 +28,7 @@ EMPTY = {32: 0xFFFFFFFF, 64: 0xFFFFFFFFFFFFFFFF}
 TOMBSTONE = {32: 0xFFFFFFFE, 64: 0xFFFFFFFFFFFFFFFE}
 _TDT = {32: torch.int32, 64: torch.int64}
 _NDT = {32: np.uint32, 64: np.uint64}
+_SIZE_MAX = (1 << 64) - 1
 
 
 def _ptr(t):
@@ -372,19 +373,28 @@ class Hashmap:
     @on_stream
     def clean_tombstones(self, stream=None, scratch_budget: int = 0):
         """clean_tombstones(s) (hashinator.h:1261-1315).  ``scratch_budget``
-        bounds the scratch in bytes (0: enough for the whole table)."""
-        sb = getattr(self._L, f"hn_clean_tombstones_scratch_bytes_{self._sfx}")(self._sp, scratch_budget)
+        bounds the scratch in bytes (0: the default, a few MB of segment lists
+        plus a pool for segments too long for LDS).  A table with every slot
+        occupied, and a retry after the pool ran short (err = FAIL with
+        tombstones left), get room for a region clean over the whole table."""
+        q = getattr(self._L, f"hn_clean_tombstones_scratch_bytes_{self._sfx}")
         i = self.info()
         if i.tombstone_counter == 0:
             return
-        if i.fill + i.tombstone_counter >= self.bucket_count():
-            sb = getattr(self._L, f"hn_clean_tombstones_scratch_bytes_{self._sfx}")(self._sp, 0)
-        scratch = torch.empty(sb, dtype=torch.uint8, device=self.device)
-        check(self._fn("clean_tombstones")(_ptr(self.buckets), _ptr(self._info), self._sp, _ptr(scratch), sb,
-                                           _stream(stream)), "clean_tombstones")
-        self._sync(stream)
+        full = i.fill + i.tombstone_counter >= self.bucket_count()
+        for attempt in range(2):
+            budget = _SIZE_MAX if (full or attempt) else scratch_budget
+            sb = q(self._sp, budget)
+            scratch = torch.empty(sb, dtype=torch.uint8, device=self.device)
+            check(self._fn("clean_tombstones")(_ptr(self.buckets), _ptr(self._info), self._sp, _ptr(scratch), sb,
+                                               _stream(stream)), "clean_tombstones")
+            self._sync(stream)
+            del scratch
+            i = self.info()
+            if full or scratch_budget or i.tombstone_counter == 0 or i.err != status.fail:
+                break
         if self.auto_reorder and not self.ordered():
-            self._reorder(stream)  # the bounded-scratch path reinserts in arrival order
+            self._reorder(stream)  # the small-scratch region path reinserts in arrival order
             self._sync(stream)
 
     @on_stream

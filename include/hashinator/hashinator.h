@@ -30,6 +30,7 @@
  */
 #pragma once
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <limits>
 #include <stdexcept>
@@ -861,16 +862,23 @@ public:
    void clean_tombstones(hipStream_t s = 0) {
       if (_info.tombstoneCounter == 0) return;
       const int sp = _info.sizePower;
-      // Whole-table scratch: the rebuild path (every live slot through the
-      // bulk build, leaving the table in Robin Hood order).
-      const size_t sb =
-          W64 ? hn_clean_tombstones_scratch_bytes_u64(sp, 0) : hn_clean_tombstones_scratch_bytes_u32(sp, 0);
-      void* scratch = nullptr;
-      SPLIT_CHECK_ERR(hipMallocAsync(&scratch, sb, s));
-      chk(W64 ? hn_clean_tombstones_u64(_buckets, dinfo(), sp, scratch, sb, st(s))
-              : hn_clean_tombstones_u32(_buckets, dinfo(), sp, scratch, sb, st(s)));
-      SPLIT_CHECK_ERR(hipFreeAsync(scratch, s));
-      changed(s);
+      // The segment pass (a few MB of lists plus a pool for the rare segments
+      // too long for LDS); a table with every slot occupied needs room for a
+      // region clean over the whole table, and so does a retry after the
+      // pool ran short (err = FAIL with TOMBSTONEs left).
+      const bool full = _info.fill + _info.tombstoneCounter >= (size_t(1) << sp);
+      for (int attempt = 0; attempt < 2; ++attempt) {
+         const size_t budget = (full || attempt) ? SIZE_MAX : 0;
+         const size_t sb = W64 ? hn_clean_tombstones_scratch_bytes_u64(sp, budget)
+                               : hn_clean_tombstones_scratch_bytes_u32(sp, budget);
+         void* scratch = nullptr;
+         SPLIT_CHECK_ERR(hipMallocAsync(&scratch, sb, s));
+         chk(W64 ? hn_clean_tombstones_u64(_buckets, dinfo(), sp, scratch, sb, st(s))
+                 : hn_clean_tombstones_u32(_buckets, dinfo(), sp, scratch, sb, st(s)));
+         SPLIT_CHECK_ERR(hipFreeAsync(scratch, s));
+         changed(s);
+         if (full || _info.tombstoneCounter == 0 || _info.err != status::fail) break;
+      }
       if (_auto_reorder > 0.0f && !ordered()) reorder(s);
    }
 

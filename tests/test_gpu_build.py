@@ -204,33 +204,25 @@ def test_build_needs_no_initialised_table(gpu, bits, case):
         assert int(inf[1]) == keys.size and (int(inf[4]) & 0xFFFFFFFF) == 0
 
 
-@pytest.mark.parametrize("bits", [32, 64])
-@pytest.mark.parametrize("sp,lf,scratch", [(10, 0.9, "two arrays"), (14, 0.9, "two arrays"), (20, 0.95, "two arrays"),
-                                           (21, 0.85, "two arrays"), (21, 0.85, "one array"),
-                                           (22, 0.9, "two arrays"), (22, 0.9, "one array")])
-def test_clean_rebuild_image_exact(gpu, bits, sp, lf, scratch):
-    """clean_tombstones with whole-table scratch rebuilds the live slots: the
-    image is the oracle's clean followed by its reorder, bit for bit.  With
-    scratch for two pair arrays a 2^21+ table takes two partition passes,
-    with one array three."""
-    keys = keyset(int(lf * (1 << sp)), bits, seed=sp + bits)
+def _tombstoned_map(bits, sp, lf, seed=0):
+    keys = keyset(int(lf * (1 << sp)), bits, seed=sp + bits + seed)
     vals = (keys * NDT[bits](9)).astype(NDT[bits])
     hm = Hashmap(sp, bits, auto_build=0, auto_reorder=0)
     hm.insert(keys, vals, targetLF=1.0)
     er = keys[np.random.default_rng(sp).choice(keys.size, keys.size // 4, replace=False)]
     hm.erase(er)
-    m = O.GpuModel(sp, bits, buckets=hm.host_buckets().reshape(-1).copy(), info=O.Info(
-        sp, 0, hm.info().fill, hm.info().current_max_bucket_overflow, hm.info().tombstone_counter, 0, 0))
-    L = _lib.load()
-    q = getattr(L, f"hn_clean_tombstones_scratch_bytes_u{bits}")
-    full = q(sp, 0)
-    budget = 0 if scratch == "two arrays" else full - 1
-    assert scratch == "two arrays" or full - q(sp, budget) >= (1 << sp) * (bits // 4)  # a smaller rebuild plan
-    hm.clean_tombstones(scratch_budget=budget)  # whole-table scratch -> rebuild
+    i = hm.info()
+    m = O.GpuModel(sp, bits, buckets=hm.host_buckets().reshape(-1).copy(),
+                   info=O.Info(sp, 0, i.fill, i.current_max_bucket_overflow, i.tombstone_counter, i.err, 0))
+    return hm, m
+
+
+def _check_clean_exact(hm, m, bits):
+    """Every key slot equals the oracle's clean followed by its reorder; values
+    equal in live slots (an EMPTY slot's value is unspecified: the oracle keeps
+    the erased value under the EMPTY key, the device writes 0); Info too."""
     m.clean_tombstones()
     assert m.reorder() == 1
-    # every key slot exact; values exact in live slots (an EMPTY slot's value is unspecified:
-    # the oracle keeps the erased value under the EMPTY key, the rebuild writes 0)
     img, ref = hm.host_buckets(), m.b.reshape(-1, 2)
     assert np.array_equal(img[:, 0], ref[:, 0])
     live = ref[:, 0] != np.iinfo(NDT[bits]).max
@@ -238,10 +230,100 @@ def test_clean_rebuild_image_exact(gpu, bits, sp, lf, scratch):
     i = hm.info()
     assert (i.fill, i.tombstone_counter, i.current_max_bucket_overflow) == (m.info.fill, 0, m.info.overflow)
     assert i.err == 0 and (i.reserved1 & _lib.HN_INFO_ORDERED)
-    # no tombstones: a no-op, Info untouched
-    before = hm.host_buckets().copy()
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+@pytest.mark.parametrize("sp,lf", [(10, 0.9), (14, 0.9), (16, 0.97), (18, 0.98), (20, 0.95), (21, 0.85), (22, 0.9),
+                                   (12, 0.5)])
+def test_clean_image_exact(gpu, bits, sp, lf):
+    """clean_tombstones by EMPTY-bounded segments: the image is the oracle's
+    clean followed by its reorder, bit for bit.  LF 0.97 and 0.98 have
+    clusters longer than the 8,192 slots LDS holds (the pass over global
+    memory; at 0.98 the default pool runs short and the map retries with a
+    larger one)."""
+    hm, m = _tombstoned_map(bits, sp, lf)
     hm.clean_tombstones()
-    assert np.array_equal(hm.host_buckets(), before)
+    _check_clean_exact(hm, m, bits)
+    # no tombstones: a no-op, Info untouched
+    before, info = hm.host_buckets().copy(), bytes(hm.info())
+    hm.clean_tombstones()
+    assert np.array_equal(hm.host_buckets(), before) and bytes(hm.info()) == info
+
+
+def _skewed_keys(bits, sp, n, lo, width, seed):
+    """n distinct keys whose home slots lie in [lo, lo + width): one cluster of about n slots."""
+    out = []
+    start = 0
+    while sum(a.size for a in out) < n:
+        cand = keyset(1 << 22, bits, seed=seed, start=start)
+        start += (1 << 22) + 2
+        h = O.fibhash(cand, sp, bits).astype(np.int64) & ((1 << sp) - 1)
+        out.append(cand[(h >= lo) & (h < lo + width)])
+    return np.concatenate(out)[:n]
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_clean_pool_too_small_then_enough(gpu, bits):
+    """A cluster of about 12,000 slots (keys with homes in a 64-slot window)
+    is longer than LDS holds.  Through the C-ABI with a 64 KiB pool it does
+    not fit: that segment is left as it was (same live set, every key
+    reachable), its tombstones stay counted, err = FAIL, the table is not
+    marked ordered.  A second call with scratch_bytes(sp, SIZE_MAX) finishes
+    the image exactly as the oracle's clean + reorder."""
+    sp = 14
+    keys = _skewed_keys(bits, sp, 12000, 100, 64, seed=5)
+    more = keyset(2000, bits, seed=6, start=10**7)  # ordinary keys elsewhere
+    allk = np.concatenate([keys, more])
+    vals = (allk * NDT[bits](3)).astype(NDT[bits])
+    hm = Hashmap(sp, bits, auto_build=0, auto_reorder=0)
+    hm.insert(allk, vals, targetLF=1.0)
+    hm.erase(allk[np.random.default_rng(1).choice(allk.size, allk.size // 4, replace=False)])
+    i = hm.info()
+    m = O.GpuModel(sp, bits, buckets=hm.host_buckets().reshape(-1).copy(),
+                   info=O.Info(sp, 0, i.fill, i.current_max_bucket_overflow, i.tombstone_counter, i.err, 0))
+    L = _lib.load()
+    sfx = "u32" if bits == 32 else "u64"
+    q = getattr(L, f"hn_clean_tombstones_scratch_bytes_{sfx}")
+    fixed = q(sp, 0) - (1 << 20)  # the default pool at this size is 1 MiB
+    tiny = fixed + (64 << 10)
+    assert q(sp, tiny) == tiny
+    s = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    clean = getattr(L, f"hn_clean_tombstones_{sfx}")
+    live0 = live_pairs(hm.host_buckets(), bits)
+    tomb0 = hm.tombstone_count()
+    sc = torch.empty(tiny, dtype=torch.uint8, device="cuda")
+    assert clean(C.c_void_p(hm.bucket_ptr()), C.c_void_p(hm.info_ptr()), sp, C.c_void_p(sc.data_ptr()), tiny, s) == 0
+    torch.cuda.synchronize()
+    i = hm.info()
+    img = hm.host_buckets()
+    assert np.array_equal(live_pairs(img, bits), live0) and reachable_all(img, bits, sp)
+    assert i.err == _lib.HN_STATUS_FAIL and 0 < i.tombstone_counter < tomb0
+    assert i.tombstone_counter == int(np.sum(img[:, 0] == np.iinfo(NDT[bits]).max - 1))
+    assert not (i.reserved1 & _lib.HN_INFO_ORDERED)
+    big = q(sp, (1 << 64) - 1)
+    sc = torch.empty(big, dtype=torch.uint8, device="cuda")
+    assert clean(C.c_void_p(hm.bucket_ptr()), C.c_void_p(hm.info_ptr()), sp, C.c_void_p(sc.data_ptr()), big, s) == 0
+    torch.cuda.synchronize()
+    _check_clean_exact(hm, m, bits)
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_clean_full_table(gpu, bits):
+    """Every slot occupied (live or TOMBSTONE): no segment boundary, the region
+    clean over the whole table; then the same image as the oracle's."""
+    sp = 12
+    n = 1 << sp
+    keys = keyset(n, bits, seed=77)
+    hm = Hashmap(sp, bits, auto_build=0, auto_reorder=0)
+    hm.insert(keys, keys, targetLF=1.0)
+    hm.erase(keys[::3])
+    i = hm.info()
+    assert i.fill + i.tombstone_counter == n
+    m = O.GpuModel(sp, bits, buckets=hm.host_buckets().reshape(-1).copy(),
+                   info=O.Info(sp, 0, i.fill, i.current_max_bucket_overflow, i.tombstone_counter, i.err, 0))
+    hm.auto_reorder = 1.0 / 16
+    hm.clean_tombstones()
+    _check_clean_exact(hm, m, bits)
 
 
 @pytest.mark.parametrize("bits", [32, 64])

@@ -44,12 +44,17 @@ def test_info_layout_matches_reference():
 def test_scratch_sizing_host_only():
     L = _lib.load()
     for sp in (10, 20, 28):
-        full = L.hn_clean_tombstones_scratch_bytes_u64(sp, 0)
-        assert full >= (1 << sp) * 16
-        one = L.hn_clean_tombstones_scratch_bytes_u64(sp, full - 1)  # rebuild through one pair array
-        assert (1 << sp) * 16 <= one <= full and (sp < 21 or full - one >= (1 << sp) * 16)
-        b = L.hn_clean_tombstones_scratch_bytes_u64(sp, 1 << 20)
-        assert min(L.hn_clean_tombstones_min_scratch(), full) <= b <= max(1 << 20, full)
+        table = (1 << sp) * 16
+        default = L.hn_clean_tombstones_scratch_bytes_u64(sp, 0)
+        # the segment pass: per-tile lists (12 B per 1024 slots), the reorder's, and a long-segment pool
+        # for 1/32 of the table -- a tenth of the table or less from 2^20 slots on, never twice the table
+        assert default >= (1 << sp) // 1024 * 12 + (1 << 20)
+        assert sp < 20 or default <= table // 8
+        anything = L.hn_clean_tombstones_scratch_bytes_u64(sp, (1 << 64) - 1)  # a full table included
+        assert anything >= table and anything >= default
+        assert L.hn_clean_tombstones_scratch_bytes_u64(sp, default + 4096) == default + 4096  # a budget is used whole
+        b = L.hn_clean_tombstones_scratch_bytes_u64(sp, 1 << 16)  # too small for the segment pass: regions
+        assert 0 < b <= 1 << 16
     assert L.hn_clean_tombstones_scratch_bytes_u32(0, 0) == 0  # invalid size power
     assert L.hn_compact_scratch_bytes(1 << 28) >= (1 << 28) // 8192 * 16  # two 8-byte state words per 8192-element tile
     assert L.hn_rehash_scratch_bytes() >= 8
