@@ -98,32 +98,24 @@ struct BuildPlan {
    uint32_t nblocks;    // partition blocks
    uint32_t sub;        // tiles per partition block
    size_t off_pairs, off_mat, off_scan, off_start, off_word, off_spec, off_carry, off_mark, off_list, off_blk, total;
-   size_t off_pairs2;   // a second pair array for the passes that cannot use the table's memory (0: none)
 };
 
 inline size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 
 // dcap: widest partition digit (1..9 bits); narrower digits mean more passes.
-// rebuild: the input is the table itself (n = 2^sp slots, live ones kept), so
-// no pass may write the table's memory before the last has read it: with
-// `two` a second pair array in the scratch takes the first of two passes,
-// otherwise the pass count is made odd (3) so that the first pass writes the
-// scratch and the second the table.
 template <typename K>
-bool build_plan(int sp, uint64_t n, int dcap, bool rebuild, BuildPlan* P, bool two = false) {
+bool build_plan(int sp, uint64_t n, int dcap, BuildPlan* P) {
    if (dcap < 1 || dcap > kMaxDigitBits) return false;
    if (sp < 1 || sp > std::min(max_size_power<K>(), 32)) return false;
    P->sp = sp;
    P->n = n;
    P->N = 1ull << sp;
-   if (n == 0 || n > P->N || (!rebuild && n == P->N)) return false;
+   if (n == 0 || n >= P->N) return false;
    P->rlog = std::min(sp, kRegionLog);
    P->R = 1ull << P->rlog;
    P->rbits = sp - P->rlog;
    P->NR = 1ull << P->rbits;
    P->passes = std::max(1, (P->rbits + dcap - 1) / dcap);
-   const bool second = rebuild && two && P->passes == 2;
-   if (rebuild && !second && P->passes == 2) P->passes = 3;
    if (P->passes > 3) return false;
    int left = P->rbits;
    for (int i = 0; i < P->passes; ++i) {
@@ -146,11 +138,6 @@ bool build_plan(int sp, uint64_t n, int dcap, bool rebuild, BuildPlan* P, bool t
    size_t o = sizeof(BuildHeader);
    P->off_pairs = o;
    o = align256(o + n * sizeof(Slot<K>));
-   P->off_pairs2 = 0;
-   if (second) {
-      P->off_pairs2 = o;
-      o = align256(o + n * sizeof(Slot<K>));
-   }
    P->off_mat = o;
    o = align256(o + mat * sizeof(uint32_t));
    P->off_scan = o;
@@ -210,18 +197,15 @@ __device__ __forceinline__ uint32_t region_of(K key, int sp, int rlog) {
    return static_cast<uint32_t>((fib_home(key, sp) & mask) >> rlog);
 }
 
-// Build from caller arrays into an empty table, (rebuild) from the table's
-// own live slots, which is clean_tombstones, or (rehash) from the live slots
-// of another, smaller table into a new one.
-enum BuildMode { BUILD_EMPTY = 0, BUILD_REBUILD = 1, BUILD_REHASH = 2 };
+// Build from caller arrays into an empty table, or (rehash) from the live
+// slots of another, smaller table into a new one.
+enum BuildMode { BUILD_EMPTY = 0, BUILD_REHASH = 2 };
 
 __global__ void k_build_begin(const hn_info* info, BuildHeader* h, int sp, uint64_t len, int mode) {
    if (threadIdx.x == 0) {
       const bool sp_ok = info->size_power == sp;
       if (mode == BUILD_EMPTY)
          h->go = (sp_ok && info->fill == 0 && info->tombstone_counter == 0) ? 1u : 0u;
-      else if (mode == BUILD_REBUILD)
-         h->go = (sp_ok && info->tombstone_counter != 0) ? 1u : 0u;
       else
          h->go = 1u; // k_rehash_info has just set the Info up for the new table
       h->fail = 0;
@@ -230,7 +214,7 @@ __global__ void k_build_begin(const hn_info* info, BuildHeader* h, int sp, uint6
       h->n_distinct = 0;
       h->fb_fail = 0;
       h->fb_len = 0;
-      h->n = mode == BUILD_EMPTY ? len : 0; // a rebuild or rehash counts the live slots in its first pass
+      h->n = mode == BUILD_EMPTY ? len : 0; // a rehash counts the live slots in its first pass
    }
 }
 
@@ -971,8 +955,7 @@ __global__ __launch_bounds__(256) void k_build_fallback_prep(Slot<K>* b, uint64_
 }
 
 // Finish the Info.  Not run (go = 0): an empty-table build reports
-// err = INVALID with the table untouched; a rebuild (no tombstones) is a
-// no-op.  Placed: fill, overflow of the layout, ordered.  Fallback: the
+// err = INVALID with the table untouched.  Placed: fill, overflow of the layout, ordered.  Fallback: the
 // insert has set fill and overflow.
 __global__ void k_build_end(hn_info* info, const BuildHeader* h, int mode) {
    if (threadIdx.x != 0) return;
@@ -1035,8 +1018,8 @@ int launch_insert_pairs_dlen(const Slot<K>* pairs, Slot<K>* buckets, hn_info* in
                              const uint64_t* d_len, int32_t* fail_sink, hipStream_t s);
 
 // Everything after the first partition pass's input is chosen.  src0 picks
-// the first pass's input kind; n_in is its element count (the slot count for
-// a rebuild, whose first pass keeps live slots only).
+// the first pass's input kind; n_in is its element count (the old table's
+// slot count for a rehash, whose first pass keeps live slots only).
 template <typename K>
 int build_core(const BuildPlan& P, int mode, int src0, PartIn<K> in0, uint64_t n_in, void* buckets, hn_info* info,
                void* scratch, hipStream_t s) {
@@ -1056,11 +1039,9 @@ int build_core(const BuildPlan& P, int mode, int src0, PartIn<K> in0, uint64_t n
    Slot<K>* T = static_cast<Slot<K>*>(buckets);
 
    hipLaunchKernelGGL(k_build_begin, dim3(1), dim3(64), 0, s, info, h, P.sp, static_cast<uint64_t>(P.n), mode);
-   // Pass i writes S when (passes - 1 - i) is even, the table's memory (or
-   // the second pair array, when the plan has one) otherwise, so the last
-   // pass lands in S; a rebuild without a second array has an odd number of
-   // passes, so its first pass (which reads the table) writes S.
-   Slot<K>* const alt = P.off_pairs2 ? reinterpret_cast<Slot<K>*>(base + P.off_pairs2) : T;
+   // Pass i writes S when (passes - 1 - i) is even, the table's memory
+   // otherwise, so the last pass lands in S.
+   Slot<K>* const alt = T;
    const Slot<K>* src = nullptr;
    for (int pass = 0; pass < P.passes; ++pass) {
       Slot<K>* dst = ((P.passes - 1 - pass) % 2 == 0) ? S : alt;
@@ -1116,7 +1097,7 @@ int build(const void* keys, const void* vals, const void* pairs, void* buckets, 
    if (!keys && !pairs) return HN_EINVAL;
    const int dcap = (t && t->reserved) ? t->reserved : kMaxDigitBits;
    BuildPlan P;
-   if (!build_plan<K>(sp, len, dcap, false, &P)) return HN_EINVAL;
+   if (!build_plan<K>(sp, len, dcap, &P)) return HN_EINVAL;
    if (!scratch || bytes < P.total) return HN_ESCRATCH;
    PartIn<K> in{static_cast<const K*>(keys), static_cast<const K*>(vals), static_cast<const Slot<K>*>(pairs)};
    const int src0 = pairs ? SRC_PAIRS : (vals ? SRC_SOA : SRC_INDEX);
@@ -1126,27 +1107,9 @@ int build(const void* keys, const void* vals, const void* pairs, void* buckets, 
 template <typename K>
 size_t build_scratch_bytes(int sp, size_t len) {
    BuildPlan P;
-   return build_plan<K>(sp, len, kMaxDigitBits, false, &P) ? P.total : 0;
+   return build_plan<K>(sp, len, kMaxDigitBits, &P) ? P.total : 0;
 }
 
-// clean_tombstones as a rebuild of the live slots (see hn_map.hip).  `two`:
-// the plan with a second pair array (one partition pass fewer where the
-// plan has two, the same plan otherwise).
-template <typename K>
-size_t rebuild_scratch_bytes(int sp, bool two) {
-   BuildPlan P;
-   return (sp >= 1 && sp <= 32 && build_plan<K>(sp, 1ull << sp, kMaxDigitBits, true, &P, two)) ? P.total : 0;
-}
-
-template <typename K>
-int rebuild(void* buckets, hn_info* info, int sp, void* scratch, size_t bytes, hipStream_t s) {
-   BuildPlan P;
-   if (!build_plan<K>(sp, 1ull << sp, kMaxDigitBits, true, &P, true)) return HN_EINVAL;
-   if (bytes < P.total && !build_plan<K>(sp, 1ull << sp, kMaxDigitBits, true, &P, false)) return HN_EINVAL;
-   if (!scratch || bytes < P.total) return HN_ESCRATCH;
-   PartIn<K> in{nullptr, nullptr, static_cast<const Slot<K>*>(buckets)};
-   return build_core<K>(P, BUILD_REBUILD, SRC_TABLE, in, P.N, buckets, info, scratch, s);
-}
 // device_rehash into a larger table as a build: the first partition pass
 // reads the old table's live slots (extractPattern + insert in the reference,
 // hashinator.h:284-336, without the intermediate array).  The new table is
@@ -1155,7 +1118,7 @@ int rebuild(void* buckets, hn_info* info, int sp, void* scratch, size_t bytes, h
 template <typename K>
 bool rehash_plan(int old_sp, int new_sp, BuildPlan* P) {
    if (old_sp < 1 || new_sp <= old_sp || new_sp > std::min(max_size_power<K>(), 32)) return false;
-   return build_plan<K>(new_sp, 1ull << old_sp, kMaxDigitBits, false, P);
+   return build_plan<K>(new_sp, 1ull << old_sp, kMaxDigitBits, P);
 }
 
 template <typename K>
@@ -1179,10 +1142,6 @@ int rehash_build(const void* old_b, int old_sp, void* new_b, int new_sp, hn_info
    return build_core<K>(P, BUILD_REHASH, SRC_TABLE, in, 1ull << old_sp, new_b, info, scratch, s);
 }
 
-template size_t rebuild_scratch_bytes<uint32_t>(int, bool);
-template size_t rebuild_scratch_bytes<uint64_t>(int, bool);
-template int rebuild<uint32_t>(void*, hn_info*, int, void*, size_t, hipStream_t);
-template int rebuild<uint64_t>(void*, hn_info*, int, void*, size_t, hipStream_t);
 
 } // namespace hn
 

@@ -52,7 +52,8 @@ struct SegCleanHeader {
    uint32_t displaced;  // a live key stood away from its home slot
    uint32_t run_region; // the region clean (and the reorder after it) must run
    uint32_t region_ok;  // the caller's scratch holds a whole-table region clean
-   uint32_t pad[51];
+   uint32_t ordered;    // the table is in Robin Hood order (HN_INFO_ORDERED)
+   uint32_t pad[50];
    // followed by bnd[tiles] (uint64_t), over[tiles] and long_list[tiles] (uint32_t)
 };
 static_assert(sizeof(SegCleanHeader) == 256, "header is one 256-byte line");
@@ -94,6 +95,7 @@ __global__ void k_seg_clean_begin(const hn_info* info, SegCleanHeader* h, uint64
       h->region_ok = region_ok;
       h->skip = info->tombstone_counter == 0 ? 1u : 0u;
       h->no_empty = (!h->skip && info->fill + info->tombstone_counter >= n) ? 1u : 0u;
+      h->ordered = (info->reserved1 & kInfoOrdered) ? 1u : 0u;
    }
 }
 
@@ -202,7 +204,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) void k_
    __shared__ int32_t wmax_s[BS / kWave];
    __shared__ uint32_t probe_s;
    __shared__ uint32_t moved_s;
-   if (h->skip || h->no_empty) return;
+   if (h->skip || h->no_empty || (!second && h->ordered)) return;
    const uint64_t n = 1ull << sp, mask = n - 1;
    const int lane = __lane_id();
    const uint32_t count = second ? h->n_over : n_tiles;
@@ -348,6 +350,201 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) void k_
       for (int e = 0; e < E; ++e) {
          const uint32_t i = threadIdx.x + e * BS;
          if (i < len && key_s[i] != empty_key<K>() && !((occ_s[i >> 5] >> (i & 31)) & 1u)) {
+            Slot<K> s;
+            s.key = empty_key<K>();
+            s.val = 0;
+            b[(b0 + i) & mask] = s;
+         }
+      }
+   }
+   // one update of the header per workgroup, and only when it changes something
+   max_probe = wave_max(max_probe);
+   __syncthreads();
+   if (lane == 0) {
+      if (max_probe) atomicMax(&probe_s, max_probe);
+      if (moved_any) moved_s = 1;
+   }
+   __syncthreads();
+   if (threadIdx.x == 0) {
+      if (probe_s > __hip_atomic_load(&h->max_probe, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+         atomic_max_relaxed(&h->max_probe, probe_s);
+      if (moved_s && !__hip_atomic_load(&h->displaced, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+         __hip_atomic_store(&h->displaced, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+   }
+}
+
+// Ordered tables.  In Robin Hood order every segment's live keys already
+// stand in (relative home, key) order, and erasing keys keeps that, so the
+// placement needs no sort: p_j = max(r_j, p_{j-1} + 1) runs over the live
+// keys in slot order.  It is a scan of the maps x -> max(a, x + c) under
+// composition, each live key contributing (r, 1); `m`, the largest relative
+// home so far, checks the order (a key whose r is below an earlier key's
+// sends the segment to the sorting pass, as does a home outside the segment
+// or a destination past it).
+constexpr int32_t kNegPos = -(1 << 30);
+struct Place {
+   int32_t a, c, m;
+};
+__device__ __forceinline__ Place place_join(Place l, Place r) { // l: the slots before r
+   return Place{std::max(r.a, l.a + r.c), l.c + r.c, std::max(l.m, r.m)};
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ Place place_dpp(Place x) {
+   Place y;
+   y.a = __builtin_amdgcn_update_dpp(kNegPos, x.a, CTRL, ROWS, 0xf, false);
+   y.c = __builtin_amdgcn_update_dpp(0, x.c, CTRL, ROWS, 0xf, false);
+   y.m = __builtin_amdgcn_update_dpp(kNegPos, x.m, CTRL, ROWS, 0xf, false);
+   return place_join(y, x);
+}
+// Inclusive scan over the wave's lanes: row shifts inside 16-lane rows, then
+// row broadcasts; lanes a step does not reach keep the identity.
+__device__ __forceinline__ Place place_wave_scan(Place x) {
+   x = place_dpp<0x111, 0xf>(x); // row_shr:1
+   x = place_dpp<0x112, 0xf>(x); // row_shr:2
+   x = place_dpp<0x114, 0xf>(x); // row_shr:4
+   x = place_dpp<0x118, 0xf>(x); // row_shr:8
+   x = place_dpp<0x142, 0xa>(x); // row_bcast:15 -> rows 1, 3
+   x = place_dpp<0x143, 0xc>(x); // row_bcast:31 -> rows 2, 3
+   return x;
+}
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, int l) {
+   const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(v)), l));
+   const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(v >> 32)), l));
+   return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+// One workgroup per segment at a time, capacity BS*E slots in rows of 64:
+// row q = e * (BS / 64) + wave holds slots [64q, 64q + 64), so short segments
+// keep every wave busy and each row is one coalesced read.  Every wave finds
+// the segment's bounds itself, one segment ahead; the occupancy bit maps
+// alternate between two buffers, so a segment takes three barriers.
+template <typename K, int BS, int E>
+__global__ __launch_bounds__(BS) void k_seg_clean_ord(Slot<K>* b, int sp, SegCleanHeader* h, const uint64_t* bnd,
+                                                      uint32_t* over, uint64_t tile, uint32_t n_tiles) {
+   constexpr int NW = BS / kWave;
+   constexpr uint32_t CAP = BS * E;
+   constexpr int NR = NW * E;
+   static_assert(NR <= kWave, "one lane per row for the row prefix");
+   __shared__ uint32_t occ_s[2][CAP / 32];
+   __shared__ int32_t agg_s[3][NR];
+   __shared__ uint32_t probe_s;
+   __shared__ uint32_t moved_s;
+   if (h->skip || h->no_empty || !h->ordered) return;
+   const uint64_t n = 1ull << sp, mask = n - 1;
+   const int lane = __lane_id();
+   const int wave = threadIdx.x / kWave;
+   if (threadIdx.x == 0) {
+      probe_s = 0;
+      moved_s = 0;
+   }
+   uint32_t max_probe = 0;
+   bool moved_any = false;
+   int par = 0;
+   // lane j: the start of tile t + j (unwrapped past the table's end)
+   const uint64_t kb = std::min<uint64_t>(std::min<uint64_t>(CAP / tile + 2, n_tiles + 1), kWave - 1);
+   auto starts = [&](uint32_t it) -> uint64_t {
+      uint64_t v = kSegNoPos;
+      if (it < n_tiles && static_cast<uint64_t>(lane) <= kb) {
+         const uint64_t u = static_cast<uint64_t>(it) + static_cast<uint64_t>(lane);
+         v = bnd[u % n_tiles];
+         if (v != kSegNoPos) v += n * (u / n_tiles);
+      }
+      return v;
+   };
+   uint64_t nxt = starts(blockIdx.x);
+   for (uint32_t it = blockIdx.x; it < n_tiles; it += gridDim.x) {
+      const uint64_t cur = nxt;
+      nxt = starts(it + gridDim.x);
+      const uint64_t b0 = readlane64(cur, 0);
+      const uint64_t later = __ballot(lane >= 1 && cur != kSegNoPos);
+      if (b0 == kSegNoPos) continue; // no EMPTY in this tile: another tile's segment
+      const uint64_t b1 = later ? readlane64(cur, first_bit(later)) : kSegNoPos;
+      if (b1 == kSegNoPos || b1 - b0 > CAP) {
+         if (threadIdx.x == 0) over[atomic_add_relaxed(&h->n_over, 1u)] = it;
+         continue;
+      }
+      const uint32_t len = static_cast<uint32_t>(b1 - b0);
+      uint32_t* occ = occ_s[par];
+      par ^= 1;
+      for (uint32_t i = threadIdx.x; i < (len + 31) / 32; i += BS) occ[i] = 0;
+      K kk[E], vv[E];
+      int32_t rr[E];
+      Place loc[E];
+      bool bad = false;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+         const uint32_t q = static_cast<uint32_t>(e * NW + wave);
+         const uint32_t i = q * kWave + static_cast<uint32_t>(lane);
+         kk[e] = empty_key<K>();
+         rr[e] = -1;
+         Place x{kNegPos, 0, kNegPos};
+         if (q * kWave < len) {
+            if (i < len) {
+               const Slot<K> s = b[(b0 + i) & mask];
+               kk[e] = s.key;
+               vv[e] = s.val;
+               if (s.key != empty_key<K>() && s.key != tomb_key<K>()) {
+                  const uint64_t rh = (fib_home(s.key, sp) - b0) & mask;
+                  bad = bad || rh > i;
+                  rr[e] = static_cast<int32_t>(std::min<uint64_t>(rh, i));
+                  x = Place{rr[e], 1, rr[e]};
+               }
+            }
+            x = place_wave_scan(x);
+         }
+         if (lane == kWave - 1) {
+            agg_s[0][q] = x.a;
+            agg_s[1][q] = x.c;
+            agg_s[2][q] = x.m;
+         }
+         loc[e] = x;
+      }
+      __syncthreads();
+      // the rows' aggregates scanned by every wave (lane q: row q)
+      Place g{kNegPos, 0, kNegPos};
+      if (lane < NR) g = Place{agg_s[0][lane], agg_s[1][lane], agg_s[2][lane]};
+      g = place_wave_scan(g);
+      int32_t pos[E];
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+         const int q = e * NW + wave;
+         pos[e] = -1;
+         if (rr[e] >= 0) {
+            Place pre{kNegPos, 0, kNegPos};
+            if (q > 0)
+               pre = Place{__builtin_amdgcn_readlane(g.a, q - 1), __builtin_amdgcn_readlane(g.c, q - 1),
+                           __builtin_amdgcn_readlane(g.m, q - 1)};
+            const Place f = place_join(pre, loc[e]);
+            pos[e] = f.a;
+            bad = bad || f.m != rr[e] || f.a >= static_cast<int32_t>(len);
+         }
+      }
+      if (__syncthreads_or(bad)) { // the sorting pass takes it
+         if (threadIdx.x == 0) over[atomic_add_relaxed(&h->n_over, 1u)] = it;
+         continue;
+      }
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+         if (rr[e] >= 0) {
+            const uint32_t i = static_cast<uint32_t>((e * NW + wave) * kWave + lane);
+            const uint32_t p = static_cast<uint32_t>(pos[e]);
+            atomicOr(&occ[p >> 5], 1u << (p & 31));
+            if (p != i) {
+               Slot<K> s;
+               s.key = kk[e];
+               s.val = vv[e];
+               b[(b0 + p) & mask] = s;
+            }
+            max_probe = std::max<uint32_t>(max_probe, p - static_cast<uint32_t>(rr[e]) + 1);
+            moved_any = moved_any || static_cast<uint32_t>(rr[e]) != i;
+         }
+      }
+      __syncthreads();
+      // slots left without a key become EMPTY (TOMBSTONEs, and slots keys left)
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+         const uint32_t i = static_cast<uint32_t>((e * NW + wave) * kWave + lane);
+         if (i < len && kk[e] != empty_key<K>() && !((occ[i >> 5] >> (i & 31)) & 1u)) {
             Slot<K> s;
             s.key = empty_key<K>();
             s.val = 0;
@@ -618,9 +815,10 @@ __global__ void k_seg_clean_end(hn_info* info, SegCleanHeader* h) {
 }
 
 // First pass: segments of up to 2,048 slots, 512 threads x 4 (the register
-// arrays of 8 elements per thread held two waves per SIMD for u64 keys);
-// second pass: up to 8,192 slots.
+// arrays of 8 elements per thread held two waves per SIMD for u64 keys), the
+// ordered kernel or the sorting one; second pass (sorting): up to 8,192 slots.
 constexpr int kSegBS1 = 512, kSegE1 = 4, kSegBS2 = 1024, kSegE2 = 8;
+constexpr int kOrdBS = 512, kOrdE = 4;
 constexpr size_t kSegPoolMin = 64 << 10;
 
 // The segment passes.  `scratch` holds seg_clean_fixed_bytes(sp) bytes;
@@ -641,7 +839,11 @@ int seg_clean(void* buckets, hn_info* info, int sp, void* scratch, void* pool, s
                       0, s, b, tile, tiles, h, bnd);
    // as many workgroups as stay resident, each looping over segments: the
    // header then sees one update per workgroup, not one per segment
-   static std::atomic<int> occ1{0};
+   static std::atomic<int> occ0{0}, occ1{0};
+   const int per_cu0 = resident_blocks_per_cu(k_seg_clean_ord<K, kOrdBS, kOrdE>, kOrdBS, occ0);
+   const uint32_t grid0 = static_cast<uint32_t>(std::min<uint64_t>(tiles, static_cast<uint64_t>(cu_count()) * per_cu0));
+   hipLaunchKernelGGL((k_seg_clean_ord<K, kOrdBS, kOrdE>), dim3(grid0), dim3(kOrdBS), 0, s, b, sp, h, bnd, over, tile,
+                      static_cast<uint32_t>(tiles));
    const int per_cu = resident_blocks_per_cu(k_seg_clean<K, kSegBS1, kSegE1>, kSegBS1, occ1);
    const uint32_t grid1 = static_cast<uint32_t>(std::min<uint64_t>(tiles, static_cast<uint64_t>(cu_count()) * per_cu));
    hipLaunchKernelGGL((k_seg_clean<K, kSegBS1, kSegE1>), dim3(grid1), dim3(kSegBS1), 0, s, b, sp, h, bnd, over, long_list,

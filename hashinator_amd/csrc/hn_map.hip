@@ -20,15 +20,6 @@
 
 namespace hn {
 
-// The rebuild path of clean_tombstones (hn_build.hip): every live slot goes
-// through the bulk build into the same table, in Robin Hood order.  With
-// `two`, the scratch that also holds a second pair array (one partition pass
-// fewer at 2^21..2^29 slots).
-template <typename K>
-size_t rebuild_scratch_bytes(int sp, bool two);
-template <typename K>
-int rebuild(void* buckets, hn_info* info, int sp, void* scratch, size_t bytes, hipStream_t s);
-
 // The segment pass of clean_tombstones (hn_clean.hip) and the reorder that
 // follows the region path when the segment pass hands a table to it
 // (hn_reorder.hip).

@@ -41,8 +41,10 @@ typedef struct ihipStream_t* hn_stream_t; /* == hipStream_t */
  * reserved0 is zero between calls (hn_info_init sets it).  reserved1
  * holds flags: bit 0 is transient (zero between calls), bit 1
  * (HN_INFO_ORDERED) says the table is in Robin Hood order -- set by
- * hn_reorder_*, cleared by every insert, clean and rehash; a caller that
- * writes bucket memory itself must clear it. */
+ * hn_reorder_*, the bulk build and a complete clean, cleared by every
+ * probing insert, reset and rehash; a caller that writes bucket memory
+ * itself must clear it (lookups stop early on its word, and the clean
+ * skips its sort for segments whose homes it finds in order). */
 typedef struct hn_info {
    int32_t size_power;
    int32_t reserved0;
@@ -190,11 +192,15 @@ HN_API int hn_reset_u64(const void* src_pairs, void* buckets, hn_info* info, siz
 
 /* ---------------------------------------------------------------- clean / rehash */
 
-/* Scratch needed by hn_clean_tombstones for a table of 2^size_power slots
- * when the caller can spare `budget_bytes` (0 = no limit).  The pass runs in
- * regions of at most (scratch - header) / slot_bytes slots, so any scratch of
- * at least hn_clean_tombstones_min_scratch() bytes works; more scratch means
- * fewer launches. */
+/* Scratch for hn_clean_tombstones on a table of 2^size_power slots when the
+ * caller can spare `budget_bytes`.  0: the default, the segment pass's fixed
+ * part (a few MiB at 2^28 slots) plus a pool for segments of up to 1/32 of
+ * the table (at least 1 MiB); 294 MiB at 2^28 u64 slots.  SIZE_MAX: enough
+ * for any table, one with no EMPTY slot or one segment spanning it included
+ * (36 bytes per slot for u64 keys, 28 for u32).  A budget of at least the fixed part plus 64 KiB
+ * is returned as it is: the pool takes the rest.  Below that, the region
+ * clean's size within the budget, at least hn_clean_tombstones_min_scratch()
+ * bytes. */
 HN_API size_t hn_clean_tombstones_scratch_bytes_u32(int size_power, size_t budget_bytes);
 HN_API size_t hn_clean_tombstones_scratch_bytes_u64(int size_power, size_t budget_bytes);
 HN_API size_t hn_clean_tombstones_min_scratch(void);
@@ -204,16 +210,20 @@ HN_API size_t hn_clean_tombstones_min_scratch(void);
  * become EMPTY and the live elements are placed again: tombstone_counter = 0,
  * fill is unchanged, overflow = max(64, nextOverflow(longest probe)), and the
  * occupied-slot set is the one a fresh insert of the live set gives.
- * With scratch of hn_clean_tombstones_scratch_bytes_*(size_power, 0) bytes
- * (about twice the table's size) the live slots are rebuilt through the bulk
- * build (hn_build_*), leaving the table in Robin Hood order
- * (HN_INFO_ORDERED); about the table's size also rebuilds, with one more
- * partition pass at 2^21..2^29 slots (the scratch_bytes query with a budget
- * between the two returns the smaller).  With less, the table is processed region by region
- * starting after an EMPTY slot: every live element not in its home slot is
- * lifted out and reinserted, so scratch is bounded by the region size; a
- * table with no EMPTY slot at all then needs scratch for the whole table,
- * and with less, err = FAIL and the table is left unchanged. */
+ * Segment pass (scratch of at least the fixed part plus 64 KiB): the table
+ * splits at its EMPTY slots into segments, which no live element's probe
+ * crosses; each is rewritten in place in Robin Hood order (home slot, ties by
+ * key), in LDS up to 8192 slots and through the pool above that, and the
+ * table ends HN_INFO_ORDERED.  A segment longer than the pool holds is left
+ * as it was: its tombstones stay counted in tombstone_counter, err = FAIL
+ * (when it held any), the others are cleaned and HN_INFO_ORDERED is not
+ * set.  A table with no EMPTY slot is cleaned by the region clean over the
+ * whole table and then reordered when the pool holds the whole table
+ * (SIZE_MAX), and left unchanged with err = FAIL otherwise.  Region clean
+ * (less scratch): region by region from an EMPTY slot, every live element
+ * not in its home slot is lifted out and reinserted; bounded by the region
+ * size, not ordered; a table with no EMPTY slot gets err = FAIL unless one
+ * region holds it. */
 HN_API int hn_clean_tombstones_u32(void* buckets, hn_info* info, int size_power, void* scratch, size_t scratch_bytes,
                             hn_stream_t s);
 HN_API int hn_clean_tombstones_u64(void* buckets, hn_info* info, int size_power, void* scratch, size_t scratch_bytes,

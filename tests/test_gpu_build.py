@@ -204,11 +204,15 @@ def test_build_needs_no_initialised_table(gpu, bits, case):
         assert int(inf[1]) == keys.size and (int(inf[4]) & 0xFFFFFFFF) == 0
 
 
-def _tombstoned_map(bits, sp, lf, seed=0):
+def _tombstoned_map(bits, sp, lf, seed=0, ordered=False):
+    """A probing insert at load factor lf (the bulk build, in Robin Hood
+    order, when `ordered`), then a quarter of the keys erased; with the
+    oracle's model of the image."""
     keys = keyset(int(lf * (1 << sp)), bits, seed=sp + bits + seed)
     vals = (keys * NDT[bits](9)).astype(NDT[bits])
-    hm = Hashmap(sp, bits, auto_build=0, auto_reorder=0)
+    hm = Hashmap(sp, bits, auto_build=1.0 / 16 if ordered else 0, auto_reorder=0, build_min_keys=1)
     hm.insert(keys, vals, targetLF=1.0)
+    assert hm.ordered() == ordered
     er = keys[np.random.default_rng(sp).choice(keys.size, keys.size // 4, replace=False)]
     hm.erase(er)
     i = hm.info()
@@ -232,16 +236,19 @@ def _check_clean_exact(hm, m, bits):
     assert i.err == 0 and (i.reserved1 & _lib.HN_INFO_ORDERED)
 
 
+@pytest.mark.parametrize("ordered", [False, True])
 @pytest.mark.parametrize("bits", [32, 64])
 @pytest.mark.parametrize("sp,lf", [(10, 0.9), (14, 0.9), (16, 0.97), (18, 0.98), (20, 0.95), (21, 0.85), (22, 0.9),
                                    (12, 0.5)])
-def test_clean_image_exact(gpu, bits, sp, lf):
+def test_clean_image_exact(gpu, bits, sp, lf, ordered):
     """clean_tombstones by EMPTY-bounded segments: the image is the oracle's
-    clean followed by its reorder, bit for bit.  LF 0.97 and 0.98 have
-    clusters longer than the 8,192 slots LDS holds (the pass over global
-    memory; at 0.98 the default pool runs short and the map retries with a
-    larger one)."""
-    hm, m = _tombstoned_map(bits, sp, lf)
+    clean followed by its reorder, bit for bit, from a table in arrival order
+    (the sorting pass) and from one in Robin Hood order (the scan without a
+    sort).  LF 0.97 and 0.98 have clusters longer than the 8,192 slots LDS
+    holds (the pass over global memory; at 0.98 the default pool runs short
+    and the map retries with a larger one)."""
+    hm, m = _tombstoned_map(bits, sp, lf, ordered=ordered)
+    assert hm.ordered() == ordered
     hm.clean_tombstones()
     _check_clean_exact(hm, m, bits)
     # no tombstones: a no-op, Info untouched
@@ -262,8 +269,9 @@ def _skewed_keys(bits, sp, n, lo, width, seed):
     return np.concatenate(out)[:n]
 
 
+@pytest.mark.parametrize("ordered", [False, True])
 @pytest.mark.parametrize("bits", [32, 64])
-def test_clean_pool_too_small_then_enough(gpu, bits):
+def test_clean_pool_too_small_then_enough(gpu, bits, ordered):
     """A cluster of about 12,000 slots (keys with homes in a 64-slot window)
     is longer than LDS holds.  Through the C-ABI with a 64 KiB pool it does
     not fit: that segment is left as it was (same live set, every key
@@ -277,6 +285,10 @@ def test_clean_pool_too_small_then_enough(gpu, bits):
     vals = (allk * NDT[bits](3)).astype(NDT[bits])
     hm = Hashmap(sp, bits, auto_build=0, auto_reorder=0)
     hm.insert(allk, vals, targetLF=1.0)
+    if ordered:  # a complete clean leaves Robin Hood order at any cluster length
+        hm.erase(allk[:64])
+        hm.clean_tombstones()
+        assert hm.ordered()
     hm.erase(allk[np.random.default_rng(1).choice(allk.size, allk.size // 4, replace=False)])
     i = hm.info()
     m = O.GpuModel(sp, bits, buckets=hm.host_buckets().reshape(-1).copy(),
@@ -305,6 +317,38 @@ def test_clean_pool_too_small_then_enough(gpu, bits):
     assert clean(C.c_void_p(hm.bucket_ptr()), C.c_void_p(hm.info_ptr()), sp, C.c_void_p(sc.data_ptr()), big, s) == 0
     torch.cuda.synchronize()
     _check_clean_exact(hm, m, bits)
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_clean_order_flag_without_order(gpu, bits):
+    """A table in arrival order whose Info claims Robin Hood order: the scan
+    pass checks every segment's order and hands the segments that break it
+    to the sorting pass, so the result keeps the live set, puts every key on
+    an unbroken probe path, occupies exactly the slots the oracle's clean
+    occupies, and lists each cluster in home order."""
+    sp, lf = 18, 0.9
+    hm, m = _tombstoned_map(bits, sp, lf, seed=3)
+    live0 = live_pairs(hm.host_buckets(), bits)
+    hm._info.view(torch.int32)[9] = _lib.HN_INFO_ORDERED  # reserved1: a claim the table does not meet
+    hm.clean_tombstones()
+    m.clean_tombstones()
+    assert m.reorder() == 1
+    img, ref = hm.host_buckets(), m.b.reshape(-1, 2)
+    assert np.array_equal(live_pairs(img, bits), live0) and reachable_all(img, bits, sp)
+    assert np.array_equal(occupancy(img, bits), occupancy(ref, bits))
+    i = hm.info()
+    assert (i.fill, i.tombstone_counter, i.err) == (m.info.fill, 0, 0)
+    # within every run of occupied slots the homes (relative to the run's start) do not decrease
+    n = 1 << sp
+    k = img[:, 0]
+    occ = k != np.iinfo(NDT[bits]).max
+    start = np.nonzero(~occ)[0][0]
+    idx = (np.arange(n) + start) % n
+    home = (O.fibhash(k[idx], sp, bits).astype(np.int64) - start) % n
+    run = np.cumsum(~occ[idx])  # run id: EMPTY slots so far
+    o = occ[idx]
+    h, r = home[o], run[o]
+    assert np.all((np.diff(h) >= 0) | (np.diff(r) != 0))
 
 
 @pytest.mark.parametrize("bits", [32, 64])
