@@ -817,8 +817,10 @@ __global__ void k_seg_clean_end(hn_info* info, SegCleanHeader* h) {
 // First pass: segments of up to 2,048 slots, 512 threads x 4 (the register
 // arrays of 8 elements per thread held two waves per SIMD for u64 keys), the
 // ordered kernel or the sorting one; second pass (sorting): up to 8,192 slots.
+// The ordered kernel holds 2 slots per thread (58 VGPRs for u64 keys, 8
+// waves per SIMD).
 constexpr int kSegBS1 = 512, kSegE1 = 4, kSegBS2 = 1024, kSegE2 = 8;
-constexpr int kOrdBS = 512, kOrdE = 4;
+constexpr int kOrdBS = 1024, kOrdE = 2;
 constexpr size_t kSegPoolMin = 64 << 10;
 
 // The segment passes.  `scratch` holds seg_clean_fixed_bytes(sp) bytes;
