@@ -85,10 +85,36 @@ def read_traffic(name):
     return None
 
 
+def host_cpu():
+    """nproc and the CPU model of the host this runs on."""
+    model = "unknown"
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model}
+
+
+def cpu_threads():
+    """Worker threads for the all-cores CPU leg: the host's cores, at most 16
+    (the CPU share of one GPU on the measurement hosts)."""
+    return max(1, min(os.cpu_count() or 1, 16))
+
+
 def cpu_baseline(sp_sample=24, alpha=0.9, seed=0x48415348494E4154 + 2):
-    """The reference's host-only algorithm (serial _at insert, const find) on
-    the same workload at 2^sp_sample slots, one core.  Uses the reference's own
-    CPU-only build when it was built into oracle/_ref, else the C restatement."""
+    """The reference's host-only algorithm on the large-map workload at
+    2^sp_sample slots, one core, stage by stage: serial _at insert
+    (hashinator.h:340-403), const find of as many present as absent keys
+    (742-768), erase(key) of a quarter of the keys (842-850; its cleanup
+    rehashes only above 25% tombstones per slot, and CPU-only inserts leave
+    the overflow at 32, 631-639), and the CPU-only clean, a rehash at the
+    same size (635-638); then compaction and scan of 2^sp_sample elements by
+    the C restatement (std::copy_if / std::exclusive_scan semantics).  The
+    map stages use the reference's own CPU-only build when it was built into
+    oracle/_ref, else the C restatement."""
     import oracle as O
     from hashinator_amd import workloads as Wl
 
@@ -97,6 +123,7 @@ def cpu_baseline(sp_sample=24, alpha=0.9, seed=0x48415348494E4154 + 2):
     vals = Wl.values(keys, seed)
     absent = Wl.absent64(n, seed, n)
     q = Wl.shuffled(np.concatenate([keys, absent]), seed + 1)
+    er = keys[np.random.default_rng(12345).choice(n, n // 4, replace=False)]
     kind = "reference" if O.ref_available() else "port"
     m = O.ref_map(sp_sample, 64) if kind == "reference" else O.CpuMap(sp_sample, 64)
     t0 = time.perf_counter()
@@ -105,7 +132,32 @@ def cpu_baseline(sp_sample=24, alpha=0.9, seed=0x48415348494E4154 + 2):
     v, h = m.find(q)
     t2 = time.perf_counter()
     assert int(h.sum()) == n
+    erased = m.erase(er)
+    t3 = time.perf_counter()
+    assert erased == n // 4 and m.info().size_power == sp_sample
+    m.rehash(sp_sample)
+    t4 = time.perf_counter()
+    i = m.info()
+    assert i.fill == n - n // 4 and i.tombstones == 0
+    m.close()
+    # compaction: pairs with Bernoulli(0.3) keep in bit 0 of the value; scan: u32
+    ne = 1 << sp_sample
+    rng = np.random.default_rng(seed)
+    pairs = np.empty((ne, 2), dtype=np.uint64)
+    pairs[:, 0] = np.arange(ne, dtype=np.uint64)
+    pairs[:, 1] = (rng.integers(0, 2**63, size=ne, dtype=np.uint64) << np.uint64(1)) | \
+        (rng.random(ne) < 0.3).astype(np.uint64)
+    pred = O.make_pred(O.PRED_MASKEQ, field=1, mask=1, value=1, width=64)
+    t5 = time.perf_counter()
+    kept = O.copy_if_pairs(pairs, pred, 64)
+    t6 = time.perf_counter()
+    x = rng.integers(0, 16, size=ne, dtype=np.uint32)
+    t7 = time.perf_counter()
+    y = O.exclusive_scan(x)
+    t8 = time.perf_counter()
+    assert y[-1] == np.uint32(int(x[:-1].sum()) & 0xFFFFFFFF)
     ops = n + q.size
+    cb_bytes = ne * 16 + kept.shape[0] * 16
     return {
         "value": round(ops / (t2 - t0) / 1e6, 2),
         "unit": "Mops/s",
@@ -113,9 +165,56 @@ def cpu_baseline(sp_sample=24, alpha=0.9, seed=0x48415348494E4154 + 2):
         "kind": kind,
         "sample": (f"Hashmap<u64,u64>(sp={sp_sample}) at LF {alpha}: insert {n} keys (_at), then find {n} present + "
                    f"{n} absent keys (const find); same key distribution as the GPU workload at 1/{1 << (28 - sp_sample)} "
-                   f"scale; insert {n / (t1 - t0) / 1e6:.2f} Mops/s, find {q.size / (t2 - t1) / 1e6:.2f} Mops/s; "
-                   f"{t2 - t0:.1f} s of CPU time"),
+                   f"scale; {t2 - t0:.1f} s of CPU time (the stages below {t8 - t0:.1f} s)"),
+        "host": host_cpu(),
+        "stages": {
+            "insert_Mops": round(n / (t1 - t0) / 1e6, 2),
+            "retrieve_Mops": round(q.size / (t2 - t1) / 1e6, 2),
+            "erase_Mops": round((n // 4) / (t3 - t2) / 1e6, 2),
+            "clean_ms": round((t4 - t3) * 1e3, 1),
+            "clean_note": "CPU-only mode cleans tombstones by a rehash at the same size (hashinator.h:635-638)",
+            "copy_if_ms": round((t6 - t5) * 1e3, 1),
+            "copy_if_GBps": round(cb_bytes / (t6 - t5) / 1e9, 2),
+            "scan_ms": round((t8 - t7) * 1e3, 1),
+            "scan_GBps": round(2 * ne * 4 / (t8 - t7) / 1e9, 2),
+            "compaction_scan_kind": "port",
+            "compaction_scan_sample": f"copy_if of {ne} hash_pair<u64,u64> (keep 0.3), exclusive scan of {ne} u32",
+        },
     }
+
+
+def cpu_batched(wl):
+    """The 4,096-map workload on the host's cores: each map serial through the
+    reference's CPU-only build (insert, const find, erase(key) of the erase
+    list, clean as a rehash at the same size), maps spread over a thread pool
+    (ctypes releases the GIL for the calls), largest first."""
+    import oracle as O
+    from concurrent.futures import ThreadPoolExecutor
+
+    if not O.ref_available():
+        return None
+    sps, ins, vals, qry, ers = wl
+    order = sorted(range(len(sps)), key=lambda m: -int(sps[m]))
+    threads = cpu_threads()
+
+    def one(mi):
+        m = O.ref_map(int(sps[mi]), 32)
+        m.insert(ins[mi], vals[mi])
+        _, h = m.find(qry[mi])
+        m.erase(ers[mi])
+        m.rehash(int(sps[mi]))
+        ok = int(h.sum()) == ins[mi].size and m.info().fill == ins[mi].size - ers[mi].size
+        m.close()
+        return ok
+
+    O.ref()  # load the library before the clock starts
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        oks = list(ex.map(one, order))
+    dt = time.perf_counter() - t0
+    ops = sum(a.size for a in ins) + sum(a.size for a in qry) + sum(a.size for a in ers)
+    return {"Mops": round(ops / dt / 1e6, 2), "ms": round(dt * 1e3, 1), "threads": threads, "kind": "reference",
+            "ok": all(oks), "host": host_cpu()}
 
 
 def main():
@@ -334,7 +433,7 @@ def main():
     if not args.no_batched:
         del buckets, out, hit, q, dk, dv
         torch.cuda.empty_cache()
-        extra["batched"] = run_batched(torch)
+        extra["batched"] = run_batched(torch, with_cpu=not args.no_cpu)
     if not args.no_serving:
         torch.cuda.empty_cache()
         extra["serving"] = run_serving(torch)
@@ -370,7 +469,11 @@ def main():
         "input_gen_s": round(t_gen, 1),
     }
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline()
+        cb = cpu_baseline()
+        bc = extra.get("batched", {}).get("cpu_reference_all_cores")
+        if bc:
+            cb["stages"]["batched_4096_maps_all_cores"] = bc
+        line["cpu_baseline"] = cb
     print(json.dumps(line), flush=True)
 
 
@@ -617,7 +720,7 @@ def batched_only(torch):
             "tune": {"probe_width": 8, "block_size": 1024}, "batched_ms": times}
 
 
-def run_batched(torch):
+def run_batched(torch, with_cpu=False):
     """Many uneven maps: 4096 Hashmap<u32,u32> of 2^8..2^16 slots at LF 0.6
     (velocity-grid keys): insert -> retrieve (keys + as many absent cells)
     -> erase 20% -> clean_tombstones, batched in one persistent launch (eager
@@ -630,6 +733,7 @@ def run_batched(torch):
     t0 = time.perf_counter()
     wl = Wl.batched_workload()
     bm = BatchedMaps(*wl, key_bits=32)
+    cpu = cpu_batched(wl) if with_cpu else None
     gen_s = time.perf_counter() - t0
     n_ins, n_qry, n_ers = bm.counts()
     ops = n_ins + n_qry + n_ers
@@ -682,6 +786,7 @@ def run_batched(torch):
         "tune": {"probe_width": tune.probe_width, "block_size": tune.block_size},
         "tune_table_ms": {k: round(v, 3) for k, v in table.items()},
         "ok": bool(ok_per and ok_bat and ok_graph), "input_gen_s": round(gen_s, 1),
+        "cpu_reference_all_cores": cpu,
     }
 
 

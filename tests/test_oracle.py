@@ -502,3 +502,18 @@ def test_copy_if_and_scan_brute_force(n):
     assert np.array_equal(ex.astype(np.uint64), ref)
     f = (rng.random(n) < 0.3).astype(np.uint8)
     assert np.array_equal(O.copy_if_flags(x, f), x[f.astype(bool)])
+
+
+def test_cpu_baseline_stages_small():
+    """bench.py's CPU leg at a small size: every stage runs, checks itself
+    and reports."""
+    import bench
+    from hashinator_amd import workloads as Wl
+    r = bench.cpu_baseline(sp_sample=12)
+    st = r["stages"]
+    for k in ("insert_Mops", "retrieve_Mops", "erase_Mops", "clean_ms", "copy_if_GBps", "scan_GBps"):
+        assert st[k] > 0
+    assert r["host"]["nproc"] >= 1 and r["cores"] == 1
+    if O.ref_available():
+        b = bench.cpu_batched(Wl.batched_workload(n_maps=16, sp_hi=12))
+        assert b["ok"] and b["threads"] >= 1
