@@ -119,6 +119,12 @@ SIGNATURES = {
     "hn_rehash_build_scratch_bytes_u64": (S, [I, I]),
     "hn_rehash_build_u32": (I, [P, I, P, I, P, P, S, P]),
     "hn_rehash_build_u64": (I, [P, I, P, I, P, P, S, P]),
+    "hn_merge_scratch_bytes_u32": (S, [I, S]),
+    "hn_merge_scratch_bytes_u64": (S, [I, S]),
+    "hn_merge_u32": (I, [P, P, P, P, P, S, I, P, S, P]),
+    "hn_merge_u64": (I, [P, P, P, P, P, S, I, P, S, P]),
+    "hn_merge_pairs_u32": (I, [P, P, P, P, S, I, P, S, P]),
+    "hn_merge_pairs_u64": (I, [P, P, P, P, S, I, P, S, P]),
     "hn_compact_scratch_bytes": (S, [S]),
     "hn_copy_if_pairs_u32": (I, [P, S, PRED, P, P, P, S, P]),
     "hn_copy_if_pairs_u64": (I, [P, S, PRED, P, P, P, S, P]),

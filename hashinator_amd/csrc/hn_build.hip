@@ -46,6 +46,7 @@
 // is not empty (fill or tombstones nonzero) is answered with err = INVALID
 // and left untouched.
 #include <algorithm>
+#include <atomic>
 
 #include "hn_device.hpp"
 #include "hn_host.hpp"
@@ -197,15 +198,18 @@ __device__ __forceinline__ uint32_t region_of(K key, int sp, int rlog) {
    return static_cast<uint32_t>((fib_home(key, sp) & mask) >> rlog);
 }
 
-// Build from caller arrays into an empty table, or (rehash) from the live
-// slots of another, smaller table into a new one.
-enum BuildMode { BUILD_EMPTY = 0, BUILD_REHASH = 2 };
+// Build from caller arrays into an empty table, (rehash) from the live
+// slots of another, smaller table into a new one, or (merge) from caller
+// arrays and an ordered table into a second array of the table's size.
+enum BuildMode { BUILD_EMPTY = 0, BUILD_REHASH = 2, BUILD_MERGE = 3 };
 
 __global__ void k_build_begin(const hn_info* info, BuildHeader* h, int sp, uint64_t len, int mode) {
    if (threadIdx.x == 0) {
       const bool sp_ok = info->size_power == sp;
       if (mode == BUILD_EMPTY)
          h->go = (sp_ok && info->fill == 0 && info->tombstone_counter == 0) ? 1u : 0u;
+      else if (mode == BUILD_MERGE)
+         h->go = (sp_ok && info->tombstone_counter == 0 && (info->reserved1 & kInfoOrdered)) ? 1u : 0u;
       else
          h->go = 1u; // k_rehash_info has just set the Info up for the new table
       h->fail = 0;
@@ -214,7 +218,7 @@ __global__ void k_build_begin(const hn_info* info, BuildHeader* h, int sp, uint6
       h->n_distinct = 0;
       h->fb_fail = 0;
       h->fb_len = 0;
-      h->n = mode == BUILD_EMPTY ? len : 0; // a rehash counts the live slots in its first pass
+      h->n = mode == BUILD_REHASH ? 0 : len; // a rehash counts the live slots in its first pass
    }
 }
 
@@ -1017,12 +1021,16 @@ template <typename K>
 int launch_insert_pairs_dlen(const Slot<K>* pairs, Slot<K>* buckets, hn_info* info, uint64_t max_len,
                              const uint64_t* d_len, int32_t* fail_sink, hipStream_t s);
 
-// Everything after the first partition pass's input is chosen.  src0 picks
-// the first pass's input kind; n_in is its element count (the old table's
-// slot count for a rehash, whose first pass keeps live slots only).
+// The partition passes and the region starts: the input pairs end in S,
+// stably partitioned by region, with start[r] the first of region r.  src0
+// picks the first pass's input kind; n_in is its element count (the old
+// table's slot count for a rehash, whose first pass keeps live slots only).
+// Pass i writes S when (passes - 1 - i) is even, `alt` (memory of the same
+// size that is written later anyway: the output table) otherwise, so the
+// last pass lands in S.
 template <typename K>
-int build_core(const BuildPlan& P, int mode, int src0, PartIn<K> in0, uint64_t n_in, void* buckets, hn_info* info,
-               void* scratch, hipStream_t s) {
+int partition_regions(const BuildPlan& P, int src0, PartIn<K> in0, uint64_t n_in, Slot<K>* alt, void* scratch,
+                      hipStream_t s) {
    char* base = static_cast<char*>(scratch);
    BuildHeader* h = reinterpret_cast<BuildHeader*>(base);
    Slot<K>* S = reinterpret_cast<Slot<K>*>(base + P.off_pairs);
@@ -1036,12 +1044,6 @@ int build_core(const BuildPlan& P, int mode, int src0, PartIn<K> in0, uint64_t n
    lb.carry = reinterpret_cast<int64_t*>(base + P.off_carry);
    lb.mark = reinterpret_cast<uint32_t*>(base + P.off_mark);
    lb.list = reinterpret_cast<uint32_t*>(base + P.off_list);
-   Slot<K>* T = static_cast<Slot<K>*>(buckets);
-
-   hipLaunchKernelGGL(k_build_begin, dim3(1), dim3(64), 0, s, info, h, P.sp, static_cast<uint64_t>(P.n), mode);
-   // Pass i writes S when (passes - 1 - i) is even, the table's memory
-   // otherwise, so the last pass lands in S.
-   Slot<K>* const alt = T;
    const Slot<K>* src = nullptr;
    for (int pass = 0; pass < P.passes; ++pass) {
       Slot<K>* dst = ((P.passes - 1 - pass) % 2 == 0) ? S : alt;
@@ -1070,6 +1072,29 @@ int build_core(const BuildPlan& P, int mode, int src0, PartIn<K> in0, uint64_t n
    }
    hipLaunchKernelGGL(k_region_starts<K>, dim3(stream_blocks(P.NR + 1, 256)), dim3(256), 0, s, S, start, lb, P.NR,
                       P.sp, P.rlog, h);
+   return static_cast<int>(hipGetLastError());
+}
+
+// Everything after the first partition pass's input is chosen (see
+// partition_regions for src0 and n_in).
+template <typename K>
+int build_core(const BuildPlan& P, int mode, int src0, PartIn<K> in0, uint64_t n_in, void* buckets, hn_info* info,
+               void* scratch, hipStream_t s) {
+   char* base = static_cast<char*>(scratch);
+   BuildHeader* h = reinterpret_cast<BuildHeader*>(base);
+   Slot<K>* S = reinterpret_cast<Slot<K>*>(base + P.off_pairs);
+   uint64_t* start = reinterpret_cast<uint64_t*>(base + P.off_start);
+   RegionLB lb;
+   lb.word = reinterpret_cast<uint64_t*>(base + P.off_word);
+   lb.spec = reinterpret_cast<int64_t*>(base + P.off_spec);
+   lb.carry = reinterpret_cast<int64_t*>(base + P.off_carry);
+   lb.mark = reinterpret_cast<uint32_t*>(base + P.off_mark);
+   lb.list = reinterpret_cast<uint32_t*>(base + P.off_list);
+   Slot<K>* T = static_cast<Slot<K>*>(buckets);
+
+   hipLaunchKernelGGL(k_build_begin, dim3(1), dim3(64), 0, s, info, h, P.sp, static_cast<uint64_t>(P.n), mode);
+   int rc = partition_regions<K>(P, src0, in0, n_in, T, scratch, s);
+   if (rc) return rc;
    const uint32_t chunk = region_chunk(P.NR);
    hipLaunchKernelGGL(k_region_build<K>, dim3(static_cast<uint32_t>((P.NR + chunk - 1) / chunk)), dim3(kRegionBS), 0,
                       s, S, start, lb, T, P.sp, P.rlog, P.NR, chunk, h);
@@ -1083,7 +1108,7 @@ int build_core(const BuildPlan& P, int mode, int src0, PartIn<K> in0, uint64_t n
    const uint32_t gfix = static_cast<uint32_t>(std::min<uint64_t>(P.NR, static_cast<uint64_t>(cu_count()) * 2));
    hipLaunchKernelGGL(k_region_fix<K>, dim3(gfix), dim3(kRegionBS), 0, s, S, start, lb, T, P.sp, P.rlog, h);
    hipLaunchKernelGGL(k_build_fallback_prep<K>, dim3(stream_blocks(P.N, 256)), dim3(256), 0, s, T, P.N, info, h);
-   int rc = launch_insert_pairs_dlen<K>(S, T, info, P.n, &h->fb_len, &h->fb_fail, s);
+   rc = launch_insert_pairs_dlen<K>(S, T, info, P.n, &h->fb_len, &h->fb_fail, s);
    if (rc) return rc;
    hipLaunchKernelGGL(k_build_end, dim3(1), dim3(64), 0, s, info, h, mode);
    return static_cast<int>(hipGetLastError());
@@ -1143,6 +1168,537 @@ int rehash_build(const void* old_b, int old_sp, void* new_b, int new_sp, hn_info
 }
 
 
+// ------------------------------------------------------------------ merge insert
+//
+// Hasher::insert (hashers.h:55-70, kernels_AMD.h insert_kernel) into a table
+// that already holds keys, followed by the reorder: the image as a function
+// of the live set, written into a second bucket array of the same size.  The
+// old table must be free of TOMBSTONEs and in Robin Hood order; then its live
+// keys stand in slot order sorted by (home, key), unwrapped from an EMPTY
+// slot, and the keys whose home lies in region r fill one run of slots
+// [s_r, s_{r+1}): s_r is the first slot at or after rR that is EMPTY or holds
+// a key whose home is at or after rR (k_merge_bounds).  The new pairs take
+// the build's partition passes.  Then one workgroup per region, in ticket
+// order (k_merge_region), loads both, sorts the union by (home offset, key)
+// with a home histogram in LDS, keeps only the newest copy of a key (old
+// before new, new in batch order: an insert of a present key overwrites its
+// value, a key repeated in the batch keeps its last value) and places it:
+// p_j = j + max(v_j, carry), v the prefix max of (home offset - j) and carry
+// the slots the earlier regions spill into this one, passed along by a
+// decoupled look-back over the maps x -> max(A, x + C) (a region: A the
+// position of its last key with no carry, C its key count).  A region
+// writes its span [carry, max(R, last + 1)) of the output, keys and EMPTY
+// slots, so the spans tile the table, and positions past the end wrap into
+// region 0's carry-in slots, which region 0 does not write.  Region 0's
+// carry comes from that wrap, known only at the end, so region 0 is parked in
+// scratch and placed by k_merge_end.  The call ends with err = INVALID (the
+// old table untouched, the rest of Info too, the output unspecified) when
+// the wrap would change what region 0 spills into region 1, a region
+// exceeds its LDS capacity (3,072 old slots or 1,536 new pairs), the old
+// table is not in order (a key outside its region's run), the union would
+// fill the table, or the table has TOMBSTONEs or no order flag.
+constexpr int kMergeBS = 512;
+constexpr int kMergeEO = 6; // old slots per thread
+constexpr int kMergeEN = 3; // new pairs per thread
+constexpr uint32_t kMergeCapO = kMergeBS * kMergeEO;
+constexpr uint32_t kMergeCapN = kMergeBS * kMergeEN;
+constexpr uint32_t kMergeCap = kMergeCapO + kMergeCapN;
+constexpr uint32_t kMergeSpanWords = (kRegionMax + kMergeCap + 31) / 32;
+static_assert(kMergeCap < 0x8000, "element ids leave the top bit of a 16-bit order entry free");
+
+struct MergeHeader {
+   uint32_t ticket;    // next region
+   uint32_t fail;      // see above
+   uint32_t max_probe; // longest probe of the keys placed by the regions
+   uint32_t n0;        // keys of region 0, parked
+   uint32_t pad[60];
+};
+static_assert(sizeof(MergeHeader) == 256, "header is one 256-byte line");
+
+// A parked key of region 0: its prefix-max term and home offset.
+template <typename K>
+struct MergePark {
+   K key, val;
+   int32_t v;
+   uint32_t hr;
+};
+
+// Region state for the look-back: 2 state bits, A + 1 and C in 31 bits each
+// (table size power at most 30).
+constexpr uint64_t kMgAgg = 1ull << 62, kMgInc = 2ull << 62;
+__device__ __forceinline__ uint64_t mg_pack(uint64_t st, RegionSum g) {
+   return st | (static_cast<uint64_t>(g.a + 1) << 31) | static_cast<uint64_t>(g.b);
+}
+__device__ __forceinline__ RegionSum mg_unpack(uint64_t w) {
+   RegionSum g;
+   g.a = static_cast<int64_t>((w >> 31) & 0x7FFFFFFFull) - 1;
+   g.b = static_cast<int64_t>(w & 0x7FFFFFFFull);
+   return g;
+}
+// No keys: (A, C) = (-1, 0) acts as the identity for region maps (A >= C - 1).
+constexpr RegionSum kMgNone = {-1, 0};
+
+__global__ void k_merge_begin(MergeHeader* mh) {
+   if (threadIdx.x == 0) {
+      mh->ticket = 0;
+      mh->fail = 0;
+      mh->max_probe = 0;
+      mh->n0 = 0;
+   }
+}
+
+// sb[r] = s_r (unwrapped, < rR + kMergeCapO), sb[NR] = N + s_0; clears the
+// region states.  One wave per region.
+template <typename K>
+__global__ __launch_bounds__(256) void k_merge_bounds(const Slot<K>* ob, int sp, int rlog, uint64_t NR, uint64_t* sb,
+                                                      uint64_t* status, MergeHeader* mh, const BuildHeader* h) {
+   if (!h->go) return;
+   const int lane = __lane_id();
+   const uint64_t N = 1ull << sp, mask = N - 1;
+   const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (blockDim.x / kWave);
+   for (uint64_t r = static_cast<uint64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave; r < NR;
+        r += waves) {
+      const int64_t lo = static_cast<int64_t>(r << rlog);
+      int64_t found = -1;
+      for (uint32_t o = 0; o < kMergeCapO; o += kWave) {
+         const int64_t p = lo + o + lane;
+         const K k = ob[static_cast<uint64_t>(p) & mask].key;
+         bool stop = k == empty_key<K>();
+         if (!stop) {
+            const int64_t d = static_cast<int64_t>((static_cast<uint64_t>(p) - fib_home(k, sp)) & mask);
+            stop = p - d >= lo;
+         }
+         const uint64_t m = __ballot(stop);
+         if (m) {
+            found = lo + o + first_bit(m);
+            break;
+         }
+      }
+      if (lane == 0) {
+         if (found < 0) {
+            __hip_atomic_store(&mh->fail, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            found = lo;
+         }
+         sb[r] = static_cast<uint64_t>(found);
+         if (r == 0) sb[NR] = N + static_cast<uint64_t>(found);
+         status[r] = 0;
+      }
+   }
+}
+
+// Inclusive prefix max of x[0, m) in place by the whole workgroup.
+template <int BS>
+__device__ void lds_incl_max16(int16_t* x, uint32_t m, int32_t* wmax) {
+   const int lane = __lane_id(), wave = threadIdx.x / kWave;
+   const uint32_t per = (m + BS - 1) / BS;
+   const uint32_t c0 = std::min<uint32_t>(threadIdx.x * per, m), c1 = std::min<uint32_t>(c0 + per, m);
+   int32_t acc = INT32_MIN;
+   for (uint32_t i = c0; i < c1; ++i) acc = std::max<int32_t>(acc, x[i]);
+   int32_t inc = acc;
+#pragma unroll
+   for (int o = 1; o < kWave; o <<= 1) {
+      const int32_t y = __shfl_up(inc, o, kWave);
+      if (lane >= o) inc = std::max(inc, y);
+   }
+   if (lane == kWave - 1) wmax[wave] = inc;
+   __syncthreads();
+   int32_t run = __shfl_up(inc, 1, kWave);
+   if (lane == 0) run = INT32_MIN;
+   for (int w = 0; w < wave; ++w) run = std::max(run, wmax[w]);
+   for (uint32_t i = c0; i < c1; ++i) {
+      run = std::max<int32_t>(run, x[i]);
+      x[i] = static_cast<int16_t>(run);
+   }
+   __syncthreads();
+}
+
+template <typename K>
+struct MergeLds {
+   K key[kMergeCap];        // by element id: old slot offset, or kMergeCapO + new index
+   uint16_t order[kMergeCap]; // ids by home offset (top bit: a dropped copy)
+   int16_t v[kMergeCap];    // prefix-max terms by sorted index
+   uint32_t cnt[kRegionMax + 1];
+   uint32_t occ[kMergeSpanWords];
+   uint32_t wsum[kMergeBS / kWave];
+   int32_t wmax[kMergeBS / kWave];
+   uint32_t ticket;
+   uint32_t probe;
+   int64_t xin;
+};
+
+// Rank of one element in its home group order[g0, g1): keys below it, and
+// whether a newer copy of its key (a larger id) drops it.
+template <typename K>
+__device__ __forceinline__ uint32_t merge_rank(const MergeLds<K>& L, uint32_t id, uint32_t g0, uint32_t g1, bool& drop,
+                                               bool& rep) {
+   const K key = L.key[id];
+   uint32_t rank = 0;
+   for (uint32_t k = g0; k < g1; ++k) {
+      const uint32_t id2 = L.order[k];
+      const K k2 = L.key[id2];
+      rank += k2 < key ? 1u : 0u;
+      if (k2 == key && id2 != id) {
+         rep = true;
+         drop = drop || id2 > id;
+      }
+   }
+   return rank;
+}
+// The same over the copies that stay (dropped ones carry the top bit).
+template <typename K>
+__device__ __forceinline__ uint32_t merge_rank_kept(const MergeLds<K>& L, uint32_t id, uint32_t g0, uint32_t g1) {
+   const K key = L.key[id];
+   uint32_t rank = 0;
+   for (uint32_t k = g0; k < g1; ++k) {
+      const uint32_t e = L.order[k];
+      if (!(e & 0x8000u)) rank += L.key[e] < key ? 1u : 0u;
+   }
+   return rank;
+}
+
+// A thread holds kMergeEO old slots (ids threadIdx.x + e * kMergeBS) and
+// kMergeEN new pairs (ids kMergeCapO + threadIdx.x + e * kMergeBS); element
+// state is packed to keep registers low: hs = home offset << 16 | arrival
+// rank within the home (kNoElem: no element), g = group start | end << 16,
+// j = sorted index, one drop bit per element.  Keys stay in LDS; values are
+// read again from memory when written.
+constexpr int kMergeE = kMergeEO + kMergeEN;
+constexpr uint32_t kNoElem = 0xFFFFFFFFu;
+
+template <typename K>
+__global__ __launch_bounds__(kMergeBS) __attribute__((amdgpu_waves_per_eu(4))) void k_merge_region(const Slot<K>* ob, const Slot<K>* S, const uint64_t* start,
+                                                          const uint64_t* sb, uint64_t* status, Slot<K>* nb,
+                                                          MergePark<K>* park, MergeHeader* mh, const BuildHeader* h,
+                                                          int sp, int rlog, uint64_t NR) {
+   __shared__ MergeLds<K> L;
+   if (!h->go) return;
+   const uint64_t N = 1ull << sp, mask = N - 1, R = 1ull << rlog;
+   const int lane = __lane_id(), wave = threadIdx.x / kWave;
+   if (threadIdx.x == 0) L.probe = 0;
+   uint32_t max_probe = 0;
+   auto elem_id = [&](int e) -> uint32_t {
+      return e < kMergeEO ? threadIdx.x + e * kMergeBS : kMergeCapO + threadIdx.x + (e - kMergeEO) * kMergeBS;
+   };
+   for (;;) {
+      __syncthreads(); // LDS of the previous region is no longer read
+      if (threadIdx.x == 0) L.ticket = atomic_add_relaxed(&mh->ticket, 1u);
+      __syncthreads();
+      const uint64_t r = L.ticket;
+      if (r >= NR) break;
+      const uint64_t rb = r << rlog;
+      const uint64_t o0 = sb[r], n0 = start[r];
+      const uint64_t Lo = sb[r + 1] - o0, Ln = start[r + 1] - n0;
+      auto give_up = [&]() { // the call fails; successors still get a state
+         if (threadIdx.x == 0) {
+            __hip_atomic_store(&mh->fail, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&status[r], mg_pack(kMgInc, kMgNone), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+         }
+      };
+      if (Lo > kMergeCapO || Ln > kMergeCapN) {
+         give_up();
+         continue;
+      }
+      for (uint32_t i = threadIdx.x; i <= R; i += kMergeBS) L.cnt[i] = 0;
+      __syncthreads();
+      // the value of element e, read again at write time
+      auto value = [&](int e) -> K {
+         return e < kMergeEO ? ob[(o0 + threadIdx.x + e * kMergeBS) & mask].val
+                             : S[n0 + threadIdx.x + (e - kMergeEO) * kMergeBS].val;
+      };
+      uint32_t hs[kMergeE], g[kMergeE], jj[kMergeE];
+      bool bad = false;
+#pragma unroll
+      for (int e = 0; e < kMergeE; ++e) {
+         hs[e] = kNoElem;
+         const uint32_t x = e < kMergeEO ? threadIdx.x + e * kMergeBS : threadIdx.x + (e - kMergeEO) * kMergeBS;
+         if (e < kMergeEO ? x < Lo : x < Ln) {
+            const K key = e < kMergeEO ? ob[(o0 + x) & mask].key : S[n0 + x].key;
+            if (e >= kMergeEO || key != empty_key<K>()) {
+               const uint64_t hr = (fib_home(key, sp) - rb) & mask;
+               if (hr >= R || (e < kMergeEO && key == tomb_key<K>())) {
+                  bad = true;
+               } else {
+                  L.key[elem_id(e)] = key;
+                  hs[e] = static_cast<uint32_t>(hr << 16) | atomicAdd(&L.cnt[hr], 1u);
+               }
+            }
+         }
+      }
+      if (__syncthreads_or(bad)) {
+         give_up();
+         continue;
+      }
+      lds_exclusive_scan<kMergeBS>(L.cnt, static_cast<uint32_t>(R) + 1, L.wsum);
+#pragma unroll
+      for (int e = 0; e < kMergeE; ++e)
+         if (hs[e] != kNoElem) L.order[L.cnt[hs[e] >> 16] + (hs[e] & 0xFFFFu)] = static_cast<uint16_t>(elem_id(e));
+      __syncthreads();
+      // sorted index: group start + rank by key; a newer copy drops an element
+      uint32_t drop = 0;
+      bool rep = false;
+#pragma unroll
+      for (int e = 0; e < kMergeE; ++e) {
+         if (hs[e] != kNoElem) {
+            const uint32_t g0 = L.cnt[hs[e] >> 16], g1 = L.cnt[(hs[e] >> 16) + 1];
+            g[e] = g0 | (g1 << 16);
+            bool d = false;
+            jj[e] = g0 + merge_rank(L, elem_id(e), g0, g1, d, rep);
+            drop |= d ? (1u << e) : 0u;
+         }
+      }
+      uint32_t ms;
+      if (!__syncthreads_or(rep)) {
+         ms = L.cnt[R];
+      } else {
+         // mark the dropped copies, then count and rank the kept ones
+#pragma unroll
+         for (int e = 0; e < kMergeE; ++e)
+            if (hs[e] != kNoElem && ((drop >> e) & 1u)) L.order[(g[e] & 0xFFFFu) + (hs[e] & 0xFFFFu)] |= 0x8000u;
+         __syncthreads();
+         for (uint32_t i = threadIdx.x; i <= R; i += kMergeBS) L.cnt[i] = 0;
+         __syncthreads();
+#pragma unroll
+         for (int e = 0; e < kMergeE; ++e)
+            if (hs[e] != kNoElem && !((drop >> e) & 1u)) atomicAdd(&L.cnt[hs[e] >> 16], 1u);
+         __syncthreads();
+         ms = lds_exclusive_scan<kMergeBS>(L.cnt, static_cast<uint32_t>(R) + 1, L.wsum);
+#pragma unroll
+         for (int e = 0; e < kMergeE; ++e)
+            if (hs[e] != kNoElem && !((drop >> e) & 1u))
+               jj[e] = L.cnt[hs[e] >> 16] + merge_rank_kept(L, elem_id(e), g[e] & 0xFFFFu, g[e] >> 16);
+      }
+#pragma unroll
+      for (int e = 0; e < kMergeE; ++e)
+         if (hs[e] != kNoElem && !((drop >> e) & 1u))
+            L.v[jj[e]] = static_cast<int16_t>(static_cast<int32_t>(hs[e] >> 16) - static_cast<int32_t>(jj[e]));
+      __syncthreads();
+      lds_incl_max16<kMergeBS>(L.v, ms, L.wmax);
+      RegionSum agg = kMgNone;
+      if (ms) {
+         agg.a = static_cast<int64_t>(rb) + static_cast<int64_t>(ms - 1) + L.v[ms - 1];
+         agg.b = ms;
+      }
+      // look-back: the carry is the position of the last key of regions < r
+      if (wave == 0) {
+         if (lane == 0)
+            __hip_atomic_store(&status[r], mg_pack(r == 0 ? kMgInc : kMgAgg, agg), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+         RegionSum excl = kMgNone;
+         if (r > 0) {
+            int64_t top = static_cast<int64_t>(r) - 1;
+            for (;;) {
+               const int64_t q = top - lane;
+               uint64_t w = kMgInc;
+               if (q >= 0) w = __hip_atomic_load(&status[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+               while (__ballot((w >> 62) == 0)) {
+                  if ((w >> 62) == 0) w = __hip_atomic_load(&status[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+               }
+               const uint64_t incm = __ballot((w >> 62) == 2);
+               const int k = incm ? first_bit(incm) : kWave - 1;
+               RegionSum gs = lane <= k ? mg_unpack(w) : kMgNone;
+#pragma unroll
+               for (int d = 1; d < kWave; d <<= 1) { // lane i + d holds earlier regions than lane i
+                  RegionSum y;
+                  y.a = __shfl_down(gs.a, d, kWave);
+                  y.b = __shfl_down(gs.b, d, kWave);
+                  if (lane + d < kWave) gs = compose(y, gs);
+               }
+               RegionSum win;
+               win.a = __shfl(gs.a, 0, kWave);
+               win.b = __shfl(gs.b, 0, kWave);
+               excl = compose(win, excl);
+               if (incm) break;
+               top -= kWave;
+            }
+            if (lane == 0)
+               __hip_atomic_store(&status[r], mg_pack(kMgInc, compose(excl, agg)), __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_AGENT);
+         }
+         if (lane == 0) L.xin = excl.a;
+      }
+      __syncthreads();
+      const int64_t carry = L.xin + 1 - static_cast<int64_t>(rb);
+      if (r == 0) { // parked until the wrap is known
+#pragma unroll
+         for (int e = 0; e < kMergeE; ++e)
+            if (hs[e] != kNoElem && !((drop >> e) & 1u))
+               park[jj[e]] = MergePark<K>{L.key[elem_id(e)], value(e), L.v[jj[e]], hs[e] >> 16};
+         if (threadIdx.x == 0) mh->n0 = ms;
+         continue;
+      }
+      const int64_t last = ms ? static_cast<int64_t>(ms - 1) + std::max<int64_t>(L.v[ms - 1], carry) : -1;
+      const int64_t c0 = std::max<int64_t>(carry, 0);
+      const int64_t span = std::max<int64_t>(static_cast<int64_t>(R), last + 1) - c0;
+      const int64_t words = span > 0 ? (span + 31) / 32 : 0;
+      for (int64_t w = threadIdx.x; w < words; w += kMergeBS) L.occ[w] = 0;
+      __syncthreads();
+#pragma unroll
+      for (int e = 0; e < kMergeE; ++e) {
+         if (hs[e] != kNoElem && !((drop >> e) & 1u)) {
+            const int64_t p = static_cast<int64_t>(jj[e]) + std::max<int64_t>(L.v[jj[e]], carry);
+            const int64_t q = p - c0;
+            atomicOr(&L.occ[q >> 5], 1u << (q & 31));
+            Slot<K> sl;
+            sl.key = L.key[elem_id(e)];
+            sl.val = value(e);
+            nb[(rb + static_cast<uint64_t>(p)) & mask] = sl;
+            max_probe = std::max<uint32_t>(max_probe, static_cast<uint32_t>(p - static_cast<int64_t>(hs[e] >> 16) + 1));
+         }
+      }
+      __syncthreads();
+      for (int64_t q = threadIdx.x; q < span; q += kMergeBS) {
+         if (!((L.occ[q >> 5] >> (q & 31)) & 1u)) {
+            Slot<K> sl;
+            sl.key = empty_key<K>();
+            sl.val = 0;
+            nb[(rb + static_cast<uint64_t>(c0 + q)) & mask] = sl;
+         }
+      }
+   }
+   max_probe = wave_max(max_probe);
+   if (lane == 0 && max_probe) atomicMax(&L.probe, max_probe);
+   __syncthreads();
+   if (threadIdx.x == 0 && L.probe > __hip_atomic_load(&mh->max_probe, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      atomic_max_relaxed(&mh->max_probe, L.probe);
+}
+
+// Region 0 with its carry from the wrap, then the Info.  One workgroup.
+template <typename K>
+__global__ __launch_bounds__(kMergeBS) void k_merge_end(const MergePark<K>* park, const uint64_t* status, Slot<K>* nb,
+                                                       hn_info* info, const MergeHeader* mh, const BuildHeader* h,
+                                                       int sp, int rlog, uint64_t NR) {
+   __shared__ uint32_t occ[kMergeSpanWords];
+   __shared__ uint32_t probe_s;
+   const uint64_t N = 1ull << sp, mask = N - 1, R = 1ull << rlog;
+   bool fail = !h->go || mh->fail;
+   const RegionSum tot = fail ? kMgNone : mg_unpack(status[NR - 1]);
+   fail = fail || tot.b >= static_cast<int64_t>(N);
+   const uint32_t n0 = mh->n0;
+   const int64_t c0 = std::max<int64_t>(tot.a + 1 - static_cast<int64_t>(N), 0); // slots the wrap fills in region 0
+   const int64_t vlast = n0 ? park[n0 - 1].v : 0;
+   const int64_t last = n0 ? static_cast<int64_t>(n0 - 1) + std::max<int64_t>(vlast, c0) : c0 - 1;
+   const int64_t spec = n0 ? static_cast<int64_t>(n0 - 1) + vlast : -1; // what region 1 was placed after
+   if (NR > 1 && std::max<int64_t>(last + 1 - static_cast<int64_t>(R), 0) !=
+                     std::max<int64_t>(spec + 1 - static_cast<int64_t>(R), 0))
+      fail = true;
+   if (fail) {
+      if (threadIdx.x == 0) info->err = HN_STATUS_INVALID;
+      return;
+   }
+   const int64_t span = std::max<int64_t>(static_cast<int64_t>(R), last + 1) - c0;
+   for (int64_t w = threadIdx.x; w < (span + 31) / 32; w += kMergeBS) occ[w] = 0;
+   if (threadIdx.x == 0) probe_s = 0;
+   __syncthreads();
+   uint32_t probe = 0;
+   for (uint32_t j = threadIdx.x; j < n0; j += kMergeBS) {
+      const MergePark<K> e = park[j];
+      const int64_t p = static_cast<int64_t>(j) + std::max<int64_t>(e.v, c0);
+      const int64_t q = p - c0;
+      atomicOr(&occ[q >> 5], 1u << (q & 31));
+      Slot<K> sl;
+      sl.key = e.key;
+      sl.val = e.val;
+      nb[static_cast<uint64_t>(p) & mask] = sl;
+      probe = std::max<uint32_t>(probe, static_cast<uint32_t>(p - static_cast<int64_t>(e.hr) + 1));
+   }
+   __syncthreads();
+   for (int64_t q = threadIdx.x; q < span; q += kMergeBS) {
+      if (!((occ[q >> 5] >> (q & 31)) & 1u)) {
+         Slot<K> sl;
+         sl.key = empty_key<K>();
+         sl.val = 0;
+         nb[static_cast<uint64_t>(c0 + q) & mask] = sl;
+      }
+   }
+   probe = wave_max(probe);
+   if (__lane_id() == 0 && probe) atomicMax(&probe_s, probe);
+   __syncthreads();
+   if (threadIdx.x == 0) {
+      const uint64_t mp = std::max<uint32_t>(probe_s, mh->max_probe);
+      info->fill = static_cast<uint64_t>(tot.b);
+      info->tombstone_counter = 0;
+      info->current_max_bucket_overflow = std::max<uint64_t>(kBucketOverflow, next_overflow(mp, kBucketOverflow));
+      info->reserved0 = 0;
+      info->reserved1 = kInfoOrdered;
+      info->err = HN_STATUS_SUCCESS;
+   }
+}
+
+template <typename K>
+struct MergePlan {
+   BuildPlan P;
+   size_t off_mh, off_sb, off_status, off_park, total;
+};
+
+template <typename K>
+bool merge_plan(int sp, uint64_t len, MergePlan<K>* M) {
+   if (sp < 1 || sp > 30 || len == 0) return false;
+   if (!build_plan<K>(sp, len, kMaxDigitBits, &M->P)) return false;
+   size_t o = align256(M->P.total);
+   M->off_mh = o;
+   o = align256(o + sizeof(MergeHeader));
+   M->off_sb = o;
+   o = align256(o + (M->P.NR + 1) * sizeof(uint64_t));
+   M->off_status = o;
+   o = align256(o + M->P.NR * sizeof(uint64_t));
+   M->off_park = o;
+   o = align256(o + kMergeCap * sizeof(MergePark<K>));
+   M->total = o;
+   return true;
+}
+
+template <typename K>
+size_t merge_scratch_bytes(int sp, size_t len) {
+   MergePlan<K> M;
+   return merge_plan<K>(sp, len, &M) ? M.total : 0;
+}
+
+template <typename K>
+int merge(const void* keys, const void* vals, const void* pairs, const void* old_b, void* new_b, hn_info* info,
+          size_t len, int sp, void* scratch, size_t bytes, hipStream_t s) {
+   if (!old_b || !new_b || !info) return HN_EINVAL;
+   if (len > 0 && !keys && !pairs) return HN_EINVAL;
+   if (sp < 1 || sp > 30) return HN_EINVAL;
+   const uint64_t N = 1ull << sp;
+   const char* o0 = static_cast<const char*>(old_b);
+   const char* n0 = static_cast<const char*>(new_b);
+   if (o0 < n0 + N * sizeof(Slot<K>) && n0 < o0 + N * sizeof(Slot<K>)) return HN_EINVAL;
+   if (len == 0) // nothing to merge: the output is the table
+      return static_cast<int>(hipMemcpyAsync(new_b, old_b, N * sizeof(Slot<K>), hipMemcpyDeviceToDevice, s));
+   MergePlan<K> M;
+   if (!merge_plan<K>(sp, len, &M)) return HN_EINVAL;
+   if (!scratch || bytes < M.total) return HN_ESCRATCH;
+   const BuildPlan& P = M.P;
+   char* base = static_cast<char*>(scratch);
+   BuildHeader* h = reinterpret_cast<BuildHeader*>(base);
+   const Slot<K>* S = reinterpret_cast<const Slot<K>*>(base + P.off_pairs);
+   const uint64_t* start = reinterpret_cast<const uint64_t*>(base + P.off_start);
+   MergeHeader* mh = reinterpret_cast<MergeHeader*>(base + M.off_mh);
+   uint64_t* sb = reinterpret_cast<uint64_t*>(base + M.off_sb);
+   uint64_t* status = reinterpret_cast<uint64_t*>(base + M.off_status);
+   MergePark<K>* park = reinterpret_cast<MergePark<K>*>(base + M.off_park);
+   const Slot<K>* ob = static_cast<const Slot<K>*>(old_b);
+   Slot<K>* nb = static_cast<Slot<K>*>(new_b);
+
+   hipLaunchKernelGGL(k_build_begin, dim3(1), dim3(64), 0, s, info, h, sp, static_cast<uint64_t>(len),
+                      static_cast<int>(BUILD_MERGE));
+   hipLaunchKernelGGL(k_merge_begin, dim3(1), dim3(64), 0, s, mh);
+   PartIn<K> in{static_cast<const K*>(keys), static_cast<const K*>(vals), static_cast<const Slot<K>*>(pairs)};
+   const int src0 = pairs ? SRC_PAIRS : (vals ? SRC_SOA : SRC_INDEX);
+   int rc = partition_regions<K>(P, src0, in, len, nb, scratch, s);
+   if (rc) return rc;
+   hipLaunchKernelGGL(k_merge_bounds<K>, dim3(static_cast<uint32_t>(std::min<uint64_t>((P.NR + 3) / 4, 65536))),
+                      dim3(256), 0, s, ob, sp, P.rlog, P.NR, sb, status, mh, h);
+   static std::atomic<int> occ{0};
+   const int per_cu = resident_blocks_per_cu(k_merge_region<K>, kMergeBS, occ);
+   const uint32_t grid =
+       static_cast<uint32_t>(std::min<uint64_t>(P.NR, static_cast<uint64_t>(cu_count()) * static_cast<uint64_t>(per_cu)));
+   hipLaunchKernelGGL(k_merge_region<K>, dim3(grid), dim3(kMergeBS), 0, s, ob, S, start, sb, status, nb, park, mh, h,
+                      sp, P.rlog, P.NR);
+   hipLaunchKernelGGL(k_merge_end<K>, dim3(1), dim3(kMergeBS), 0, s, park, status, nb, info, mh, h, sp, P.rlog, P.NR);
+   return static_cast<int>(hipGetLastError());
+}
+
 } // namespace hn
 
 using namespace hn;
@@ -1176,5 +1732,23 @@ int hn_build_pairs_u32(const void* p, void* b, hn_info* i, size_t n, int sp, voi
 int hn_build_pairs_u64(const void* p, void* b, hn_info* i, size_t n, int sp, void* sc, size_t sb, hn_stream_t s,
                        const hn_tune* t) {
    return build<uint64_t>(nullptr, nullptr, p, b, i, n, sp, sc, sb, s, t);
+}
+size_t hn_merge_scratch_bytes_u32(int sp, size_t len) { return merge_scratch_bytes<uint32_t>(sp, len); }
+size_t hn_merge_scratch_bytes_u64(int sp, size_t len) { return merge_scratch_bytes<uint64_t>(sp, len); }
+int hn_merge_u32(const void* k, const void* v, const void* ob, void* nb, hn_info* i, size_t n, int sp, void* sc,
+                 size_t sb, hn_stream_t s) {
+   return merge<uint32_t>(k, v, nullptr, ob, nb, i, n, sp, sc, sb, s);
+}
+int hn_merge_u64(const void* k, const void* v, const void* ob, void* nb, hn_info* i, size_t n, int sp, void* sc,
+                 size_t sb, hn_stream_t s) {
+   return merge<uint64_t>(k, v, nullptr, ob, nb, i, n, sp, sc, sb, s);
+}
+int hn_merge_pairs_u32(const void* p, const void* ob, void* nb, hn_info* i, size_t n, int sp, void* sc, size_t sb,
+                       hn_stream_t s) {
+   return merge<uint32_t>(nullptr, nullptr, p, ob, nb, i, n, sp, sc, sb, s);
+}
+int hn_merge_pairs_u64(const void* p, const void* ob, void* nb, hn_info* i, size_t n, int sp, void* sc, size_t sb,
+                       hn_stream_t s) {
+   return merge<uint64_t>(nullptr, nullptr, p, ob, nb, i, n, sp, sc, sb, s);
 }
 } // extern "C"

@@ -291,6 +291,40 @@ HN_API int hn_build_pairs_u32(const void* pairs, void* buckets, hn_info* info, s
 HN_API int hn_build_pairs_u64(const void* pairs, void* buckets, hn_info* info, size_t len, int size_power,
                               void* scratch, size_t scratch_bytes, hn_stream_t s, const hn_tune* t);
 
+/* Insert into a table that already holds keys (Hasher::insert,
+ * hashers.h:55-70, kernels_AMD.h insert_kernel 136-405), written to a
+ * second bucket array of the same size: new_buckets receives the image
+ * hn_insert_* followed by hn_reorder_* gives -- the union of the table's
+ * live set and the batch, every cluster in home-slot order, ties by key; a
+ * key already present takes the batch's value, a key repeated in the batch
+ * its last one -- and *info becomes that table's (fill, overflow =
+ * max(64, nextOverflow(longest probe)), HN_INFO_ORDERED, err = SUCCESS).
+ * old_buckets is only read.  No probing and no atomics on the table: the
+ * batch takes the bulk build's partition passes, then each 2048-slot region
+ * merges its run of the old table (in Robin Hood order the keys of a region
+ * stand together) with its share of the batch in LDS and places the union,
+ * regions chained by a decoupled look-back.  Needs a table with
+ * HN_INFO_ORDERED and no TOMBSTONEs, size_power <= 30, len < 2^size_power,
+ * buffers that do not overlap, and `scratch` of
+ * hn_merge_scratch_bytes_*(size_power, len) bytes.  When the table does not
+ * qualify, a region holds more than 3,072 old slots or 1,536 batch pairs
+ * (skewed keys), the wrap past the table's end would move region 0's
+ * spill, or the union would fill the table, the call sets err = INVALID and
+ * changes nothing else (old_buckets and the rest of Info untouched;
+ * new_buckets unspecified): insert with hn_insert_* instead.  len == 0
+ * copies the table.  `vals` NULL stores each key's index in the batch; the
+ * _pairs variant reads hash_pair elements. */
+HN_API size_t hn_merge_scratch_bytes_u32(int size_power, size_t len);
+HN_API size_t hn_merge_scratch_bytes_u64(int size_power, size_t len);
+HN_API int hn_merge_u32(const void* keys, const void* vals, const void* old_buckets, void* new_buckets, hn_info* info,
+                        size_t len, int size_power, void* scratch, size_t scratch_bytes, hn_stream_t s);
+HN_API int hn_merge_u64(const void* keys, const void* vals, const void* old_buckets, void* new_buckets, hn_info* info,
+                        size_t len, int size_power, void* scratch, size_t scratch_bytes, hn_stream_t s);
+HN_API int hn_merge_pairs_u32(const void* pairs, const void* old_buckets, void* new_buckets, hn_info* info, size_t len,
+                              int size_power, void* scratch, size_t scratch_bytes, hn_stream_t s);
+HN_API int hn_merge_pairs_u64(const void* pairs, const void* old_buckets, void* new_buckets, hn_info* info, size_t len,
+                              int size_power, void* scratch, size_t scratch_bytes, hn_stream_t s);
+
 /* Hashmap::device_rehash into a LARGER table (hashinator.h:284-336: extract
  * the valid elements, Info(newSizePower), insert them) as one bulk build
  * whose first partition pass reads old_buckets' live slots: no intermediate
