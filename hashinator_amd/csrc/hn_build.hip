@@ -209,7 +209,7 @@ __global__ void k_build_begin(const hn_info* info, BuildHeader* h, int sp, uint6
       if (mode == BUILD_EMPTY)
          h->go = (sp_ok && info->fill == 0 && info->tombstone_counter == 0) ? 1u : 0u;
       else if (mode == BUILD_MERGE)
-         h->go = (sp_ok && info->tombstone_counter == 0 && (info->reserved1 & kInfoOrdered)) ? 1u : 0u;
+         h->go = sp_ok ? 1u : 0u; // the batch is partitioned whether the merge or its fallback runs
       else
          h->go = 1u; // k_rehash_info has just set the Info up for the new table
       h->fail = 0;
@@ -1191,12 +1191,14 @@ int rehash_build(const void* old_b, int old_sp, void* new_b, int new_sp, hn_info
 // slots, so the spans tile the table, and positions past the end wrap into
 // region 0's carry-in slots, which region 0 does not write.  Region 0's
 // carry comes from that wrap, known only at the end, so region 0 is parked in
-// scratch and placed by k_merge_end.  The call ends with err = INVALID (the
-// old table untouched, the rest of Info too, the output unspecified) when
-// the wrap would change what region 0 spills into region 1, a region
-// exceeds its LDS capacity (3,072 old slots or 1,536 new pairs), the old
-// table is not in order (a key outside its region's run), the union would
-// fill the table, or the table has TOMBSTONEs or no order flag.
+// scratch and placed by k_merge_end.  When the table has TOMBSTONEs or no
+// order flag, a region exceeds its LDS capacity (3,072 old slots or 1,536 new
+// pairs), the old table is not in order after all (a key outside its
+// region's run), the wrap would change what region 0 spills into region 1,
+// or the union would fill the table, the output is made the other way, on
+// the device: the old table copied, then the probing insert of the
+// partitioned batch (the partition keeps the batch order of equal keys), so
+// the output is hn_insert_*'s result and the order flag is cleared.
 constexpr int kMergeBS = 512;
 constexpr int kMergeEO = 6; // old slots per thread
 constexpr int kMergeEN = 3; // new pairs per thread
@@ -1207,11 +1209,14 @@ constexpr uint32_t kMergeSpanWords = (kRegionMax + kMergeCap + 31) / 32;
 static_assert(kMergeCap < 0x8000, "element ids leave the top bit of a 16-bit order entry free");
 
 struct MergeHeader {
-   uint32_t ticket;    // next region
-   uint32_t fail;      // see above
-   uint32_t max_probe; // longest probe of the keys placed by the regions
-   uint32_t n0;        // keys of region 0, parked
-   uint32_t pad[60];
+   uint32_t ticket;     // next region
+   uint32_t fail;       // a region gave up, or a bound was not found
+   uint32_t max_probe;  // longest probe of the keys placed by the regions
+   uint32_t n0;         // keys of region 0, parked
+   uint32_t ineligible; // TOMBSTONEs or no order flag: the regions do not run
+   int32_t fb_fail;     // the fallback insert found no slot for a key
+   uint64_t fb_len;     // pairs the fallback insert takes (0: the merge stands)
+   uint32_t pad[56];
 };
 static_assert(sizeof(MergeHeader) == 256, "header is one 256-byte line");
 
@@ -1238,12 +1243,15 @@ __device__ __forceinline__ RegionSum mg_unpack(uint64_t w) {
 // No keys: (A, C) = (-1, 0) acts as the identity for region maps (A >= C - 1).
 constexpr RegionSum kMgNone = {-1, 0};
 
-__global__ void k_merge_begin(MergeHeader* mh) {
+__global__ void k_merge_begin(const hn_info* info, MergeHeader* mh) {
    if (threadIdx.x == 0) {
       mh->ticket = 0;
       mh->fail = 0;
       mh->max_probe = 0;
       mh->n0 = 0;
+      mh->ineligible = (info->tombstone_counter != 0 || !(info->reserved1 & kInfoOrdered)) ? 1u : 0u;
+      mh->fb_fail = 0;
+      mh->fb_len = 0;
    }
 }
 
@@ -1252,7 +1260,7 @@ __global__ void k_merge_begin(MergeHeader* mh) {
 template <typename K>
 __global__ __launch_bounds__(256) void k_merge_bounds(const Slot<K>* ob, int sp, int rlog, uint64_t NR, uint64_t* sb,
                                                       uint64_t* status, MergeHeader* mh, const BuildHeader* h) {
-   if (!h->go) return;
+   if (!h->go || mh->ineligible) return;
    const int lane = __lane_id();
    const uint64_t N = 1ull << sp, mask = N - 1;
    const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (blockDim.x / kWave);
@@ -1321,6 +1329,9 @@ struct MergeLds {
    uint32_t occ[kMergeSpanWords];
    uint32_t wsum[kMergeBS / kWave];
    int32_t wmax[kMergeBS / kWave];
+   int64_t lb_a[kMergeBS / kWave]; // look-back: each wave's composed window
+   int64_t lb_b[kMergeBS / kWave];
+   int32_t lb_inc[kMergeBS / kWave];
    uint32_t ticket;
    uint32_t probe;
    int64_t xin;
@@ -1371,7 +1382,7 @@ __global__ __launch_bounds__(kMergeBS) __attribute__((amdgpu_waves_per_eu(4))) v
                                                           MergePark<K>* park, MergeHeader* mh, const BuildHeader* h,
                                                           int sp, int rlog, uint64_t NR) {
    __shared__ MergeLds<K> L;
-   if (!h->go) return;
+   if (!h->go || mh->ineligible) return; // the same for every workgroup: no look-back waits on a skipped region
    const uint64_t N = 1ull << sp, mask = N - 1, R = 1ull << rlog;
    const int lane = __lane_id(), wave = threadIdx.x / kWave;
    if (threadIdx.x == 0) L.probe = 0;
@@ -1478,44 +1489,58 @@ __global__ __launch_bounds__(kMergeBS) __attribute__((amdgpu_waves_per_eu(4))) v
          agg.a = static_cast<int64_t>(rb) + static_cast<int64_t>(ms - 1) + L.v[ms - 1];
          agg.b = ms;
       }
-      // look-back: the carry is the position of the last key of regions < r
-      if (wave == 0) {
-         if (lane == 0)
-            __hip_atomic_store(&status[r], mg_pack(r == 0 ? kMgInc : kMgAgg, agg), __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-         RegionSum excl = kMgNone;
-         if (r > 0) {
-            int64_t top = static_cast<int64_t>(r) - 1;
-            for (;;) {
-               const int64_t q = top - lane;
-               uint64_t w = kMgInc;
-               if (q >= 0) w = __hip_atomic_load(&status[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-               while (__ballot((w >> 62) == 0)) {
-                  if ((w >> 62) == 0) w = __hip_atomic_load(&status[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-               }
-               const uint64_t incm = __ballot((w >> 62) == 2);
-               const int k = incm ? first_bit(incm) : kWave - 1;
-               RegionSum gs = lane <= k ? mg_unpack(w) : kMgNone;
-#pragma unroll
-               for (int d = 1; d < kWave; d <<= 1) { // lane i + d holds earlier regions than lane i
-                  RegionSum y;
-                  y.a = __shfl_down(gs.a, d, kWave);
-                  y.b = __shfl_down(gs.b, d, kWave);
-                  if (lane + d < kWave) gs = compose(y, gs);
-               }
-               RegionSum win;
-               win.a = __shfl(gs.a, 0, kWave);
-               win.b = __shfl(gs.b, 0, kWave);
-               excl = compose(win, excl);
-               if (incm) break;
-               top -= kWave;
+      // look-back: the carry is the position of the last key of regions < r.
+      // Every wave reads 64 predecessors, so one round of loads covers the
+      // kMergeBS regions in flight (lane i of wave w: region top - 64w - i).
+      if (threadIdx.x == 0)
+         __hip_atomic_store(&status[r], mg_pack(r == 0 ? kMgInc : kMgAgg, agg), __ATOMIC_RELAXED,
+                            __HIP_MEMORY_SCOPE_AGENT);
+      RegionSum excl = kMgNone;
+      if (r > 0) {
+         int64_t top = static_cast<int64_t>(r) - 1;
+         for (;;) {
+            const int64_t q = top - static_cast<int64_t>(threadIdx.x);
+            uint64_t w = kMgInc;
+            if (q >= 0) w = __hip_atomic_load(&status[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            while (__ballot((w >> 62) == 0)) {
+               if ((w >> 62) == 0) w = __hip_atomic_load(&status[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
-            if (lane == 0)
-               __hip_atomic_store(&status[r], mg_pack(kMgInc, compose(excl, agg)), __ATOMIC_RELAXED,
-                                  __HIP_MEMORY_SCOPE_AGENT);
+            const uint64_t incm = __ballot((w >> 62) == 2);
+            const int k = incm ? first_bit(incm) : kWave - 1;
+            RegionSum gs = lane <= k ? mg_unpack(w) : kMgNone;
+#pragma unroll
+            for (int d = 1; d < kWave; d <<= 1) { // lane i + d holds earlier regions than lane i
+               RegionSum y;
+               y.a = __shfl_down(gs.a, d, kWave);
+               y.b = __shfl_down(gs.b, d, kWave);
+               if (lane + d < kWave) gs = compose(y, gs);
+            }
+            if (lane == 0) {
+               L.lb_a[wave] = gs.a;
+               L.lb_b[wave] = gs.b;
+               L.lb_inc[wave] = incm ? 1 : 0;
+            }
+            __syncthreads();
+            // waves in order from the nearest; stop at the first holding an inclusive state
+            bool found = false;
+            RegionSum win = kMgNone;
+            for (int v = 0; v < kMergeBS / kWave && !found; ++v) {
+               RegionSum gv;
+               gv.a = L.lb_a[v];
+               gv.b = L.lb_b[v];
+               win = compose(gv, win);
+               found = L.lb_inc[v] != 0;
+            }
+            excl = compose(win, excl);
+            __syncthreads();
+            if (found) break;
+            top -= kMergeBS;
          }
-         if (lane == 0) L.xin = excl.a;
+         if (threadIdx.x == 0)
+            __hip_atomic_store(&status[r], mg_pack(kMgInc, compose(excl, agg)), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
       }
+      if (threadIdx.x == 0) L.xin = excl.a;
       __syncthreads();
       const int64_t carry = L.xin + 1 - static_cast<int64_t>(rb);
       if (r == 0) { // parked until the wrap is known
@@ -1562,18 +1587,23 @@ __global__ __launch_bounds__(kMergeBS) __attribute__((amdgpu_waves_per_eu(4))) v
       atomic_max_relaxed(&mh->max_probe, L.probe);
 }
 
-// Region 0 with its carry from the wrap, then the Info.  One workgroup.
+// Region 0 with its carry from the wrap, then the Info; or, when the merge
+// does not stand, the fallback's length.  One workgroup.
 template <typename K>
 __global__ __launch_bounds__(kMergeBS) void k_merge_end(const MergePark<K>* park, const uint64_t* status, Slot<K>* nb,
-                                                       hn_info* info, const MergeHeader* mh, const BuildHeader* h,
+                                                       hn_info* info, MergeHeader* mh, const BuildHeader* h,
                                                        int sp, int rlog, uint64_t NR) {
    __shared__ uint32_t occ[kMergeSpanWords];
    __shared__ uint32_t probe_s;
    const uint64_t N = 1ull << sp, mask = N - 1, R = 1ull << rlog;
-   bool fail = !h->go || mh->fail;
+   if (!h->go) { // a size power that is not the table's
+      if (threadIdx.x == 0) info->err = HN_STATUS_INVALID;
+      return;
+   }
+   bool fail = mh->ineligible || mh->fail;
    const RegionSum tot = fail ? kMgNone : mg_unpack(status[NR - 1]);
    fail = fail || tot.b >= static_cast<int64_t>(N);
-   const uint32_t n0 = mh->n0;
+   const uint32_t n0 = fail ? 0 : mh->n0;
    const int64_t c0 = std::max<int64_t>(tot.a + 1 - static_cast<int64_t>(N), 0); // slots the wrap fills in region 0
    const int64_t vlast = n0 ? park[n0 - 1].v : 0;
    const int64_t last = n0 ? static_cast<int64_t>(n0 - 1) + std::max<int64_t>(vlast, c0) : c0 - 1;
@@ -1582,7 +1612,7 @@ __global__ __launch_bounds__(kMergeBS) void k_merge_end(const MergePark<K>* park
                      std::max<int64_t>(spec + 1 - static_cast<int64_t>(R), 0))
       fail = true;
    if (fail) {
-      if (threadIdx.x == 0) info->err = HN_STATUS_INVALID;
+      if (threadIdx.x == 0) mh->fb_len = h->n;
       return;
    }
    const int64_t span = std::max<int64_t>(static_cast<int64_t>(R), last + 1) - c0;
@@ -1622,6 +1652,21 @@ __global__ __launch_bounds__(kMergeBS) void k_merge_end(const MergePark<K>* park
       info->reserved1 = kInfoOrdered;
       info->err = HN_STATUS_SUCCESS;
    }
+}
+
+// Fallback: the old table into the output, then (the probing insert of the
+// partitioned batch runs between) err and the flags.
+template <typename K>
+__global__ __launch_bounds__(256) void k_merge_copy(const Slot<K>* ob, Slot<K>* nb, uint64_t N, const MergeHeader* mh) {
+   if (!mh->fb_len) return;
+   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < N; i += stride) nb[i] = ob[i];
+}
+__global__ void k_merge_fin(hn_info* info, const MergeHeader* mh) {
+   if (threadIdx.x != 0 || !mh->fb_len) return;
+   info->err = mh->fb_fail ? HN_STATUS_FAIL : HN_STATUS_SUCCESS;
+   info->reserved0 = 0;
+   info->reserved1 = 0;
 }
 
 template <typename K>
@@ -1682,7 +1727,7 @@ int merge(const void* keys, const void* vals, const void* pairs, const void* old
 
    hipLaunchKernelGGL(k_build_begin, dim3(1), dim3(64), 0, s, info, h, sp, static_cast<uint64_t>(len),
                       static_cast<int>(BUILD_MERGE));
-   hipLaunchKernelGGL(k_merge_begin, dim3(1), dim3(64), 0, s, mh);
+   hipLaunchKernelGGL(k_merge_begin, dim3(1), dim3(64), 0, s, info, mh);
    PartIn<K> in{static_cast<const K*>(keys), static_cast<const K*>(vals), static_cast<const Slot<K>*>(pairs)};
    const int src0 = pairs ? SRC_PAIRS : (vals ? SRC_SOA : SRC_INDEX);
    int rc = partition_regions<K>(P, src0, in, len, nb, scratch, s);
@@ -1696,6 +1741,10 @@ int merge(const void* keys, const void* vals, const void* pairs, const void* old
    hipLaunchKernelGGL(k_merge_region<K>, dim3(grid), dim3(kMergeBS), 0, s, ob, S, start, sb, status, nb, park, mh, h,
                       sp, P.rlog, P.NR);
    hipLaunchKernelGGL(k_merge_end<K>, dim3(1), dim3(kMergeBS), 0, s, park, status, nb, info, mh, h, sp, P.rlog, P.NR);
+   hipLaunchKernelGGL(k_merge_copy<K>, dim3(stream_blocks(N, 256)), dim3(256), 0, s, ob, nb, N, mh);
+   rc = launch_insert_pairs_dlen<K>(S, nb, info, len, &mh->fb_len, &mh->fb_fail, s);
+   if (rc) return rc;
+   hipLaunchKernelGGL(k_merge_fin, dim3(1), dim3(64), 0, s, info, mh);
    return static_cast<int>(hipGetLastError());
 }
 

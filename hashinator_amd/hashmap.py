@@ -124,7 +124,8 @@ class Hashmap:
     """Hashinator::Hashmap<K,V> with K = V = uint32_t or uint64_t, accelerated API."""
 
     def __init__(self, sizePower: int = 5, key_bits: int = 32, device="cuda", tune: hn_tune | None = None,
-                 auto_reorder: float = 1.0 / 16, auto_build: float = 1.0 / 16, build_min_keys: int = 1 << 17):
+                 auto_reorder: float = 1.0 / 16, auto_build: float = 1.0 / 16, build_min_keys: int = 1 << 17,
+                 merge_min: float = 0.25):
         """``auto_reorder``: after an insert of at least this fraction of the
         bucket count, and after clean_tombstones and device_rehash, run the
         Robin Hood reorder pass (``reorder()``), which makes later misses
@@ -138,13 +139,22 @@ class Hashmap:
         reorder are faster (one MI355X, u32: 0.088 against 0.090 ms at
         131,072 keys and LF 0.5, 0.10 against 0.19 ms at 235,929 keys and LF
         0.9, 0.10 against 0.26 ms at 524,288 keys and LF 0.5).  Growing a
-        map of at least this many keys rehashes it by the build too."""
+        map of at least this many keys rehashes it by the build too.
+        ``merge_min``: an insert of at least this fraction of the bucket count
+        (and of ``build_min_keys``) into a table that holds keys and no
+        tombstones takes the merge (hn_merge_*), which writes the insert +
+        reorder image into a fresh bucket array; the merge reads and writes
+        the whole table (about 7 ms at 2^28 u64 slots, against 0.18 ns per
+        key for the probing insert there), so it pays from about a quarter of
+        the table.  A table not in Robin Hood order is reordered first when
+        the batch is at least 4/3 of this fraction.  0 turns it off."""
         if key_bits not in (32, 64):
             raise ValueError("key_bits must be 32 or 64")
         self.kb = key_bits
         self.auto_reorder = auto_reorder
         self.auto_build = auto_build
         self.build_min_keys = build_min_keys
+        self.merge_min = merge_min
         self.device = torch.device(device)
         self.tune = tune
         self._L = _lib.load()
@@ -257,6 +267,8 @@ class Hashmap:
         self._grow_for(n, targetLF, stream)
         if self._try_build(k, None if vals is None else v, n, vals is None, stream):
             return
+        if self._try_merge(k, None if vals is None else v, n, vals is None, stream):
+            return
         if vals is None:
             rc = self._fn("insert_pairs")(_ptr(k), _ptr(self.buckets), _ptr(self._info), n, _stream(stream), self._tune())
         else:
@@ -291,6 +303,43 @@ class Hashmap:
         self._sync(stream)
         del scratch
         return self.info().err == status.success
+
+    def _try_merge(self, k, v, n, pairs, stream) -> bool:
+        """Large insert into a table that holds keys: hn_merge_* writes the
+        insert + reorder image into a fresh bucket array, which then replaces
+        the old one.  Where the merge itself declines (keys crowding one
+        region, say), the device writes the probing insert's result into the
+        new array instead, so the insert is complete either way."""
+        N = self.bucket_count()
+        if not self.merge_min or n < self.merge_min * N or n < self.build_min_keys or n >= N or self._sp > 30:
+            return False
+        i = self.info()
+        if i.fill == 0 or i.tombstone_counter != 0:
+            return False
+        if not (i.reserved1 & _lib.HN_INFO_ORDERED):
+            if n < self.merge_min * N * 4 / 3:
+                return False
+            self._reorder(stream)
+        sb = getattr(self._L, f"hn_merge_scratch_bytes_{self._sfx}")(self._sp, n)
+        if sb == 0:
+            return False
+        scratch = torch.empty(sb, dtype=torch.uint8, device=self.device)
+        nb = self._new_table(self._sp, init=False)  # the merge (or its fallback) writes every slot
+        if pairs:
+            rc = self._fn("merge_pairs")(_ptr(k), _ptr(self.buckets), _ptr(nb), _ptr(self._info), n, self._sp,
+                                         _ptr(scratch), sb, _stream(stream))
+        else:
+            rc = self._fn("merge")(_ptr(k), _ptr(v) if v is not None else None, _ptr(self.buckets), _ptr(nb),
+                                   _ptr(self._info), n, self._sp, _ptr(scratch), sb, _stream(stream))
+        check(rc, "merge")
+        old = self.buckets  # read by the merge until the stream is synchronised
+        self.buckets = nb
+        self._sync(stream)
+        del old, scratch
+        if self.auto_reorder and not self.ordered() and n >= self.auto_reorder * N:
+            self._reorder(stream)  # the fallback inserted in arrival order
+            self._sync(stream)
+        return True
 
     def _after_insert(self, n, stream):
         if self.auto_reorder and n >= self.auto_reorder * self.bucket_count():
@@ -328,6 +377,8 @@ class Hashmap:
             return
         self._grow_for(n, targetLF, stream)
         if self._try_build(k, None, n, False, stream):
+            return
+        if self._try_merge(k, None, n, False, stream):
             return
         check(self._fn("insert_index")(_ptr(k), _ptr(self.buckets), _ptr(self._info), n, _stream(stream),
                                        self._tune()), "insertIndex")

@@ -303,17 +303,19 @@ HN_API int hn_build_pairs_u64(const void* pairs, void* buckets, hn_info* info, s
  * batch takes the bulk build's partition passes, then each 2048-slot region
  * merges its run of the old table (in Robin Hood order the keys of a region
  * stand together) with its share of the batch in LDS and places the union,
- * regions chained by a decoupled look-back.  Needs a table with
- * HN_INFO_ORDERED and no TOMBSTONEs, size_power <= 30, len < 2^size_power,
- * buffers that do not overlap, and `scratch` of
- * hn_merge_scratch_bytes_*(size_power, len) bytes.  When the table does not
- * qualify, a region holds more than 3,072 old slots or 1,536 batch pairs
- * (skewed keys), the wrap past the table's end would move region 0's
- * spill, or the union would fill the table, the call sets err = INVALID and
- * changes nothing else (old_buckets and the rest of Info untouched;
- * new_buckets unspecified): insert with hn_insert_* instead.  len == 0
- * copies the table.  `vals` NULL stores each key's index in the batch; the
- * _pairs variant reads hash_pair elements. */
+ * regions chained by a decoupled look-back.  Needs size_power <= 30 (the
+ * table's; otherwise err = INVALID and nothing else changes), len <
+ * 2^size_power, buffers that do not overlap, and `scratch` of
+ * hn_merge_scratch_bytes_*(size_power, len) bytes.  The merge runs on a
+ * table with HN_INFO_ORDERED and no TOMBSTONEs; when the table is not such,
+ * a region holds more than 3,072 old slots or 1,536 batch pairs (skewed
+ * keys), the wrap past the table's end would move region 0's spill, or the
+ * union would fill the table, new_buckets receives instead, on the device,
+ * the old table with the batch inserted by probing: hn_insert_*'s result
+ * (its live set, arrival order, err = SUCCESS or FAIL, HN_INFO_ORDERED
+ * cleared; of a key repeated in the batch one of its values, as there).  Either way the result is in new_buckets and
+ * *info describes it.  len == 0 copies the table.  `vals` NULL stores each
+ * key's index in the batch; the _pairs variant reads hash_pair elements. */
 HN_API size_t hn_merge_scratch_bytes_u32(int size_power, size_t len);
 HN_API size_t hn_merge_scratch_bytes_u64(int size_power, size_t len);
 HN_API int hn_merge_u32(const void* keys, const void* vals, const void* old_buckets, void* new_buckets, hn_info* info,
