@@ -427,6 +427,8 @@ def main():
                            rscratch if (reorder or use_build) else None, bscratch)
     if bscratch is not None:
         del bscratch
+    if not args.no_extra:
+        extra["merge_insert"] = run_merge_insert(torch, L, dev, S, P, dk, dv, n, N, sp, seed)
 
     if not args.no_compaction:
         extra["compaction"] = run_compaction(torch, L, dev, S, P, buckets, int(info.cpu().numpy()[1]))
@@ -518,6 +520,83 @@ def calib_sweep(torch, L, dev, S, P):
         out["points"].append({"gran": gran, "per_lane": True, "GBps": round((1 << 34) / (best * 1e-3) / 1e9, 1),
                               "Greads_per_s": round(n_access / (best * 1e-3) / 1e9, 2)})
     return out
+
+
+def run_merge_insert(torch, L, dev, S, P, dk, dv, n, N, sp, seed):
+    """A large insert into a half-full map: about 0.45 of the table (120 M
+    keys at 2^28 slots) merged into a table holding N/2 keys in Robin Hood
+    order (the bulk build's), hn_merge_u64 into a second array, best of 3
+    after a warm-up; beside it the probing insert of the same batch into a
+    copy of the same table.  Checks: fill, the order flag, and every key of
+    the union retrieving its value from the merged table."""
+    from hashinator_amd import _lib
+    from hashinator_amd import workloads as Wl
+
+    n_old = N // 2
+    n_new = min(120_000_000, int(0.447 * N))
+    ok_, ov_ = dk[:n_old], dv[:n_old]
+    absent = Wl.absent64(n_new, seed, n)  # disjoint from the map's keys
+    nk = torch.from_numpy(absent.view(np.int64)).to(dev)
+    nv = torch.from_numpy(Wl.values(absent, seed + 1).view(np.int64)).to(dev)
+    del absent
+    old = torch.empty((N, 2), dtype=torch.int64, device=dev)
+    info = torch.zeros(5, dtype=torch.int64, device=dev)
+    _lib.check(L.hn_info_init(P(info), sp, S), "info_init")
+    bsb = L.hn_build_scratch_bytes_u64(sp, n_old)
+    bs = torch.empty(bsb, dtype=torch.uint8, device=dev)
+    _lib.check(L.hn_build_u64(P(ok_), P(ov_), P(old), P(info), n_old, sp, P(bs), bsb, S, None), "build")
+    torch.cuda.synchronize()
+    del bs
+    info0 = info.clone()
+    nb = torch.empty_like(old)
+    msb = L.hn_merge_scratch_bytes_u64(sp, n_new)
+    ms = torch.empty(msb, dtype=torch.uint8, device=dev)
+    stream = torch.cuda.current_stream()
+
+    def merge():
+        info.copy_(info0)
+        _lib.check(L.hn_merge_u64(P(nk), P(nv), P(old), P(nb), P(info), n_new, sp, P(ms), msb, S), "merge")
+
+    merge()  # first call: code objects, occupancy queries
+    torch.cuda.synchronize()
+    best = float("inf")
+    for _ in range(3):
+        info.copy_(info0)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        _lib.check(L.hn_merge_u64(P(nk), P(nv), P(old), P(nb), P(info), n_new, sp, P(ms), msb, S), "merge")
+        e1.record(stream)
+        torch.cuda.synchronize()
+        best = min(best, e0.elapsed_time(e1))
+    inf = info.cpu().numpy()
+    fill, ordered = int(inf[1]), bool((int(inf[4]) >> 32) & _lib.HN_INFO_ORDERED)
+    qk = torch.cat([ok_, nk])
+    qv = torch.cat([ov_, nv])
+    out = torch.full_like(qk, MISS_VAL)
+    hit = torch.empty(qk.numel(), dtype=torch.uint8, device=dev)
+    _lib.check(L.hn_retrieve_u64(P(qk), P(out), P(hit), P(nb), P(info), qk.numel(), S, None), "retrieve")
+    torch.cuda.synchronize()
+    ok = fill == n_old + n_new and ordered and bool(hit.bool().all()) and bool(torch.equal(out, qv))
+    del out, hit, qk, qv, ms
+    b2 = old.clone()
+    info.copy_(info0)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    _lib.check(L.hn_insert_u64(P(nk), P(nv), P(b2), P(info), n_new, S, None), "insert")
+    e1.record(stream)
+    torch.cuda.synchronize()
+    t_probe = e0.elapsed_time(e1)
+    pinf = info.cpu().numpy()
+    return {
+        "workload": f"insert {n_new} new keys into Hashmap<u64,u64> sp={sp} holding {n_old} keys (Robin Hood order)",
+        "merge_ms": round(best, 3), "merge_Gkeys_per_s": round(n_new / best / 1e6, 2),
+        "merge_scratch_MiB": round(msb / 2**20, 1), "overflow_after_merge": int(inf[2]),
+        "probing_insert_ms": round(t_probe, 3), "probing_Gkeys_per_s": round(n_new / t_probe / 1e6, 2),
+        "overflow_after_probing": int(pinf[2]), "speedup": round(t_probe / best, 2),
+        "fill_after": fill, "ordered_after": ordered, "ok": ok and int(pinf[1]) == n_old + n_new,
+    }
 
 
 def run_compaction(torch, L, dev, S, P, buckets, fill):

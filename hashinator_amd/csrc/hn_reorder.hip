@@ -48,7 +48,10 @@ __global__ void k_reorder_begin(const hn_info* info, ReorderHeader* h, uint64_t 
       h->n_over = 0;
       h->fail = 0;
       h->max_probe = 0;
-      h->skip = ((gate && *gate == 0) || info->tombstone_counter != 0 || info->fill >= n) ? 1u : 0u;
+      h->skip = ((gate && *gate == 0) || info->tombstone_counter != 0 || info->fill >= n ||
+                 (info->reserved1 & kInfoOrdered))
+                    ? 1u
+                    : 0u; // an ordered table is its own reorder
    }
 }
 

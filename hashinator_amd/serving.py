@@ -90,7 +90,7 @@ class _Map:
 
 class ServingEngine:
     def __init__(self, sps, key_bits=64, n_streams=8, hbm_budget_bytes=32 << 30, grow_lf=0.85, device="cuda",
-                 tune=None, build_rehash_min=None):
+                 tune=None, build_rehash_min=None, merge_min_keys=None, merge_min_frac=0.25):
         self.kb = key_bits
         self.sfx = "u32" if key_bits == 32 else "u64"
         self.slot = 8 if key_bits == 32 else 16
@@ -101,6 +101,12 @@ class ServingEngine:
         # grow maps of at least this many keys by the bulk rehash (None: never;
         # its scratch, ~16 B per old slot, is an allocation on the request path)
         self.build_rehash_min = build_rehash_min
+        # inserts of at least this many keys and merge_min_frac of the table
+        # (after any growth) take the reorder + merge into a fresh array
+        # (hn_merge_*: the insert + reorder image; a table the merge cannot
+        # take gets the probing insert's result from the same call); None: never
+        self.merge_min_keys = merge_min_keys
+        self.merge_min_frac = merge_min_frac
         self.budget = hbm_budget_bytes
         self.tune = C.byref(tune) if tune is not None else None
         self.maps = []
@@ -118,6 +124,7 @@ class ServingEngine:
         self.inflight = []  # (event, stream index, map id, [requests], bytes, keepalive tensors, t_admit)
         self.bytes_inflight = 0
         self.rehashes = 0
+        self.merges = 0
         self.launches = 0
         self.completed = 0
 
@@ -148,6 +155,15 @@ class ServingEngine:
         sc2 = torch.empty(sb2, dtype=torch.uint8, device=self.dev)
         check(getattr(L, f"hn_rehash_build_{sfx}")(nb.data_ptr(), sp + 1, nb2.data_ptr(), sp + 2, info.data_ptr(),
                                                    sc2.data_ptr(), sb2, s), "warm-up")
+        rsb = L.hn_reorder_scratch_bytes(sp + 2)
+        rsc = torch.empty(rsb, dtype=torch.uint8, device=self.dev)
+        check(getattr(L, f"hn_reorder_{sfx}")(nb2.data_ptr(), info.data_ptr(), sp + 2, rsc.data_ptr(), rsb, s),
+              "warm-up")
+        nb3 = torch.empty_like(nb2)
+        msb = getattr(L, f"hn_merge_scratch_bytes_{sfx}")(sp + 2, k.numel())
+        msc = torch.empty(msb, dtype=torch.uint8, device=self.dev)
+        check(getattr(L, f"hn_merge_{sfx}")(k.data_ptr(), k.data_ptr(), nb2.data_ptr(), nb3.data_ptr(),
+                                             info.data_ptr(), k.numel(), sp + 2, msc.data_ptr(), msb, s), "warm-up")
         torch.cuda.synchronize()
         # pinned result buffers come from torch's caching host allocator: fill
         # its size classes (powers of two, 4 KiB .. 64 MiB, two of each) now
@@ -176,7 +192,18 @@ class ServingEngine:
         if kind == INSERT and self._needs_grow(m, n):
             new_sp = self._grow_sp(m, n)
             b += (1 << new_sp) * self.slot + self._rehash_scratch(m, new_sp)[0]
+        if kind == INSERT:
+            sp = self._grow_sp(m, n) if self._needs_grow(m, n) else m.sp
+            if self._merges(sp, n):
+                b += (1 << sp) * self.slot + self._merge_scratch(sp, n)
         return b
+
+    def _merges(self, sp, n):
+        return self.merge_min_keys is not None and n >= self.merge_min_keys and n >= self.merge_min_frac * (1 << sp)
+
+    def _merge_scratch(self, sp, n):
+        return (getattr(self.L, f"hn_merge_scratch_bytes_{self.sfx}")(sp, n)
+                + (self.L.hn_reorder_scratch_bytes(sp) + 255) // 256 * 256)
 
     def _needs_grow(self, m: _Map, n):
         return m.fill_upper + n > self.grow_lf * (1 << m.sp)
@@ -248,8 +275,24 @@ class ServingEngine:
                 hv = self._host_input(reqs, "vals")
                 dv = hv.to(self.dev, non_blocking=True)
                 keep += [hv, dv]
-                check(getattr(L, f"hn_insert_{sfx}")(dk.data_ptr(), dv.data_ptr(), m.buckets.data_ptr(),
-                                                      m.info.data_ptr(), n, s.cuda_stream, self.tune), "insert")
+                if self._merges(m.sp, n):
+                    # reorder (a no-op on an ordered table), then the merge into a fresh array
+                    rsb = (L.hn_reorder_scratch_bytes(m.sp) + 255) // 256 * 256
+                    msb = getattr(L, f"hn_merge_scratch_bytes_{sfx}")(m.sp, n)
+                    sc = torch.empty(rsb + msb, dtype=torch.uint8, device=self.dev)
+                    check(getattr(L, f"hn_reorder_{sfx}")(m.buckets.data_ptr(), m.info.data_ptr(), m.sp,
+                                                           sc.data_ptr(), rsb, s.cuda_stream), "reorder")
+                    nb = torch.empty((1 << m.sp, 2), dtype=_TDT[self.kb], device=self.dev)
+                    check(getattr(L, f"hn_merge_{sfx}")(dk.data_ptr(), dv.data_ptr(), m.buckets.data_ptr(),
+                                                         nb.data_ptr(), m.info.data_ptr(), n, m.sp,
+                                                         sc.data_ptr() + rsb, msb, s.cuda_stream), "merge")
+                    m.buckets.record_stream(s)
+                    keep += [m.buckets, sc]
+                    m.buckets = nb
+                    self.merges += 1
+                else:
+                    check(getattr(L, f"hn_insert_{sfx}")(dk.data_ptr(), dv.data_ptr(), m.buckets.data_ptr(),
+                                                          m.info.data_ptr(), n, s.cuda_stream, self.tune), "insert")
                 m.fill_upper += n
             elif kind == RETRIEVE:
                 dv = torch.zeros(n, dtype=_TDT[self.kb], device=self.dev)

@@ -25,8 +25,9 @@ def build(seed=1):
     return pre, reqs
 
 
-def engine(pre, n_streams=8, budget=1 << 30, bulk_rehash_min=1 << 19):
-    e = ServingEngine(SPS, key_bits=64, n_streams=n_streams, hbm_budget_bytes=budget, build_rehash_min=bulk_rehash_min)
+def engine(pre, n_streams=8, budget=1 << 30, bulk_rehash_min=1 << 19, merge_min_keys=None):
+    e = ServingEngine(SPS, key_bits=64, n_streams=n_streams, hbm_budget_bytes=budget, build_rehash_min=bulk_rehash_min,
+                      merge_min_keys=merge_min_keys)
     for m, (k, v) in enumerate(pre):
         e.preload(m, k, v)
     return e
@@ -38,14 +39,17 @@ def clone(reqs):
     return [Request(r.rid, r.map_id, r.kind, r.keys, r.vals, r.arrival) for r in reqs]
 
 
-@pytest.mark.parametrize("realtime,budget,bulk_rehash_min", [(False, 1 << 30, 1 << 19), (True, 1 << 30, 1 << 19),
-                                                             (False, 200_000, 1 << 19), (False, 1 << 30, 0)])
-def test_mix_equals_alone_equals_oracle(gpu, realtime, budget, bulk_rehash_min):
-    # bulk_rehash_min 0: every growth is the bulk rehash (one build from the live slots)
+@pytest.mark.parametrize("realtime,budget,bulk_rehash_min,merge_min_keys",
+                         [(False, 1 << 30, 1 << 19, None), (True, 1 << 30, 1 << 19, None),
+                          (False, 200_000, 1 << 19, None), (False, 1 << 30, 0, None), (False, 1 << 30, 0, 64),
+                          (True, 1 << 30, 0, 64)])
+def test_mix_equals_alone_equals_oracle(gpu, realtime, budget, bulk_rehash_min, merge_min_keys):
+    # bulk_rehash_min 0: every growth is the bulk rehash (one build from the live slots);
+    # merge_min_keys 64: inserts of a quarter of a table take the reorder + merge
     pre, reqs = build(seed=3)
     mixed = clone(reqs)
     alone = clone(reqs)
-    e1 = engine(pre, budget=budget, bulk_rehash_min=bulk_rehash_min)
+    e1 = engine(pre, budget=budget, bulk_rehash_min=bulk_rehash_min, merge_min_keys=merge_min_keys)
     wall = e1.serve(mixed, realtime=realtime)
     rep = latency_report(mixed, wall)
     assert rep["requests"] == len(reqs) and all(r.t_done >= r.arrival for r in mixed)
@@ -80,6 +84,8 @@ def test_mix_equals_alone_equals_oracle(gpu, realtime, budget, bulk_rehash_min):
         assert i1[m][1] == i2[m][1] == models[m].info.fill
         assert (i1[m][4] & 0xFFFFFFFF) == 0
     assert e1.rehashes >= 1  # the stream crosses the 0.85 ceiling somewhere
+    if merge_min_keys is not None:
+        assert e1.merges >= 1
 
 
 def test_serving_full_size(gpu):
