@@ -873,7 +873,10 @@ def run_serving(torch, rates=(1000.0, 4000.0), n_requests=300):
     """The serving loop: 64 resident Hashmap<u64,u64> of 2^20..2^26 slots
     preloaded to LF 0.4; a Poisson stream of insert/retrieve/erase batches
     (45/45/10%, log-uniform 1k..4M keys, 30% repeats of live keys) from host
-    memory; growth at LF 0.85; admission by a 64 GiB in-flight budget."""
+    memory; growth at LF 0.85; admission by a 64 GiB in-flight budget.  Each
+    rate runs twice: inserts by probing and growth by reinsertion, then
+    inserts of at least 2^20 keys and a quarter of their table by the
+    reorder + merge and growth of maps of 2^20+ keys by the bulk rehash."""
     from hashinator_amd import workloads as Wl
     from hashinator_amd.serving import ServingEngine, latency_report, make_requests
 
@@ -886,8 +889,9 @@ def run_serving(torch, rates=(1000.0, 4000.0), n_requests=300):
         k = Wl.unique_keys(int(0.4 * (1 << sp)), 64, 7000 + m)
         pre.append(k)
     gen = time.perf_counter() - t0
-    for rate in rates:
-        eng = ServingEngine(sps, key_bits=64, n_streams=8, hbm_budget_bytes=64 << 30)
+    for rate, big in [(r, b) for r in rates for b in (False, True)]:
+        eng = ServingEngine(sps, key_bits=64, n_streams=8, hbm_budget_bytes=64 << 30,
+                            build_rehash_min=(1 << 20) if big else None, merge_min_keys=(1 << 20) if big else None)
         for m, k in enumerate(pre):
             eng.preload(m, k, Wl.values(k, 1))
         torch.cuda.synchronize()
@@ -896,8 +900,8 @@ def run_serving(torch, rates=(1000.0, 4000.0), n_requests=300):
         gen += time.perf_counter() - t1
         wall = eng.serve(reqs, realtime=True)
         rep = latency_report(reqs, wall)
-        rep.update({"rate_hz": rate, "rehashes": eng.rehashes, "launches": eng.launches,
-                    "merged_requests": n_requests - eng.launches})
+        rep.update({"rate_hz": rate, "bulk_paths": big, "rehashes": eng.rehashes, "merge_inserts": eng.merges,
+                    "launches": eng.launches, "merged_requests": n_requests - eng.launches})
         # every retrieve and every final fill against a sequential host replay (not timed)
         t2 = time.perf_counter()
         infos = eng.infos()
