@@ -73,6 +73,7 @@ class Hashmap {
    float _auto_reorder = 1.0f / 16; // reorder after inserts of at least this fraction of the buckets
    float _auto_build = 1.0f / 16;   // bulk-build inserts of at least this fraction into an empty table
    size_t _build_min_keys = size_t(1) << 17; // ... and of at least this many keys (below, launches dominate)
+   float _auto_merge = 0.0f; // merge inserts of at least this fraction into a table holding keys (0: off)
 
    uint32_t hash(KEY_TYPE in) const { return static_cast<uint32_t>(HashFunction::_hash(in, _info.sizePower)); }
    const hn_tune* tune() const { return _has_tune ? &_tune : nullptr; }
@@ -827,7 +828,53 @@ public:
       _build_min_keys = min_keys;
    }
 
+   /* Fraction of the bucket count an insert into a table that already holds
+    * keys (and no tombstones) must reach to take the merge (hn_merge_*: the
+    * insert + reorder image written into a fresh bucket array, about a
+    * table's read and write; it pays from about a quarter of the table);
+    * a table not in Robin Hood order is reordered first when the batch is
+    * at least 4/3 of the fraction.  Off by default: like a resize, the
+    * merge replaces the bucket array, so a device copy from upload() must
+    * be taken again after such an insert.  The batch must also hold the
+    * build's min_keys. */
+   void setAutoMerge(float fraction) noexcept { _auto_merge = fraction; }
+
 private:
+   /* A large insert into a table holding keys, by the merge into a fresh
+    * array; false when it does not apply.  The device writes the probing
+    * insert's result instead where the merge cannot take the table, so a
+    * true return means the insert is done. */
+   bool try_merge(const void* keys, const void* vals, const void* pairs, size_t len, hipStream_t s) {
+      const double N = static_cast<double>(_nbuckets);
+      const int sp = _info.sizePower;
+      if (_auto_merge <= 0.0f || static_cast<double>(len) < _auto_merge * N || len < _build_min_keys ||
+          len >= _nbuckets || sp > 30 || _info.fill == 0 || _info.tombstoneCounter != 0)
+         return false;
+      if (!ordered()) {
+         if (static_cast<double>(len) < _auto_merge * N * 4.0 / 3.0) return false;
+         reorder(s);
+      }
+      const size_t sb = W64 ? hn_merge_scratch_bytes_u64(sp, len) : hn_merge_scratch_bytes_u32(sp, len);
+      if (sb == 0) return false;
+      pair_t* fresh = nullptr;
+      SPLIT_CHECK_ERR(hipMalloc(&fresh, _nbuckets * sizeof(pair_t)));
+      void* scratch = nullptr;
+      SPLIT_CHECK_ERR(hipMallocAsync(&scratch, sb, s));
+      if (pairs)
+         chk(W64 ? hn_merge_pairs_u64(pairs, _buckets, fresh, dinfo(), len, sp, scratch, sb, st(s))
+                 : hn_merge_pairs_u32(pairs, _buckets, fresh, dinfo(), len, sp, scratch, sb, st(s)));
+      else
+         chk(W64 ? hn_merge_u64(keys, vals, _buckets, fresh, dinfo(), len, sp, scratch, sb, st(s))
+                 : hn_merge_u32(keys, vals, _buckets, fresh, dinfo(), len, sp, scratch, sb, st(s)));
+      SPLIT_CHECK_ERR(hipFreeAsync(scratch, s));
+      SPLIT_CHECK_ERR(hipStreamSynchronize(s));
+      SPLIT_CHECK_ERR(hipFree(_buckets));
+      _buckets = fresh;
+      changed(s);
+      if (_auto_reorder > 0.0f && !ordered() && static_cast<double>(len) >= _auto_reorder * N) reorder(s);
+      return true;
+   }
+
    /* The bulk path; false when it does not apply or the device declined it
     * (then the table is empty and the probing insert runs). */
    bool try_build(const void* keys, const void* vals, const void* pairs, size_t len, hipStream_t s) {
@@ -890,7 +937,7 @@ public:
       }
       const int64_t neededPowerSize = std::ceil(std::log2((_info.fill + len) * (1.0 / targetLF)));
       if (neededPowerSize > _info.sizePower) resize(static_cast<int>(neededPowerSize), targets::device, s);
-      if (try_build(keys, vals, nullptr, len, s)) return;
+      if (try_build(keys, vals, nullptr, len, s) || try_merge(keys, vals, nullptr, len, s)) return;
       DeviceHasher::insert(keys, vals, _buckets, _dinfo, len, s, tune());
       after_insert(len, s);
    }
@@ -902,7 +949,7 @@ public:
       }
       const int64_t neededPowerSize = std::ceil(std::log2((_info.fill + len) * (1.0 / targetLF)));
       if (neededPowerSize > _info.sizePower) resize(static_cast<int>(neededPowerSize), targets::device, s);
-      if (try_build(keys, nullptr, nullptr, len, s)) return;
+      if (try_build(keys, nullptr, nullptr, len, s) || try_merge(keys, nullptr, nullptr, len, s)) return;
       DeviceHasher::insertIndex(keys, _buckets, _dinfo, len, s, tune());
       after_insert(len, s);
    }
@@ -914,7 +961,7 @@ public:
       }
       const int64_t neededPowerSize = std::ceil(std::log2(((_info.fill) + len) * (1.0 / targetLF)));
       if (neededPowerSize > _info.sizePower) resize(static_cast<int>(neededPowerSize), targets::device, s);
-      if (try_build(nullptr, nullptr, src, len, s)) return;
+      if (try_build(nullptr, nullptr, src, len, s) || try_merge(nullptr, nullptr, src, len, s)) return;
       DeviceHasher::insert(src, _buckets, _dinfo, len, s, tune());
       after_insert(len, s);
    }

@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <map>
 #include <random>
 #include <unordered_map>
 #include <vector>
@@ -645,6 +646,68 @@ bool splitvector_device_copy_if() {
    return true;
 }
 
+/* setAutoMerge: a quarter of the table inserted into a half-full ordered
+ * map (with keys already present and keys repeated in the batch) takes the
+ * merge; every key then retrieves its last value, the fill is the union's
+ * and the map stays ordered. */
+template <typename K>
+bool merge_insert() {
+   using Map = Hashmap<K, K>;
+   const int power = 16;
+   const size_t N = size_t(1) << power, n_old = N / 2, n_new = N / 4;
+   split::SplitVector<K> ok(n_old), ov(n_old), nk(n_new), nv(n_new);
+   std::map<K, K> expect;
+   for (size_t i = 0; i < n_old; ++i) {
+      ok[i] = static_cast<K>(i * 2654435761ull + 17) & (std::numeric_limits<K>::max() >> 2);
+      ov[i] = static_cast<K>(i);
+      expect[ok[i]] = ov[i];
+   }
+   std::mt19937_64 rng(7);
+   for (size_t i = 0; i < n_new; ++i) {
+      K k;
+      if (i % 10 == 0)
+         k = ok[rng() % n_old]; // already present: overwritten
+      else if (i % 17 == 0 && i > 0)
+         k = nk[rng() % i]; // repeated in the batch: the last value stays
+      else
+         k = static_cast<K>((i * 40503ull + 5) | (K(1) << (sizeof(K) * 8 - 3)));
+      nk[i] = k;
+      nv[i] = static_cast<K>(1000000 + i);
+      expect[k] = nv[i];
+   }
+   Map hmap(power);
+   hmap.setAutoBuild(1.0f / 16, 1);
+   hmap.setAutoMerge(0.25f);
+   ok.optimizeGPU();
+   ov.optimizeGPU();
+   nk.optimizeGPU();
+   nv.optimizeGPU();
+   hmap.insert(ok.data(), ov.data(), n_old, 1.0f);
+   CHECK(hmap.ordered() && hmap.size() == n_old);
+   auto* before = hmap.template expose_bucketdata<false>();
+   CHECK(hmap.reorder()); // expose_bucketdata dropped the flag; the reorder sets it again
+   hmap.insert(nk.data(), nv.data(), n_new, 1.0f);
+   CHECK(hmap.peek_status() == status::success);
+   CHECK(hmap.template expose_bucketdata<false>() != before); // a fresh array: the merge ran
+   CHECK(hmap.size() == expect.size());
+   split::SplitVector<K> q(expect.size()), out(expect.size());
+   size_t i = 0;
+   for (auto& kv : expect) q[i++] = kv.first;
+   q.optimizeGPU();
+   out.optimizeGPU();
+   uint8_t* hit = nullptr;
+   SPLIT_CHECK_ERR(hipMallocManaged(&hit, expect.size()));
+   hmap.retrieve(q.data(), out.data(), expect.size(), 0, hit);
+   SPLIT_CHECK_ERR(hipDeviceSynchronize());
+   i = 0;
+   for (auto& kv : expect) {
+      CHECK(hit[i] == 1 && out[i] == kv.second);
+      ++i;
+   }
+   SPLIT_CHECK_ERR(hipFree(hit));
+   return true;
+}
+
 int main() {
    struct Case {
       const char* name;
@@ -667,6 +730,8 @@ int main() {
        {"splitvector_device_script<unified>", splitvector_device_script<split::split_unified_allocator<uint32_t>>},
        {"splitvector_device_script<device>", splitvector_device_script<split::split_device_allocator<uint32_t>>},
        {"splitvector_device_copy_if", splitvector_device_copy_if},
+       {"merge_insert<u32>", merge_insert<uint32_t>},
+       {"merge_insert<u64>", merge_insert<uint64_t>},
    };
    for (auto& c : cases) {
       printf("%s ...\n", c.name);
