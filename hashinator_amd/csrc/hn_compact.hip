@@ -26,7 +26,7 @@ namespace hn {
 // atomic on one word) and its step in the look-back chain.  At 2^28 elements,
 // 256-thread tiles ran copy_if in 2.0 ms and the u32 scan in 1.0 ms; 1024
 // threads with 8 pairs each (copy_if) and 256 bytes each (scan) take 1.33 and
-// 0.60 ms.  Fewer pairs per thread (4) or more (16) were slower for copy_if.
+// 0.41 ms.  Fewer pairs per thread (4) or more (16) were slower for copy_if.
 constexpr int kBS = 1024;        // threads per tile
 constexpr int kWaves = kBS / kWave;
 constexpr int kCompactIPT = 8;   // elements per thread in a compaction tile
@@ -80,6 +80,43 @@ __device__ __forceinline__ uint64_t lb_read(const Lookback& L, uint64_t p, uint6
    if ((a >> 62) != (b >> 62)) return 0;
    *v = ((b & 0xFFFFFFFFull) << 32) | (a & 0xFFFFFFFFull);
    return a >> 62;
+}
+
+// One-word form for values below 2^62 (element counts, the compaction's):
+// state in the top two bits, the value below, so a predecessor costs one
+// device-scope load, not two (copy_if of 2^28 pairs: 1.32 -> 1.23 ms).
+__device__ __forceinline__ void lb1_publish(const Lookback& L, uint64_t tile, uint64_t flag, uint64_t v) {
+   __hip_atomic_store(&L.lo[tile], (flag << 62) | v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ uint64_t lookback_prefix1(const Lookback& L, uint32_t tile, uint64_t mine) {
+   const int lane = __lane_id();
+   if (tile == 0) {
+      if (lane == 0) lb1_publish(L, 0, kFlagInc, mine);
+      return 0;
+   }
+   if (lane == 0) lb1_publish(L, tile, kFlagAgg, mine);
+   uint64_t prefix = 0;
+   int64_t hi = static_cast<int64_t>(tile) - 1;
+   for (;;) {
+      const int64_t p = hi - lane;
+      uint64_t w = kFlagInc << 62;
+      if (p >= 0) w = __hip_atomic_load(&L.lo[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const uint64_t f = w >> 62;
+      const uint64_t not_ready = __ballot(f == 0);
+      const uint64_t inc_mask = __ballot(f == kFlagInc);
+      const int first_inc = inc_mask ? first_bit(inc_mask) : 64;
+      const uint64_t upto = first_inc >= 63 ? ~0ull : ((2ull << first_inc) - 1ull);
+      if (not_ready & upto) {
+         __builtin_amdgcn_s_sleep(1);
+         continue;
+      }
+      const uint64_t take = (lane <= first_inc) ? (w & ((1ull << 62) - 1)) : 0;
+      prefix += wave_sum(take);
+      if (first_inc < 64) break;
+      hi -= 64;
+   }
+   if (lane == 0) lb1_publish(L, tile, kFlagInc, prefix + mine);
+   return prefix;
 }
 
 // Exclusive prefix of tile `tile` whose own aggregate is `mine`; called by
@@ -227,7 +264,7 @@ __global__ __launch_bounds__(kBS) void k_compact(const In* __restrict__ in, uint
          run += c[i];
       }
       const uint64_t total = static_cast<uint32_t>(__shfl(static_cast<int>(incl), kWave - 1, kWave));
-      const uint64_t pre = lookback_prefix(L, tile, total);
+      const uint64_t pre = lookback_prefix1(L, tile, total);
       if (lane == 0) {
          s_prefix = pre;
          if (tile == tiles - 1 && d_count) *d_count = pre + total;
@@ -247,63 +284,100 @@ __global__ __launch_bounds__(kBS) void k_compact(const In* __restrict__ in, uint
 
 // ------------------------------------------------------------ scan
 
-// Each thread owns IPT consecutive elements (kScanBytes); a tile is kBS*IPT.
+// A tile is kRows rows of kBS 16-byte vectors; thread t holds vector t of
+// every row, so each load and store of a wave covers 1 KB of consecutive
+// bytes.  Row sums are scanned across the wave, the (row, wave) totals in
+// LDS, in row-major order.  (With 256 consecutive bytes per thread instead,
+// a wave's 16-byte loads land 256 B apart and the 2^28-element u32 scan took
+// 0.60 ms; striped, 0.41 ms.)
+constexpr int kRows = kScanBytes / 16;
 template <typename T, bool VEC>
 __global__ __launch_bounds__(kBS) void k_scan(const T* in, T* out, uint64_t n, Lookback L, uint64_t tiles) {
-   constexpr int IPT = kScanBytes / sizeof(T);
+   constexpr int V = 16 / sizeof(T);
+   constexpr int IPT = kRows * V;
    constexpr uint64_t TILE = static_cast<uint64_t>(kBS) * IPT;
    __shared__ uint32_t s_tile;
-   __shared__ T s_wave[kWaves];
+   __shared__ T s_rw[kRows * kWaves];
+   __shared__ T s_part[kRows * kWaves / kWave];
    __shared__ uint64_t s_prefix;
    const int lane = __lane_id();
    const int wave = threadIdx.x / kWave;
    if (threadIdx.x == 0) s_tile = __hip_atomic_fetch_add(L.ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
    __syncthreads();
    const uint32_t tile = s_tile;
-   const uint64_t i0 = static_cast<uint64_t>(tile) * TILE + static_cast<uint64_t>(threadIdx.x) * IPT;
-
+   const uint64_t t0 = static_cast<uint64_t>(tile) * TILE;
+   const bool full = VEC && t0 + TILE <= n;
    T x[IPT];
-   if (VEC && i0 + IPT <= n) {
-      const uint4* src = reinterpret_cast<const uint4*>(in + i0);
+   if (full) {
+      const uint4* src = reinterpret_cast<const uint4*>(in + t0);
 #pragma unroll
-      for (int c = 0; c < kScanBytes / 16; ++c) reinterpret_cast<uint4*>(x)[c] = src[c];
+      for (int c = 0; c < kRows; ++c) reinterpret_cast<uint4*>(x)[c] = src[c * kBS + threadIdx.x];
    } else {
 #pragma unroll
-      for (int j = 0; j < IPT; ++j) x[j] = (i0 + j < n) ? in[i0 + j] : T(0);
-   }
-   T sum = 0;
+      for (int c = 0; c < kRows; ++c)
 #pragma unroll
-   for (int j = 0; j < IPT; ++j) sum += x[j];
-   const T incl = wave_incl_scan<T>(sum);
-   if (lane == kWave - 1) s_wave[wave] = incl;
+         for (int k = 0; k < V; ++k) {
+            const uint64_t i = t0 + (static_cast<uint64_t>(c) * kBS + threadIdx.x) * V + k;
+            x[c * V + k] = i < n ? in[i] : T(0);
+         }
+   }
+   T rs[kRows], wi[kRows];
+#pragma unroll
+   for (int c = 0; c < kRows; ++c) {
+      T a = 0;
+#pragma unroll
+      for (int k = 0; k < V; ++k) a += x[c * V + k];
+      rs[c] = a;
+      wi[c] = wave_incl_scan<T>(a);
+      if (lane == kWave - 1) s_rw[c * kWaves + wave] = wi[c];
+   }
    __syncthreads();
-   T wave_pre = 0, tile_total = 0;
-#pragma unroll
-   for (int w = 0; w < kWaves; ++w) {
-      const T c = s_wave[w];
-      if (w < wave) wave_pre += c;
-      tile_total += c;
+   // exclusive scan of the kRows * kWaves (row, wave) totals, row-major
+   constexpr int NE = kRows * kWaves;
+   static_assert(NE % kWave == 0 && NE / kWave <= kWaves, "one entry per thread of the first waves");
+   T e = 0, ei = 0;
+   if (threadIdx.x < NE) {
+      e = s_rw[threadIdx.x];
+      ei = wave_incl_scan<T>(e);
+      if (lane == kWave - 1) s_part[wave] = ei;
    }
+   __syncthreads();
+   if (threadIdx.x < NE) {
+      T before = 0;
+      for (int w = 0; w < wave; ++w) before += s_part[w];
+      s_rw[threadIdx.x] = before + ei - e;
+   }
+   T tile_total = 0;
+#pragma unroll
+   for (int w = 0; w < NE / kWave; ++w) tile_total += s_part[w];
    if (wave == 0) {
       const uint64_t pre = lookback_prefix(L, tile, static_cast<uint64_t>(tile_total));
       if (lane == 0) s_prefix = pre;
    }
    __syncthreads();
-   T run = static_cast<T>(s_prefix) + wave_pre + (incl - sum);
+   const T pre = static_cast<T>(s_prefix);
 #pragma unroll
-   for (int j = 0; j < IPT; ++j) {
-      const T e = x[j];
-      x[j] = run;
-      run += e;
+   for (int c = 0; c < kRows; ++c) {
+      T run = pre + s_rw[c * kWaves + wave] + (wi[c] - rs[c]);
+#pragma unroll
+      for (int k = 0; k < V; ++k) {
+         const T v = x[c * V + k];
+         x[c * V + k] = run;
+         run += v;
+      }
    }
-   if (VEC && i0 + IPT <= n) {
-      uint4* dst = reinterpret_cast<uint4*>(out + i0);
+   if (full) {
+      uint4* dst = reinterpret_cast<uint4*>(out + t0);
 #pragma unroll
-      for (int c = 0; c < kScanBytes / 16; ++c) dst[c] = reinterpret_cast<uint4*>(x)[c];
+      for (int c = 0; c < kRows; ++c) dst[c * kBS + threadIdx.x] = reinterpret_cast<uint4*>(x)[c];
    } else {
 #pragma unroll
-      for (int j = 0; j < IPT; ++j)
-         if (i0 + j < n) out[i0 + j] = x[j];
+      for (int c = 0; c < kRows; ++c)
+#pragma unroll
+         for (int k = 0; k < V; ++k) {
+            const uint64_t i = t0 + (static_cast<uint64_t>(c) * kBS + threadIdx.x) * V + k;
+            if (i < n) out[i] = x[c * V + k];
+         }
    }
 }
 
