@@ -600,7 +600,8 @@ def run_merge_insert(torch, L, dev, S, P, dk, dv, n, N, sp, seed):
 
 
 def run_compaction(torch, L, dev, S, P, buckets, fill):
-    """split::tools on 2^28 hash_pair<u64,u64> {i, random with bit 0 = keep}:
+    """split::tools on 2^28 hash_pair<u64,u64> {i, random with bit 0 = keep},
+    next to a plain streaming copy of the same 4 GiB (hn_calib_stream_copy):
     stable copy_if with the rule (second & 1) == 1, keep drawn Bernoulli(0.3)
     and in alternating runs of Geometric(1/64) lengths; exclusive scan of the
     2^28 uint32 keep flags; extractPattern (valid keys) over the large map's
@@ -636,7 +637,12 @@ def run_compaction(torch, L, dev, S, P, buckets, fill):
     def copy_if():
         _lib.check(L.hn_copy_if_pairs_u64(P(pairs), n, C.byref(pred), P(out), P(cnt), P(sc), sb, S), "copy_if")
 
-    res = {}
+    # the streaming ceiling of this device: a plain 16-byte-vector copy of the
+    # 4 GiB pair array (read + write)
+    t_copy = best(lambda: _lib.check(L.hn_calib_stream_copy(P(pairs), P(out), n * 16, S), "stream copy"))
+    copy_rate = 2 * n * 16 / (t_copy * 1e-3)
+    res = {"stream_copy": {"ms": round(t_copy, 3), "bytes": 2 * n * 16, "GBps": round(copy_rate / 1e9, 1),
+                           "frac_of_8TBps": round(copy_rate / 8e12, 4)}}
     rng = np.random.default_rng(64)
     for mode in ("bernoulli", "clustered"):
         if mode == "bernoulli":
@@ -660,7 +666,8 @@ def run_compaction(torch, L, dev, S, P, buckets, fill):
             base += m
         alg = n * 16 + k * 16
         res[f"copy_if_{mode}"] = {"ms": round(t, 3), "kept": k, "GBps_alg": round(alg / (t * 1e-3) / 1e9, 1),
-                                  "frac_of_8TBps": round(alg / (t * 1e-3) / 8e12, 4), "ok": bool(ok)}
+                                  "frac_of_8TBps": round(alg / (t * 1e-3) / 8e12, 4),
+                                  "frac_of_stream_copy": round(alg / (t * 1e-3) / copy_rate, 4), "ok": bool(ok)}
     flags = keep.to(torch.int32)
     sout = torch.empty_like(flags)
     t = best(lambda: _lib.check(L.hn_exclusive_scan_u32(P(flags), P(sout), n, P(sc), sb, S), "scan"))
@@ -672,7 +679,8 @@ def run_compaction(torch, L, dev, S, P, buckets, fill):
         ok = ok and torch.equal(sout[c0:c0 + (1 << 24)].to(torch.int64), ref)
         run += int(fc.sum().item())
     res["exclusive_scan_u32"] = {"ms": round(t, 3), "GBps_alg": round(8 * n / (t * 1e-3) / 1e9, 1),
-                                 "frac_of_8TBps": round(8 * n / (t * 1e-3) / 8e12, 4), "ok": bool(ok)}
+                                 "frac_of_8TBps": round(8 * n / (t * 1e-3) / 8e12, 4),
+                                 "frac_of_stream_copy": round(8 * n / (t * 1e-3) / copy_rate, 4), "ok": bool(ok)}
     del flags, sout, keep, pairs
     N = buckets.shape[0]
     vpred = _lib.hn_pred(_lib.HN_PRED_VALID, 0, 0, 0)

@@ -148,6 +148,7 @@ SIGNATURES = {
     # include/hashinator_amd_calib.h
     "hn_calib_random_gather": (I, [P, S, I, S, I, C.c_uint64, P, P]),
     "hn_calib_random_gather_lane": (I, [P, S, I, S, C.c_uint64, P, P]),
+    "hn_calib_stream_copy": (I, [P, P, S, P]),
 }
 
 _LIB = None

@@ -30,6 +30,11 @@ HN_API int hn_calib_random_gather(const void* buf, size_t bytes, int gran, size_
 HN_API int hn_calib_random_gather_lane(const void* buf, size_t bytes, int gran, size_t n_access, uint64_t seed,
                                        uint64_t* sink, hn_stream_t s);
 
+/* Streaming copy of `bytes` (a multiple of 16) from src to dst, 16-byte
+ * vectors, grid-stride: the read + write rate a streaming kernel reaches on
+ * this device, the ceiling compaction and scan are held against. */
+HN_API int hn_calib_stream_copy(const void* src, void* dst, size_t bytes, hn_stream_t s);
+
 #ifdef __cplusplus
 }
 #endif
