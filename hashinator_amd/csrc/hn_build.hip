@@ -55,12 +55,18 @@
 namespace hn {
 
 constexpr int kRegionLog = 11;   // 2048 slots per region
-constexpr int kPartBS = 256;     // partition workgroup
-constexpr int kPartIPT = 16;     // pairs per thread and tile (4096-pair tiles: ~8 pairs per 9-bit digit run)
-constexpr int kPartTile = kPartBS * kPartIPT;
+constexpr int kPartBS = 256;     // count workgroup
+constexpr int kPartRounds = 16;  // pairs per thread of a scatter tile
+// Scatter tiles: 8,192 pairs (512 threads, 16-pair runs per 9-bit digit,
+// one block per CU) for inputs of at least kPartLargeMin pairs, 4,096 (256
+// threads, two blocks per CU) below, where the large tile would leave CUs
+// idle.  At 2^28 slots the large tile builds in 9.19 ms against 9.64 ms:
+// longer runs per digit, fewer partial lines; a 2,048-pair tile with three
+// blocks per CU took 10.7 ms.
+constexpr uint32_t kPartTileS = 4096;
+constexpr uint32_t kPartTileL = 8192;
+constexpr uint64_t kPartLargeMin = 1ull << 22;
 constexpr int kPartSub = 8;      // tiles per partition block at most (fewer for small inputs)
-constexpr int kScatBS = 256;     // threads of a scatter block (512 measured the same)
-constexpr int kPartSuper = kPartTile * kPartSub;
 constexpr int kMaxDigitBits = 9; // 512 digits per pass at most
 constexpr int kRegionBS = 512;
 constexpr int kRegionIPT = 5;
@@ -97,6 +103,7 @@ struct BuildPlan {
    int passes;          // partition passes
    int dbits[3];        // digit bits per pass (low digit first)
    uint32_t nblocks;    // partition blocks
+   uint32_t tile;       // pairs per scatter tile (kPartTileS or kPartTileL)
    uint32_t sub;        // tiles per partition block
    size_t off_pairs, off_mat, off_scan, off_start, off_word, off_spec, off_carry, off_mark, off_list, off_blk, total;
 };
@@ -124,13 +131,14 @@ bool build_plan(int sp, uint64_t n, int dcap, BuildPlan* P) {
       left -= P->dbits[i];
    }
    // Tiles per block: kPartSub, or as few as keep the blocks within one
-   // resident round (2 per CU) for inputs below ~2^22 pairs: more blocks
-   // than that would leave a partial second round.
-   const uint64_t tiles = (n + kPartTile - 1) / kPartTile;
-   const uint64_t slots = static_cast<uint64_t>(cu_count()) * 2;
+   // resident round (2 small-tile or 1 large-tile scatter blocks per CU):
+   // more blocks than that would leave a partial second round.
+   P->tile = n >= kPartLargeMin ? kPartTileL : kPartTileS;
+   const uint64_t tiles = (n + P->tile - 1) / P->tile;
+   const uint64_t slots = static_cast<uint64_t>(cu_count()) * (P->tile == kPartTileL ? 1 : 2);
    const uint64_t per = (tiles + slots - 1) / slots;
    P->sub = static_cast<uint32_t>(per < 1 ? 1 : (per > kPartSub ? kPartSub : per));
-   const uint64_t super = static_cast<uint64_t>(kPartTile) * P->sub;
+   const uint64_t super = static_cast<uint64_t>(P->tile) * P->sub;
    const uint64_t nb = (n + super - 1) / super;
    if (nb > 0xFFFFFFFFull) return false;
    P->nblocks = static_cast<uint32_t>(nb);
@@ -228,12 +236,12 @@ __device__ __forceinline__ bool live_key(K k) {
 }
 
 // Per-block digit counts, written digit-major: mat[d * nblocks + block].  A
-// block covers `sub` consecutive tiles (up to kPartSuper pairs), which keeps the
+// block covers `sub` consecutive tiles of `tile` pairs, which keeps the
 // count matrix -- and its scan -- small.
 template <typename K, int SRC, bool FILTER>
 __global__ __launch_bounds__(kPartBS) void k_part_count(PartIn<K> in, uint64_t n_in, const uint64_t* d_n, uint32_t* mat,
-                                                        uint32_t nblocks, uint32_t nsub, int sp, int rlog, int dshift,
-                                                        int dbits, BuildHeader* h) {
+                                                        uint32_t nblocks, uint32_t tile, uint32_t nsub, int sp,
+                                                        int rlog, int dshift, int dbits, BuildHeader* h) {
    __shared__ uint32_t hist[1 << kMaxDigitBits];
    __shared__ uint32_t live_s;
    if (!h->go) return;
@@ -242,7 +250,7 @@ __global__ __launch_bounds__(kPartBS) void k_part_count(PartIn<K> in, uint64_t n
    for (uint32_t d = threadIdx.x; d < D; d += kPartBS) hist[d] = 0;
    if (threadIdx.x == 0) live_s = 0;
    __syncthreads();
-   const uint64_t super = static_cast<uint64_t>(kPartTile) * nsub;
+   const uint64_t super = static_cast<uint64_t>(tile) * nsub;
    const uint64_t t0 = static_cast<uint64_t>(blockIdx.x) * super;
    const uint64_t t1 = std::min<uint64_t>(t0 + super, n);
    uint32_t live = 0;
@@ -268,17 +276,18 @@ __global__ __launch_bounds__(kPartBS) void k_part_count(PartIn<K> in, uint64_t n
 // the earlier rounds (no block barrier per round).  Per-wave counts then give
 // each pair's rank among equal digits in tile order; the tile is staged in
 // LDS in digit order and leaves in runs.
-template <typename K, int SRC, bool FILTER>
-__global__ __launch_bounds__(kScatBS) void k_part_scatter(PartIn<K> in, uint64_t n_in, const uint64_t* d_n,
+template <typename K, int SRC, bool FILTER, int BS>
+__global__ __launch_bounds__(BS) void k_part_scatter(PartIn<K> in, uint64_t n_in, const uint64_t* d_n,
                                                           Slot<K>* out, const uint32_t* offs, uint32_t nblocks,
                                                           uint32_t nsub, int sp, int rlog, int dshift, int dbits,
                                                           const BuildHeader* h) {
-   constexpr int NW = kScatBS / kWave;
-   constexpr int ROUNDS = kPartTile / kScatBS;
+   constexpr int NW = BS / kWave;
+   constexpr int ROUNDS = kPartRounds;
+   constexpr int TILE = BS * kPartRounds;
    constexpr uint32_t DMAX = 1u << kMaxDigitBits;
-   constexpr int DPT = DMAX / kScatBS; // digits per thread in the digit loops
-   static_assert(DPT * kScatBS == DMAX && ROUNDS * kScatBS == kPartTile, "digit and tile split");
-   __shared__ Slot<K> tile_s[kPartTile];
+   constexpr int DPT = DMAX / BS; // digits per thread in the digit loops
+   static_assert(DPT * BS == DMAX && ROUNDS * BS == TILE, "digit and tile split");
+   __shared__ Slot<K> tile_s[TILE];
    __shared__ uint32_t wcnt_s[NW][DMAX];
    __shared__ uint32_t lstart_s[DMAX];
    __shared__ uint32_t goff_s[DMAX];
@@ -290,11 +299,11 @@ __global__ __launch_bounds__(kScatBS) void k_part_scatter(PartIn<K> in, uint64_t
    const uint64_t lt = (1ull << lane) - 1ull;
    const uint32_t d0 = threadIdx.x * DPT; // this thread's digits in the digit loops
    for (uint32_t d = d0; d < d0 + DPT && d < D; ++d) goff_s[d] = offs[static_cast<uint64_t>(d) * nblocks + blockIdx.x];
-   const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kPartTile * nsub;
+   const uint64_t base = static_cast<uint64_t>(blockIdx.x) * TILE * nsub;
    // The pairs of tile `sub` are loaded while tile sub-1 is written out.
    Slot<K> el[ROUNDS];
    auto load_tile = [&](int sub) {
-      const uint64_t w0 = base + static_cast<uint64_t>(sub) * kPartTile + static_cast<uint64_t>(wave) * (kPartTile / NW) +
+      const uint64_t w0 = base + static_cast<uint64_t>(sub) * TILE + static_cast<uint64_t>(wave) * (TILE / NW) +
                           static_cast<uint64_t>(lane);
 #pragma unroll
       for (int r = 0; r < ROUNDS; ++r) {
@@ -304,7 +313,7 @@ __global__ __launch_bounds__(kScatBS) void k_part_scatter(PartIn<K> in, uint64_t
    };
    load_tile(0);
    for (int sub = 0; sub < static_cast<int>(nsub); ++sub) {
-      const uint64_t t0 = base + static_cast<uint64_t>(sub) * kPartTile;
+      const uint64_t t0 = base + static_cast<uint64_t>(sub) * TILE;
       if (t0 >= n) break;
       for (uint32_t d = d0; d < d0 + DPT && d < D; ++d) {
 #pragma unroll
@@ -313,7 +322,7 @@ __global__ __launch_bounds__(kScatBS) void k_part_scatter(PartIn<K> in, uint64_t
       __syncthreads();
       uint32_t dg[ROUNDS], rk[ROUNDS];
       bool ok[ROUNDS];
-      const uint64_t w0 = t0 + static_cast<uint64_t>(wave) * (kPartTile / NW) + lane;
+      const uint64_t w0 = t0 + static_cast<uint64_t>(wave) * (TILE / NW) + lane;
 #pragma unroll
       for (int r = 0; r < ROUNDS; ++r) {
          const uint64_t e = w0 + static_cast<uint64_t>(r) * kWave;
@@ -386,7 +395,7 @@ __global__ __launch_bounds__(kScatBS) void k_part_scatter(PartIn<K> in, uint64_t
       __syncthreads();
       uint32_t cnt = 0; // pairs of this tile that are written: the digit scan's total
       for (int w = 0; w < NW; ++w) cnt += wsum_s[w];
-      for (uint32_t p = threadIdx.x; p < cnt; p += kScatBS) {
+      for (uint32_t p = threadIdx.x; p < cnt; p += BS) {
          const Slot<K> e = tile_s[p];
          const uint32_t d = (region_of(e.key, sp, rlog) >> dshift) & dmask; // recomputed: cheaper than LDS for it
          out[static_cast<uint64_t>(goff_s[d]) + (p - lstart_s[d])] = e;
@@ -1007,11 +1016,17 @@ int launch_part(const BuildPlan& P, int pass, PartIn<K> in, uint64_t n_in, const
    const int dbits = P.dbits[pass];
    const uint64_t D = 1ull << dbits;
    hipLaunchKernelGGL((k_part_count<K, SRC, FILTER>), dim3(P.nblocks), dim3(kPartBS), 0, s, in, n_in, d_n, mat,
-                      P.nblocks, P.sub, P.sp, P.rlog, dshift, dbits, h);
+                      P.nblocks, P.tile, P.sub, P.sp, P.rlog, dshift, dbits, h);
    int rc = hn_exclusive_scan_u32(mat, mat, D * P.nblocks, scan_scratch, scan_bytes, reinterpret_cast<hn_stream_t>(s));
    if (rc) return rc;
-   hipLaunchKernelGGL((k_part_scatter<K, SRC, FILTER>), dim3(P.nblocks), dim3(kScatBS), 0, s, in, n_in, d_n, out, mat,
-                      P.nblocks, P.sub, P.sp, P.rlog, dshift, dbits, h);
+   if (P.tile == kPartTileL)
+      hipLaunchKernelGGL((k_part_scatter<K, SRC, FILTER, kPartTileL / kPartRounds>), dim3(P.nblocks),
+                         dim3(kPartTileL / kPartRounds), 0, s, in, n_in, d_n, out, mat, P.nblocks, P.sub, P.sp, P.rlog,
+                         dshift, dbits, h);
+   else
+      hipLaunchKernelGGL((k_part_scatter<K, SRC, FILTER, kPartTileS / kPartRounds>), dim3(P.nblocks),
+                         dim3(kPartTileS / kPartRounds), 0, s, in, n_in, d_n, out, mat, P.nblocks, P.sub, P.sp, P.rlog,
+                         dshift, dbits, h);
    return static_cast<int>(hipGetLastError());
 }
 

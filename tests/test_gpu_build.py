@@ -70,8 +70,9 @@ def check(img, inf, m):
 
 
 @pytest.mark.parametrize("bits", [32, 64])
+# sp 23 at LF 0.9 (7.5 M keys) takes the 8,192-pair partition tiles (inputs of 2^22 pairs and more).
 @pytest.mark.parametrize("sp,lf", [(1, 0.5), (3, 0.9), (8, 0.99), (11, 0.9), (12, 0.5), (14, 0.9), (16, 0.95),
-                                   (20, 0.9), (21, 0.9)])
+                                   (20, 0.9), (21, 0.9), (23, 0.9)])
 def test_build_image_exact(gpu, bits, sp, lf):
     n = max(1, min((1 << sp) - 1, int(lf * (1 << sp))))
     keys = keyset(n, bits, seed=sp * 7 + bits)
@@ -111,11 +112,11 @@ def test_build_index_and_pairs(gpu, bits):
 
 
 @pytest.mark.parametrize("bits", [32, 64])
-def test_build_duplicates_last_wins(gpu, bits):
-    sp = 14
-    base = keyset(9000, bits, seed=4)
+@pytest.mark.parametrize("sp,nb,nr", [(14, 9000, 3000), (23, 4_000_000, 1_000_000)])  # small and large partition tiles
+def test_build_duplicates_last_wins(gpu, bits, sp, nb, nr):
+    base = keyset(nb, bits, seed=4)
     rng = np.random.default_rng(4)
-    rep = base[rng.choice(base.size, 3000, replace=True)]
+    rep = base[rng.choice(base.size, nr, replace=True)]
     keys = np.concatenate([base, rep])
     rng.shuffle(keys)
     vals = np.arange(keys.size).astype(NDT[bits])
