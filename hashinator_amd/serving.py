@@ -118,8 +118,8 @@ class ServingEngine:
             check(self.L.hn_info_init(info.data_ptr(), int(sp), s0.cuda_stream), "info")
             self.maps.append(_Map(int(sp), b, info, 0, torch.zeros(5, dtype=torch.int64, pin_memory=True)))
         torch.cuda.synchronize()
-        self._warm_up()
         self.streams = [torch.cuda.Stream(device=self.dev) for _ in range(n_streams)]
+        self._warm_up()
         self.free = list(range(n_streams))
         self.inflight = []  # (event, stream index, map id, [requests], bytes, keepalive tensors, t_admit)
         self.bytes_inflight = 0
@@ -168,10 +168,20 @@ class ServingEngine:
         # pinned result buffers come from torch's caching host allocator: fill
         # its size classes (powers of two, 4 KiB .. 64 MiB, two of each) now
         held = [torch.empty(1 << e, dtype=torch.uint8, pin_memory=True) for e in range(12, 27) for _ in range(2)]
-        # and the device-side request buffers come from torch's caching
-        # allocator: one cached 1 GiB segment serves them without hipMalloc
-        held.append(torch.empty(1 << 30, dtype=torch.uint8, device=self.dev))
         del held
+        # Device-side request buffers, fresh bucket arrays and scratch come
+        # from torch's caching allocator, which reuses a freed block only on
+        # the stream that allocated it: leave one cached segment in every
+        # launch stream's pool (up to 2 GiB, within the in-flight budget), so
+        # that the first requests on a stream do not wait for hipMalloc
+        # (~1.5 ms per new segment of tens of MB on the host's launch path).
+        reserve = max(min(self.budget // len(self.streams), 2 << 30), 64 << 20)
+        for st in self.streams:
+            with torch.cuda.stream(st):
+                t = torch.empty(reserve, dtype=torch.uint8, device=self.dev)
+                u = torch.empty(1 << 20, dtype=torch.uint8, device=self.dev)  # the small-block pool (<= 1 MiB)
+            del t, u
+        torch.cuda.synchronize()
 
     # ------------------------------------------------------------ preload (not timed)
     def preload(self, map_id, keys, vals):
