@@ -90,7 +90,7 @@ class _Map:
 
 class ServingEngine:
     def __init__(self, sps, key_bits=64, n_streams=8, hbm_budget_bytes=32 << 30, grow_lf=0.85, device="cuda",
-                 tune=None, build_rehash_min=None, merge_min_keys=None, merge_min_frac=0.25):
+                 tune=None, build_rehash_min=1 << 20, merge_min_keys=1 << 20, merge_min_frac=0.25):
         self.kb = key_bits
         self.sfx = "u32" if key_bits == 32 else "u64"
         self.slot = 8 if key_bits == 32 else 16
@@ -98,8 +98,9 @@ class ServingEngine:
         self.L = _lib.load()
         self.dev = torch.device(device)
         self.grow_lf = grow_lf
-        # grow maps of at least this many keys by the bulk rehash (None: never;
-        # its scratch, ~16 B per old slot, is an allocation on the request path)
+        # grow maps of at least this many keys by the bulk rehash (None: never,
+        # always by reinsertion; its scratch, ~16 B per old slot, comes from
+        # the launch stream's warmed allocator pool)
         self.build_rehash_min = build_rehash_min
         # inserts of at least this many keys and merge_min_frac of the table
         # (after any growth) take the reorder + merge into a fresh array
