@@ -383,7 +383,9 @@ class ServingEngine:
                 dispatched = True
             done = self.completed - done0
             if not dispatched and self.inflight:
-                self.inflight[0][0].synchronize() if not pending else time.sleep(0)
+                # poll every in-flight launch: blocking on the oldest one would
+                # hold back the completions of faster launches behind it
+                time.sleep(0)
         torch.cuda.synchronize()
         return time.perf_counter() - t0
 
