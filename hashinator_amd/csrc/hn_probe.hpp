@@ -129,11 +129,27 @@ __device__ __forceinline__ void insert_range(const InsertArgs<K>& a, const int s
          while (em) {
             const int j = first_bit(em);
             K old = kEmpty;
-            if (gl == j) old = atomic_cas(&sl->key, kEmpty, key);
+            bool whole = false; // the claim stored the value too
+            if (gl == j) {
+               if constexpr (sizeof(Slot<K>) == 8) {
+                  // 32-bit keys: one 64-bit CAS of the whole slot, (EMPTY, value
+                  // seen by this window's load) -> (key, value), so a claim is
+                  // one random store, not two (random stores are the insert's
+                  // bound); if only the value bits differ, the key-only CAS.
+                  const uint64_t exp = (static_cast<uint64_t>(s.val) << 32) | static_cast<uint64_t>(kEmpty);
+                  const uint64_t des = (static_cast<uint64_t>(val) << 32) | static_cast<uint64_t>(key);
+                  const uint64_t o = atomic_cas(reinterpret_cast<uint64_t*>(sl), exp, des);
+                  whole = o == exp;
+                  old = static_cast<K>(o);
+                  if (!whole && old == kEmpty) old = atomic_cas(&sl->key, kEmpty, key);
+               } else {
+                  old = atomic_cas(&sl->key, kEmpty, key);
+               }
+            }
             old = __shfl(old, gbase + j, kWave);
             if (old == kEmpty) {
                if (gl == j) {
-                  sl->val = val;
+                  if (!whole) sl->val = val;
                   ++placed;
                   const uint64_t pr = static_cast<uint64_t>(off) + 1;
                   max_probe = pr > max_probe ? pr : max_probe;
