@@ -136,12 +136,19 @@ __device__ __forceinline__ void insert_range(const InsertArgs<K>& a, const int s
                   // seen by this window's load) -> (key, value), so a claim is
                   // one random store, not two (random stores are the insert's
                   // bound); if only the value bits differ, the key-only CAS.
-                  const uint64_t exp = (static_cast<uint64_t>(s.val) << 32) | static_cast<uint64_t>(kEmpty);
-                  const uint64_t des = (static_cast<uint64_t>(val) << 32) | static_cast<uint64_t>(key);
-                  const uint64_t o = atomic_cas(reinterpret_cast<uint64_t*>(sl), exp, des);
-                  whole = o == exp;
-                  old = static_cast<K>(o);
-                  if (!whole && old == kEmpty) old = atomic_cas(&sl->key, kEmpty, key);
+                  // The reference's pair of 32-bit fields is only 4-byte
+                  // aligned: a bucket array that is not 8-byte aligned takes
+                  // the key-only CAS throughout.
+                  if ((reinterpret_cast<uintptr_t>(sl) & 7u) == 0) {
+                     const uint64_t exp = (static_cast<uint64_t>(s.val) << 32) | static_cast<uint64_t>(kEmpty);
+                     const uint64_t des = (static_cast<uint64_t>(val) << 32) | static_cast<uint64_t>(key);
+                     const uint64_t o = atomic_cas(reinterpret_cast<uint64_t*>(sl), exp, des);
+                     whole = o == exp;
+                     old = static_cast<K>(o);
+                     if (!whole && old == kEmpty) old = atomic_cas(&sl->key, kEmpty, key);
+                  } else {
+                     old = atomic_cas(&sl->key, kEmpty, key);
+                  }
                } else {
                   old = atomic_cas(&sl->key, kEmpty, key);
                }

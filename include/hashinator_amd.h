@@ -103,6 +103,12 @@ enum {
 
 /* ---------------------------------------------------------------- map setup */
 
+/* Bucket arrays are n_slots hash_pair<K,K> in device memory, expected to be
+ * aligned to the slot size (8 bytes for u32, 16 for u64), as hipMalloc and
+ * the library's allocators give.  A 32-bit-key insert claims an EMPTY slot
+ * with one 64-bit CAS of key and value when the slot is 8-byte aligned, and
+ * with a CAS of the key followed by a value store otherwise. */
+
 /* Info(int) (defaults.h:46-48) written to device memory: size_power, fill 0,
  * overflow HN_BUCKET_OVERFLOW, tombstones 0, err HN_STATUS_INVALID. */
 HN_API int hn_info_init(hn_info* info, int size_power, hn_stream_t s);

@@ -96,6 +96,30 @@ def test_insert_set_and_occupancy(gpu, bits, width, lf):
 
 
 @pytest.mark.parametrize("bits", [32, 64])
+@pytest.mark.parametrize("width", [1, 4, 8, 64])
+def test_insert_over_empty_slots_with_garbage_values(gpu, bits, width):
+    # EMPTY slots may hold any value bits (a table the build wrote over
+    # uninitialised memory, or user memory): the 32-bit claim compares the
+    # whole 8-byte slot against the value its window load saw, so it must
+    # still claim such slots, and no EMPTY slot's value may change.
+    sp = 12
+    rng = np.random.default_rng(40 + width)
+    model = fresh_model(sp, bits)
+    model.b[1::2] = rng.integers(0, np.iinfo(NDT[bits]).max, (1 << sp), dtype=NDT[bits])
+    hm = Hashmap(sp, bits, tune=hn_tune(width, 256, 0, 0), auto_reorder=0, auto_build=0)
+    upload(hm, model)
+    before = hm.host_buckets().copy()
+    keys = keyset(int(0.9 * (1 << sp)), bits, seed=width + bits)
+    vals = (keys * NDT[bits](5) + NDT[bits](2)).astype(NDT[bits])
+    hm.insert(keys, vals, targetLF=1.0)
+    model.insert(keys, vals)
+    check_insert_invariants(hm, model, bits)
+    img = hm.host_buckets()
+    still_empty = img[:, 0] == EMPTY[bits]
+    assert np.array_equal(img[still_empty], before[still_empty])
+
+
+@pytest.mark.parametrize("bits", [32, 64])
 def test_insert_duplicates_and_overwrite(gpu, bits):
     sp = 12
     rng = np.random.default_rng(5)
