@@ -16,6 +16,12 @@ cross the load-factor ceiling, 1329-1332 -- here at 0.85).
   when they are next in line together -- both are order-insensitive among
   themselves -- which is the batching; inserts are not merged (a key repeated
   across two inserts must keep the later value).
+* Maps of at least ``build_rehash_min`` keys (default 2^20) grow by the bulk
+  rehash, and inserts of at least ``merge_min_keys`` keys (default 2^20) and
+  ``merge_min_frac`` of their table go through the reorder and the merge
+  into a fresh array; ``None`` turns either off (growth by reinsertion,
+  inserts by probing).  Both give the same map contents as the probing
+  paths.
 
 Latency is completion minus arrival; the loop reports p50/p99 and Mops/s.
 """
