@@ -82,9 +82,10 @@ def check_merge(bits, img, inf, m):
 
 
 @pytest.mark.parametrize("bits", [32, 64])
+# sp 23: a batch of ~4.3 M keys, partitioned in the 8,192-pair tiles of inputs of 2^22 pairs and more.
 @pytest.mark.parametrize("sp,old_lf,new_lf", [(4, 0.5, 0.3), (6, 0.6, 0.3), (8, 0.8, 0.15), (10, 0.5, 0.4),
                                               (11, 0.45, 0.45), (12, 0.5, 0.45), (13, 0.9, 0.05), (14, 0.3, 0.6),
-                                              (16, 0.7, 0.2), (18, 0.5, 0.45), (20, 0.5, 0.45)])
+                                              (16, 0.7, 0.2), (18, 0.5, 0.45), (20, 0.5, 0.45), (23, 0.4, 0.45)])
 def test_merge_image_exact(gpu, bits, sp, old_lf, new_lf):
     N = 1 << sp
     old_keys = keyset(int(old_lf * N), bits, seed=3 * sp + bits)
