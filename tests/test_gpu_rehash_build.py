@@ -47,9 +47,10 @@ def rehash_build(bits, hm, new_sp, seed=0):
 
 
 @pytest.mark.parametrize("bits", [32, 64])
+# 2^22 old slots: the first partition pass reads the old table in the 8,192-pair tiles.
 @pytest.mark.parametrize("old_sp,new_sp,lf,erase", [(3, 4, 0.75, 0.0), (10, 11, 0.9, 0.25), (12, 14, 0.7, 0.0),
                                                      (14, 15, 0.95, 0.3), (16, 17, 0.85, 0.1), (20, 21, 0.9, 0.25),
-                                                     (12, 13, 0.0, 0.0)])
+                                                     (12, 13, 0.0, 0.0), (22, 23, 0.85, 0.2)])
 def test_rehash_build_image_exact(gpu, bits, old_sp, new_sp, lf, erase):
     keys = keyset(int(lf * (1 << old_sp)), bits, seed=old_sp + bits)
     hm = old_map(bits, old_sp, keys, erase, seed=old_sp)
