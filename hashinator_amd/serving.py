@@ -378,9 +378,14 @@ class ServingEngine:
                 self.maps[r.map_id].queue.append(r)
             self._complete(now)
             dispatched = False
-            for map_id, m in enumerate(self.maps):
-                if m.busy or not m.queue or not self.free:
-                    continue
+            # maps in the order their oldest waiting request arrived: with the
+            # streams all busy, a freed stream goes to the longest wait
+            ready = sorted((m.queue[0].arrival, map_id) for map_id, m in enumerate(self.maps)
+                           if m.queue and not m.busy)
+            for _, map_id in ready:
+                m = self.maps[map_id]
+                if not self.free:
+                    break
                 reqs = self._take(m)
                 need = self._request_bytes(m, reqs)
                 if self.inflight and self.bytes_inflight + need > self.budget:
