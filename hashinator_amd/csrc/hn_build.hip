@@ -816,13 +816,17 @@ __global__ __launch_bounds__(kRegionBS) __attribute__((amdgpu_waves_per_eu(4))) 
          vv[k] = nv[k];
       }
       const uint32_t ms = region_sort<K, true>(S, lo, static_cast<uint32_t>(m), sp, R, L, kk, vv, uu, ff);
-      const RegionSum g = region_summary<K>(L, ms, R, ff);
+      if (r + 1 < r1) prefetch(r + 1);
+      probe = std::max(probe, region_place<K>(L, ms, c, r * R, sp, R, 0, kk, vv, uu, ff, b));
+      // The region's summary from the prefix max the placement left in v:
+      // v[ms - 1] = max over its keys of (home offset - sorted index).
+      RegionSum g;
+      g.b = static_cast<int64_t>(ms) - Ri;
+      g.a = ms > 0 ? std::max<int64_t>(0, static_cast<int64_t>(ms) - Ri + L.u2.v[ms - 1]) : 0;
       if (threadIdx.x == 0) {
          lb.word[r] = pack_sum(g, Ri);
          lb.spec[r] = c;
       }
-      if (r + 1 < r1) prefetch(r + 1);
-      probe = std::max(probe, region_place<K>(L, ms, c, r * R, sp, R, 0, kk, vv, uu, ff, b));
       c = std::max(g.a, c + g.b);
    }
    probe_to_header<K>(L, probe, h);
