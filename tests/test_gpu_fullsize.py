@@ -12,6 +12,7 @@ import torch
 from hashinator_amd import Hashmap, tools
 from hashinator_amd import _lib
 from hashinator_amd import workloads as W
+from tests.helpers import live_pairs
 
 pytestmark = pytest.mark.gpu
 
@@ -130,4 +131,30 @@ def test_fullsize_other_load_factors(gpu, lf, build):
     expect = torch.cat([dv, torch.full((n,), MISS, dtype=torch.int64, device="cuda")])[perm]
     check_retrieve(hm, q, expect, perm < n)
     del hm, q, expect, perm, dk, dv, da
+    torch.cuda.empty_cache()
+
+
+def test_fullsize_u32_probing_insert(gpu):
+    """2^26 u32 slots at load factor 0.9 through the probing insert, whose
+    32-bit claim stores key and value with one 64-bit CAS of the slot: every
+    key retrieves its value, and the table holds exactly the inserted pairs
+    (checked on the host)."""
+    sp = 26
+    n = int(0.9 * (1 << sp))
+    keys = W.unique_keys(n, 32, 11)
+    dk = torch.from_numpy(keys.view(np.int32)).cuda()
+    dv = dk * 3 + 1
+    hm = Hashmap(sp, 32, auto_build=0, auto_reorder=0)
+    hm.insert(dk, dv, targetLF=1.0)
+    i = hm.info()
+    assert (hm.getSizePower(), i.fill, i.tombstone_counter, i.err) == (sp, n, 0, 0)
+    out = torch.full_like(dk, 12345)
+    hit = torch.empty(n, dtype=torch.uint8, device="cuda")
+    hm.retrieve(dk, out, hit)
+    assert bool(hit.bool().all()) and torch.equal(out, dv)
+    rows = live_pairs(hm.host_buckets(), 32)
+    order = np.argsort(keys)
+    exp = np.stack([keys[order], dv.cpu().numpy().view(np.uint32)[order]], axis=1)
+    assert np.array_equal(rows, exp)
+    del hm, dk, dv, out, hit
     torch.cuda.empty_cache()
