@@ -1373,6 +1373,10 @@ struct MergeLds {
    uint16_t order[kMergeCap]; // ids by home offset (top bit: a dropped copy)
    int16_t v[kMergeCap];    // prefix-max terms by sorted index
    uint32_t cnt[kRegionMax + 1];
+   // With repeats: the group starts over all copies, kept for the re-ranking
+   // of the survivors (in LDS rather than per-element registers: with them in
+   // registers the u64 kernel spilled 28 VGPRs and merged 10.18 ms, not 8.99).
+   uint32_t cnt0[kRegionMax + 1];
    uint32_t occ[kMergeSpanWords];
    uint32_t wsum[kMergeBS / kWave];
    int32_t wmax[kMergeBS / kWave];
@@ -1464,7 +1468,7 @@ __global__ __launch_bounds__(kMergeBS) __attribute__((amdgpu_waves_per_eu(4))) v
          return e < kMergeEO ? ob[(o0 + threadIdx.x + e * kMergeBS) & mask].val
                              : S[n0 + threadIdx.x + (e - kMergeEO) * kMergeBS].val;
       };
-      uint32_t hs[kMergeE], g[kMergeE], jj[kMergeE];
+      uint32_t hs[kMergeE], jj[kMergeE];
       bool bad = false;
 #pragma unroll
       for (int e = 0; e < kMergeE; ++e) {
@@ -1500,7 +1504,6 @@ __global__ __launch_bounds__(kMergeBS) __attribute__((amdgpu_waves_per_eu(4))) v
       for (int e = 0; e < kMergeE; ++e) {
          if (hs[e] != kNoElem) {
             const uint32_t g0 = L.cnt[hs[e] >> 16], g1 = L.cnt[(hs[e] >> 16) + 1];
-            g[e] = g0 | (g1 << 16);
             bool d = false;
             jj[e] = g0 + merge_rank(L, elem_id(e), g0, g1, d, rep);
             drop |= d ? (1u << e) : 0u;
@@ -1534,7 +1537,8 @@ __global__ __launch_bounds__(kMergeBS) __attribute__((amdgpu_waves_per_eu(4))) v
          // mark the dropped copies, then count and rank the kept ones
 #pragma unroll
          for (int e = 0; e < kMergeE; ++e)
-            if (hs[e] != kNoElem && ((drop >> e) & 1u)) L.order[(g[e] & 0xFFFFu) + (hs[e] & 0xFFFFu)] |= 0x8000u;
+            if (hs[e] != kNoElem && ((drop >> e) & 1u)) L.order[L.cnt[hs[e] >> 16] + (hs[e] & 0xFFFFu)] |= 0x8000u;
+         for (uint32_t i = threadIdx.x; i <= R; i += kMergeBS) L.cnt0[i] = L.cnt[i];
          __syncthreads();
          for (uint32_t i = threadIdx.x; i <= R; i += kMergeBS) L.cnt[i] = 0;
          __syncthreads();
@@ -1546,7 +1550,8 @@ __global__ __launch_bounds__(kMergeBS) __attribute__((amdgpu_waves_per_eu(4))) v
 #pragma unroll
          for (int e = 0; e < kMergeE; ++e)
             if (hs[e] != kNoElem && !((drop >> e) & 1u))
-               jj[e] = L.cnt[hs[e] >> 16] + merge_rank_kept(L, elem_id(e), g[e] & 0xFFFFu, g[e] >> 16);
+               jj[e] = L.cnt[hs[e] >> 16] +
+                       merge_rank_kept(L, elem_id(e), L.cnt0[hs[e] >> 16], L.cnt0[(hs[e] >> 16) + 1]);
       }
 #pragma unroll
       for (int e = 0; e < kMergeE; ++e)
