@@ -284,21 +284,23 @@ __global__ __launch_bounds__(kBS) void k_compact(const In* __restrict__ in, uint
 
 // ------------------------------------------------------------ scan
 
-// A tile is kRows rows of kBS 16-byte vectors; thread t holds vector t of
-// every row, so each load and store of a wave covers 1 KB of consecutive
-// bytes.  Row sums are scanned across the wave, the (row, wave) totals in
+// A tile is ROWS rows of kBS 16-byte vectors (kRows when both pointers are
+// 16-byte aligned, kScanRowsScalar rows of element loads otherwise); thread
+// t holds vector t of every row, so each load and store of a wave covers
+// 1 KB of consecutive bytes.  Row sums are scanned across the wave, the (row, wave) totals in
 // LDS, in row-major order.  (With 256 consecutive bytes per thread instead,
 // a wave's 16-byte loads land 256 B apart and the 2^28-element u32 scan took
 // 0.60 ms; striped, 0.41 ms.)
 constexpr int kRows = kScanBytes / 16;
-template <typename T, bool VEC>
+constexpr int kScanRowsScalar = 4; // rows of a tile read element by element (unaligned pointers)
+template <typename T, bool VEC, int ROWS>
 __global__ __launch_bounds__(kBS) void k_scan(const T* in, T* out, uint64_t n, Lookback L, uint64_t tiles) {
    constexpr int V = 16 / sizeof(T);
-   constexpr int IPT = kRows * V;
+   constexpr int IPT = ROWS * V;
    constexpr uint64_t TILE = static_cast<uint64_t>(kBS) * IPT;
    __shared__ uint32_t s_tile;
-   __shared__ T s_rw[kRows * kWaves];
-   __shared__ T s_part[kRows * kWaves / kWave];
+   __shared__ T s_rw[ROWS * kWaves];
+   __shared__ T s_part[ROWS * kWaves / kWave];
    __shared__ uint64_t s_prefix;
    const int lane = __lane_id();
    const int wave = threadIdx.x / kWave;
@@ -311,19 +313,19 @@ __global__ __launch_bounds__(kBS) void k_scan(const T* in, T* out, uint64_t n, L
    if (full) {
       const uint4* src = reinterpret_cast<const uint4*>(in + t0);
 #pragma unroll
-      for (int c = 0; c < kRows; ++c) reinterpret_cast<uint4*>(x)[c] = src[c * kBS + threadIdx.x];
+      for (int c = 0; c < ROWS; ++c) reinterpret_cast<uint4*>(x)[c] = src[c * kBS + threadIdx.x];
    } else {
 #pragma unroll
-      for (int c = 0; c < kRows; ++c)
+      for (int c = 0; c < ROWS; ++c)
 #pragma unroll
          for (int k = 0; k < V; ++k) {
             const uint64_t i = t0 + (static_cast<uint64_t>(c) * kBS + threadIdx.x) * V + k;
             x[c * V + k] = i < n ? in[i] : T(0);
          }
    }
-   T rs[kRows], wi[kRows];
+   T rs[ROWS], wi[ROWS];
 #pragma unroll
-   for (int c = 0; c < kRows; ++c) {
+   for (int c = 0; c < ROWS; ++c) {
       T a = 0;
 #pragma unroll
       for (int k = 0; k < V; ++k) a += x[c * V + k];
@@ -332,8 +334,8 @@ __global__ __launch_bounds__(kBS) void k_scan(const T* in, T* out, uint64_t n, L
       if (lane == kWave - 1) s_rw[c * kWaves + wave] = wi[c];
    }
    __syncthreads();
-   // exclusive scan of the kRows * kWaves (row, wave) totals, row-major
-   constexpr int NE = kRows * kWaves;
+   // exclusive scan of the ROWS * kWaves (row, wave) totals, row-major
+   constexpr int NE = ROWS * kWaves;
    static_assert(NE % kWave == 0 && NE / kWave <= kWaves, "one entry per thread of the first waves");
    T e = 0, ei = 0;
    if (threadIdx.x < NE) {
@@ -357,7 +359,7 @@ __global__ __launch_bounds__(kBS) void k_scan(const T* in, T* out, uint64_t n, L
    __syncthreads();
    const T pre = static_cast<T>(s_prefix);
 #pragma unroll
-   for (int c = 0; c < kRows; ++c) {
+   for (int c = 0; c < ROWS; ++c) {
       T run = pre + s_rw[c * kWaves + wave] + (wi[c] - rs[c]);
 #pragma unroll
       for (int k = 0; k < V; ++k) {
@@ -369,10 +371,10 @@ __global__ __launch_bounds__(kBS) void k_scan(const T* in, T* out, uint64_t n, L
    if (full) {
       uint4* dst = reinterpret_cast<uint4*>(out + t0);
 #pragma unroll
-      for (int c = 0; c < kRows; ++c) dst[c * kBS + threadIdx.x] = reinterpret_cast<uint4*>(x)[c];
+      for (int c = 0; c < ROWS; ++c) dst[c * kBS + threadIdx.x] = reinterpret_cast<uint4*>(x)[c];
    } else {
 #pragma unroll
-      for (int c = 0; c < kRows; ++c)
+      for (int c = 0; c < ROWS; ++c)
 #pragma unroll
          for (int k = 0; k < V; ++k) {
             const uint64_t i = t0 + (static_cast<uint64_t>(c) * kBS + threadIdx.x) * V + k;
@@ -406,16 +408,22 @@ template <typename T>
 int run_scan(const T* in, T* out, uint64_t n, void* scr, size_t bytes, hipStream_t s) {
    if (n && (!in || !out)) return HN_EINVAL;
    if (n == 0) return 0;
-   constexpr uint64_t TILE = static_cast<uint64_t>(kBS) * (kScanBytes / sizeof(T));
+   // Pointers not 16-byte aligned take element loads and a tile of
+   // kScanRowsScalar rows (16 rows of element loads ran out of registers);
+   // either tile holds at least a compaction tile, so the scratch sized by
+   // hn_compact_scratch_bytes covers its look-back words.
+   const bool vec = (reinterpret_cast<uintptr_t>(in) % 16 == 0) && (reinterpret_cast<uintptr_t>(out) % 16 == 0);
+   const uint64_t TILE = static_cast<uint64_t>(kBS) * (16 / sizeof(T)) * (vec ? kRows : kScanRowsScalar);
    const uint64_t tiles = (n + TILE - 1) / TILE;
    if (!scr || bytes < lookback_bytes(tiles)) return HN_ESCRATCH;
    Lookback L = lookback_layout(scr, tiles);
    hipLaunchKernelGGL(k_lookback_init, dim3(stream_blocks(tiles, 256)), dim3(256), 0, s, L, tiles);
-   const bool vec = (reinterpret_cast<uintptr_t>(in) % 16 == 0) && (reinterpret_cast<uintptr_t>(out) % 16 == 0);
    if (vec)
-      hipLaunchKernelGGL((k_scan<T, true>), dim3(static_cast<uint32_t>(tiles)), dim3(kBS), 0, s, in, out, n, L, tiles);
+      hipLaunchKernelGGL((k_scan<T, true, kRows>), dim3(static_cast<uint32_t>(tiles)), dim3(kBS), 0, s, in, out, n, L,
+                         tiles);
    else
-      hipLaunchKernelGGL((k_scan<T, false>), dim3(static_cast<uint32_t>(tiles)), dim3(kBS), 0, s, in, out, n, L, tiles);
+      hipLaunchKernelGGL((k_scan<T, false, kScanRowsScalar>), dim3(static_cast<uint32_t>(tiles)), dim3(kBS), 0, s, in,
+                         out, n, L, tiles);
    return static_cast<int>(hipGetLastError());
 }
 

@@ -72,6 +72,23 @@ def test_exclusive_scan(gpu, bits, n):
     assert np.array_equal(to_numpy(t, bits), O.exclusive_scan(x))
 
 
+@pytest.mark.parametrize("bits", [32, 64])
+# pointers off 16-byte alignment take element loads and tiles of 4 rows
+# (16,384 u32 or 8,192 u64 elements): sizes around one and two such tiles
+@pytest.mark.parametrize("n", [1, 1000, 8191, 8192, 8193, 16_383, 16_384, 16_385, 32_769, 1_000_003])
+def test_exclusive_scan_unaligned(gpu, bits, n):
+    rng = np.random.default_rng(n * bits + 7)
+    x = rng.integers(0, np.iinfo(NDT[bits]).max, size=n, dtype=NDT[bits], endpoint=True)
+    src = torch.empty(n + 1, dtype=torch.int32 if bits == 32 else torch.int64, device="cuda")
+    src[1:].copy_(_t(x, bits))
+    dst = torch.zeros(n + 3, dtype=src.dtype, device="cuda")
+    inp, out = src[1:], dst[3:]  # 4 / 12 (u32) or 8 / 24 (u64) bytes off the allocations' alignment
+    assert inp.data_ptr() % 16 and out.data_ptr() % 16
+    tools.split_prefix_scan(inp, out)
+    assert np.array_equal(to_numpy(out, bits), O.exclusive_scan(x))
+    assert int(dst[:3].abs().sum()) == 0  # nothing written before the output
+
+
 def test_scan_flags_closed_form(gpu):
     n = 1 << 24
     flags = np.ones(n, dtype=np.uint32)
