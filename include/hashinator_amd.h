@@ -314,7 +314,7 @@ HN_API int hn_build_pairs_u64(const void* pairs, void* buckets, hn_info* info, s
  * 2^size_power, buffers that do not overlap, and `scratch` of
  * hn_merge_scratch_bytes_*(size_power, len) bytes.  The merge runs on a
  * table with HN_INFO_ORDERED and no TOMBSTONEs; when the table is not such,
- * a region holds more than 3,072 old slots or 1,536 batch pairs (skewed
+ * a region holds more than 2,560 old slots or 1,536 batch pairs (skewed
  * keys), the wrap past the table's end would move region 0's spill, or the
  * union would fill the table, new_buckets receives instead, on the device,
  * the old table with the batch inserted by probing: hn_insert_*'s result
