@@ -1,36 +1,35 @@
-// hn_compact.hip -- single-pass stream compaction and exclusive scan (gfx950).
+// hn_compact.hip -- the C-ABI's stream compaction and exclusive scan (gfx950).
 //
-// Both are one kernel over tiles with decoupled look-back: a tile takes a
-// ticket (so every tile it waits on has already started), publishes its own
-// aggregate, then walks back over its predecessors 64 at a time with one
-// wave until it meets an inclusive prefix, and publishes its own inclusive
-// prefix.  The input is read once and the output written once, against the
-// reference's count pass + Blelloch scan + scatter pass with a host sync
-// between kernels (split_tools.h:920-989, 363-418).
-//
-// Compaction is stable: tile-local ranks come from wave ballots over a
-// striped layout (element j*256 + t of the tile is item j of thread t), so
-// ranks follow input order.
+// Compaction is the single-pass kernel of <splitvector/split_compact.h>,
+// instantiated here for the built-in predicates (hn_pred), flag bytes and
+// bit masks.  The scan is one kernel over tiles with decoupled look-back: a
+// tile takes a ticket (so every tile it waits on has already started),
+// publishes its own aggregate, then walks back over its predecessors 64 at a
+// time with one wave until it meets an inclusive prefix, and publishes its
+// own inclusive prefix.  Each reads its input once and writes its output
+// once, against the reference's count pass + Blelloch scan + scatter pass
+// with a host sync between kernels (split_tools.h:920-989, 363-418).
 #include <hip/hip_runtime.h>
 
 #include <atomic>
 #include <cstdint>
 #include <mutex>
 
+#include <splitvector/split_compact.h>
+
 #include "hn_device.hpp"
 #include "hn_host.hpp"
 
 namespace hn {
 
-// Tiles are large because every tile pays a serial cost: its ticket (one
-// atomic on one word) and its step in the look-back chain.  At 2^28 elements,
-// 256-thread tiles ran copy_if in 2.0 ms and the u32 scan in 1.0 ms; 1024
-// threads with 8 pairs each (copy_if) and 256 bytes each (scan) take 1.33 and
-// 0.41 ms.  Fewer pairs per thread (4) or more (16) were slower for copy_if.
+namespace sc = split::tools::compact;
+
+// Scan tiles are large because every tile pays a serial cost: its ticket (one
+// atomic on one word) and its step in the look-back chain.  At 2^28 u32,
+// 256-thread tiles ran the scan in 1.0 ms; 1024 threads with 256 bytes each
+// take 0.41 ms.
 constexpr int kBS = 1024;        // threads per tile
 constexpr int kWaves = kBS / kWave;
-constexpr int kCompactIPT = 8;   // elements per thread in a compaction tile
-constexpr int kCompactTile = kBS * kCompactIPT;
 constexpr int kScanBytes = 256;  // consecutive bytes per thread in a scan tile
 constexpr uint64_t kFlagAgg = 1, kFlagInc = 2;
 
@@ -66,8 +65,6 @@ __global__ __launch_bounds__(256) void k_lookback_init(Lookback L, uint64_t tile
    if (blockIdx.x == 0 && threadIdx.x == 0) *L.ticket = 0;
 }
 
-__global__ void k_store_u64(uint64_t* p, uint64_t v) { *p = v; }
-
 __device__ __forceinline__ void lb_publish(const Lookback& L, uint64_t tile, uint64_t flag, uint64_t v) {
    __hip_atomic_store(&L.hi[tile], (flag << 62) | (v >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
    __hip_atomic_store(&L.lo[tile], (flag << 62) | (v & 0xFFFFFFFFull), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -80,43 +77,6 @@ __device__ __forceinline__ uint64_t lb_read(const Lookback& L, uint64_t p, uint6
    if ((a >> 62) != (b >> 62)) return 0;
    *v = ((b & 0xFFFFFFFFull) << 32) | (a & 0xFFFFFFFFull);
    return a >> 62;
-}
-
-// One-word form for values below 2^62 (element counts, the compaction's):
-// state in the top two bits, the value below, so a predecessor costs one
-// device-scope load, not two (copy_if of 2^28 pairs: 1.32 -> 1.23 ms).
-__device__ __forceinline__ void lb1_publish(const Lookback& L, uint64_t tile, uint64_t flag, uint64_t v) {
-   __hip_atomic_store(&L.lo[tile], (flag << 62) | v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ uint64_t lookback_prefix1(const Lookback& L, uint32_t tile, uint64_t mine) {
-   const int lane = __lane_id();
-   if (tile == 0) {
-      if (lane == 0) lb1_publish(L, 0, kFlagInc, mine);
-      return 0;
-   }
-   if (lane == 0) lb1_publish(L, tile, kFlagAgg, mine);
-   uint64_t prefix = 0;
-   int64_t hi = static_cast<int64_t>(tile) - 1;
-   for (;;) {
-      const int64_t p = hi - lane;
-      uint64_t w = kFlagInc << 62;
-      if (p >= 0) w = __hip_atomic_load(&L.lo[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      const uint64_t f = w >> 62;
-      const uint64_t not_ready = __ballot(f == 0);
-      const uint64_t inc_mask = __ballot(f == kFlagInc);
-      const int first_inc = inc_mask ? first_bit(inc_mask) : 64;
-      const uint64_t upto = first_inc >= 63 ? ~0ull : ((2ull << first_inc) - 1ull);
-      if (not_ready & upto) {
-         __builtin_amdgcn_s_sleep(1);
-         continue;
-      }
-      const uint64_t take = (lane <= first_inc) ? (w & ((1ull << 62) - 1)) : 0;
-      prefix += wave_sum(take);
-      if (first_inc < 64) break;
-      hi -= 64;
-   }
-   if (lane == 0) lb1_publish(L, tile, kFlagInc, prefix + mine);
-   return prefix;
 }
 
 // Exclusive prefix of tile `tile` whose own aggregate is `mine`; called by
@@ -213,74 +173,6 @@ template <typename K>
 struct KeyOf {
    __device__ K operator()(const Slot<K>& s) const { return s.key; }
 };
-
-template <typename In, typename Out, class Sel, class Proj>
-__global__ __launch_bounds__(kBS) void k_compact(const In* __restrict__ in, uint64_t n, Out* __restrict__ out,
-                                                 uint64_t* d_count, Lookback L, uint64_t tiles, Sel sel, Proj proj) {
-   __shared__ uint32_t s_tile;
-   __shared__ uint32_t s_cnt[kCompactIPT][kWaves];
-   __shared__ uint64_t s_prefix;
-   const int lane = __lane_id();
-   const int wave = threadIdx.x / kWave;
-   if (threadIdx.x == 0) s_tile = __hip_atomic_fetch_add(L.ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-   __syncthreads();
-   const uint32_t tile = s_tile;
-   const uint64_t base = static_cast<uint64_t>(tile) * kCompactTile;
-
-   In v[kCompactIPT];
-   uint64_t bal[kCompactIPT];
-#pragma unroll
-   for (int j = 0; j < kCompactIPT; ++j) {
-      const uint64_t i = base + static_cast<uint64_t>(j) * kBS + threadIdx.x;
-      bool keep = false;
-      if (i < n) {
-         v[j] = in[i];
-         keep = sel(i, v[j]);
-      }
-      bal[j] = __ballot(keep);
-      if (lane == 0) s_cnt[j][wave] = static_cast<uint32_t>(__popcll(bal[j]));
-   }
-   __syncthreads();
-   if (wave == 0) {
-      // Exclusive prefix of the counts, j-major and wave-minor (= input
-      // order), by one wave: each lane scans PER consecutive counts.
-      constexpr int E = kCompactIPT * kWaves;
-      constexpr int PER = (E + kWave - 1) / kWave;
-      uint32_t* flat = &s_cnt[0][0];
-      uint32_t c[PER];
-      uint32_t sum = 0;
-#pragma unroll
-      for (int i = 0; i < PER; ++i) {
-         const int idx = lane * PER + i;
-         c[i] = idx < E ? flat[idx] : 0u;
-         sum += c[i];
-      }
-      const uint32_t incl = wave_incl_scan<uint32_t>(sum);
-      uint32_t run = incl - sum;
-#pragma unroll
-      for (int i = 0; i < PER; ++i) {
-         const int idx = lane * PER + i;
-         if (idx < E) flat[idx] = run;
-         run += c[i];
-      }
-      const uint64_t total = static_cast<uint32_t>(__shfl(static_cast<int>(incl), kWave - 1, kWave));
-      const uint64_t pre = lookback_prefix1(L, tile, total);
-      if (lane == 0) {
-         s_prefix = pre;
-         if (tile == tiles - 1 && d_count) *d_count = pre + total;
-      }
-   }
-   __syncthreads();
-   const uint64_t pre = s_prefix;
-   const uint64_t below = (1ull << lane) - 1ull;
-#pragma unroll
-   for (int j = 0; j < kCompactIPT; ++j) {
-      if ((bal[j] >> lane) & 1ull) {
-         const uint64_t r = pre + s_cnt[j][wave] + static_cast<uint64_t>(__popcll(bal[j] & below));
-         out[r] = proj(v[j]);
-      }
-   }
-}
 
 // ------------------------------------------------------------ scan
 
@@ -385,23 +277,14 @@ __global__ __launch_bounds__(kBS) void k_scan(const T* in, T* out, uint64_t n, L
 
 // ------------------------------------------------------------ host side
 
-constexpr uint64_t compact_tiles(uint64_t n) { return n ? (n + kCompactTile - 1) / kCompactTile : 0; }
+constexpr uint64_t compact_tiles(uint64_t n) { return sc::tiles_for(n); }
 
 template <typename In, typename Out, class Sel, class Proj>
 int run_compact(const In* in, uint64_t n, Out* out, uint64_t* d_count, void* scr, size_t bytes, hipStream_t s,
                 Sel sel, Proj proj) {
    if ((n && (!in || !out)) || !d_count) return HN_EINVAL;
-   if (n == 0) {
-      hipLaunchKernelGGL(k_store_u64, dim3(1), dim3(1), 0, s, d_count, 0ull); // nothing selected
-      return static_cast<int>(hipGetLastError());
-   }
-   const uint64_t tiles = compact_tiles(n);
-   if (!scr || bytes < lookback_bytes(tiles)) return HN_ESCRATCH;
-   Lookback L = lookback_layout(scr, tiles);
-   hipLaunchKernelGGL(k_lookback_init, dim3(stream_blocks(tiles, 256)), dim3(256), 0, s, L, tiles);
-   hipLaunchKernelGGL((k_compact<In, Out, Sel, Proj>), dim3(static_cast<uint32_t>(tiles)), dim3(kBS), 0, s, in, n,
-                      out, d_count, L, tiles, sel, proj);
-   return static_cast<int>(hipGetLastError());
+   if (!scr || bytes < sc::scratch_bytes(n)) return HN_ESCRATCH;
+   return static_cast<int>(sc::launch(in, n, out, d_count, scr, s, sel, proj));
 }
 
 template <typename T>

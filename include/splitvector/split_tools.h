@@ -5,18 +5,20 @@
  * splitStackArena (491-535), estimateMemoryForCompaction (1064-1089).
  *
  * A user Rule (a __host__ __device__ functor taking T&) cannot cross the
- * C-ABI, so it is evaluated here, once per element, by a small kernel
- * instantiated in the caller's translation unit that packs the answers one
- * bit per element; the stable single-pass compaction of libhashinator_amd
- * then moves the selected elements.  (The reference evaluates the rule twice,
- * in its count and its scatter pass; a rule with side effects on the element
- * sees them once here.)
+ * C-ABI, so copy_if and copy_keys_if instantiate the library's single-pass
+ * compaction kernel (split_compact.h) here, in the caller's translation unit,
+ * with the rule as its selector: the input is read once and each element's
+ * rule evaluated once.  (The reference evaluates the rule twice, in its
+ * count and its scatter pass.)  The rule receives a register copy of the
+ * element; the selected copy is written as the rule left it and the input is
+ * not written.
  */
 #pragma once
 #include <hashinator_amd.h>
 
 #include <cassert>
 
+#include "split_compact.h"
 #include "splitvec.h"
 
 namespace split {
@@ -55,42 +57,24 @@ public:
    size_t free_space() const { return total_bytes - bytes_used; }
 };
 
-/* Device scratch needed to compact n elements with a user rule: the rule's
- * bit mask, the device-side count, and the look-back tile state. */
+/* Device scratch needed to compact n elements with a user rule: the
+ * device-side count and the look-back tile state. */
 [[nodiscard]] inline size_t estimateMemoryForCompaction(const size_t inputSize) noexcept {
-   return ((inputSize + 31) / 32) * 4 + 256 + 8 + 256 + hn_compact_scratch_bytes(inputSize) + 256;
+   return 256 + compact::scratch_bytes(inputSize) + 256;
 }
 template <typename T, class A>
 [[nodiscard]] size_t estimateMemoryForCompaction(const SplitVector<T, A>& input) noexcept {
    return estimateMemoryForCompaction(input.size());
 }
 
-/* One rule evaluation per element, packed 32 per word (LSB first). */
-template <typename T, typename Rule>
-__global__ void split_rule_bits_kernel(T* input, size_t n, Rule rule, uint32_t* bits) {
-   const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-   const bool keep = i < n ? rule(input[i]) : false;
-   const unsigned long long m = __ballot(keep);
-   const int lane = __lane_id();
-   const size_t w0 = (i - lane) / 32; // first word of this wave
-   if (lane == 0 && (i - lane) < n) bits[w0] = static_cast<uint32_t>(m);
-   if (lane == 32 && i < n + 32 && (i - lane) + 32 < n) bits[w0 + 1] = static_cast<uint32_t>(m >> 32);
-}
-
 namespace detail {
-template <typename T, typename Rule>
-uint64_t* stage_rule(T* input, size_t n, Rule rule, splitStackArena& pool, uint32_t** bits_out, void** scratch,
-                     size_t* scratch_bytes, hipStream_t s) {
-   uint32_t* bits = static_cast<uint32_t*>(pool.allocate(((n + 63) / 64) * 8));
+/* Compaction of in[0, size) with a rule into out (element or key), counted
+ * into device memory from the arena; returns the device count's address. */
+template <typename T, typename Out, typename Rule, class Proj>
+uint64_t* run_rule(T* input, Out* output, size_t size, Rule rule, Proj proj, splitStackArena& pool, hipStream_t s) {
    uint64_t* d_count = static_cast<uint64_t*>(pool.allocate(8));
-   *scratch_bytes = hn_compact_scratch_bytes(n);
-   *scratch = pool.allocate(*scratch_bytes);
-   if (n) {
-      const unsigned blocks = static_cast<unsigned>((n + 255) / 256);
-      hipLaunchKernelGGL((split_rule_bits_kernel<T, Rule>), dim3(blocks), dim3(256), 0, s, input, n, rule, bits);
-      SPLIT_CHECK_ERR(hipGetLastError());
-   }
-   *bits_out = bits;
+   void* scratch = pool.allocate(compact::scratch_bytes(size));
+   SPLIT_CHECK_ERR(compact::launch(input, size, output, d_count, scratch, s, compact::RuleSel<Rule>{rule}, proj));
    return d_count;
 }
 inline size_t read_count(uint64_t* d_count, hipStream_t s) {
@@ -112,30 +96,13 @@ inline void check_abi(int rc) {
  * written to `output` (copy_if_raw, split_tools.h:956-989). */
 template <typename T, typename Rule>
 size_t copy_if_raw(T* input, T* output, size_t size, Rule rule, splitStackArena& mPool, hipStream_t s = 0) {
-   static_assert(sizeof(T) == 4 || sizeof(T) == 8 || sizeof(T) == 16, "element size must be 4, 8 or 16 bytes");
-   uint32_t* bits;
-   void* scratch;
-   size_t sb;
-   uint64_t* d_count = detail::stage_rule(input, size, rule, mPool, &bits, &scratch, &sb, s);
-   detail::check_abi(hn_copy_if_bits(input, bits, size, sizeof(T), output, d_count, scratch, sb, s));
-   return detail::read_count(d_count, s);
+   return detail::read_count(detail::run_rule(input, output, size, rule, compact::Whole{}, mPool, s), s);
 }
 
+/* Keys of the selected pairs (copy_keys_if_raw, split_tools.h:1022-1056). */
 template <typename T, typename U, typename Rule>
 size_t copy_keys_if_raw(T* input, U* output, size_t size, Rule rule, splitStackArena& mPool, hipStream_t s = 0) {
-   static_assert(sizeof(U) == 4 || sizeof(U) == 8, "keys must be 4 or 8 bytes");
-   static_assert(sizeof(T) == 2 * sizeof(U), "input must be hash_pair<U,U>");
-   uint32_t* bits;
-   void* scratch;
-   size_t sb;
-   uint64_t* d_count = detail::stage_rule(input, size, rule, mPool, &bits, &scratch, &sb, s);
-   if (sizeof(U) == 4)
-      detail::check_abi(hn_copy_keys_if_bits_u32(input, bits, size, reinterpret_cast<uint32_t*>(output), d_count,
-                                                 scratch, sb, s));
-   else
-      detail::check_abi(hn_copy_keys_if_bits_u64(input, bits, size, reinterpret_cast<uint64_t*>(output), d_count,
-                                                 scratch, sb, s));
-   return detail::read_count(d_count, s);
+   return detail::read_count(detail::run_rule(input, output, size, rule, compact::FirstOf<U>{}, mPool, s), s);
 }
 
 /* copy_if (split_tools.h:1128-1144): output must already hold room for the
