@@ -471,6 +471,77 @@ bool compaction_and_scan() {
    return true;
 }
 
+/* The raw-pointer copy_if over a caller's stack, driven through the
+ * split_gpu* runtime names the reference's tests use
+ * (unit_tests/stream_compaction/race.cu, preallocated.cu): ragged sizes
+ * around the 8,192-element tile (0 included), a 12-byte element, and a rule
+ * that writes its element -- the output holds the rule's copy, the input is
+ * left as it was. */
+struct Triple {
+   uint32_t a, b, c;
+};
+struct TripleEven {
+   __host__ __device__ bool operator()(Triple& t) const { return (t.b % 2) == 0; }
+};
+struct BumpOdd {
+   __host__ __device__ bool operator()(hash_pair<uint64_t, uint64_t>& e) const {
+      const bool odd = (e.second & 1) != 0;
+      e.second += 10;
+      return odd;
+   }
+};
+bool raw_compaction_split_gpu_api() {
+   split_gpuStream_t s;
+   SPLIT_CHECK_ERR(split_gpuStreamCreate(&s));
+   std::mt19937 rng(17);
+   for (size_t n : {size_t(0), size_t(1), size_t(8191), size_t(8192), size_t(8193), size_t(3 * 8192 + 5), size_t(200001)}) {
+      std::vector<Triple> h(n);
+      for (size_t i = 0; i < n; ++i) h[i] = Triple{static_cast<uint32_t>(i), static_cast<uint32_t>(rng()), 7u};
+      Triple *d_in = nullptr, *d_out = nullptr;
+      void* stack = nullptr;
+      const size_t need = split::tools::estimateMemoryForCompaction(n);
+      SPLIT_CHECK_ERR(split_gpuMalloc((void**)&d_in, (n ? n : 1) * sizeof(Triple)));
+      SPLIT_CHECK_ERR(split_gpuMalloc((void**)&d_out, (n ? n : 1) * sizeof(Triple)));
+      SPLIT_CHECK_ERR(split_gpuMalloc(&stack, need));
+      SPLIT_CHECK_ERR(split_gpuMemsetAsync(d_out, 0xff, (n ? n : 1) * sizeof(Triple), s));
+      SPLIT_CHECK_ERR(split_gpuMemcpyAsync(d_in, h.data(), n * sizeof(Triple), split_gpuMemcpyHostToDevice, s));
+      const size_t m = split::tools::copy_if(d_in, d_out, n, TripleEven(), stack, need, s);
+      std::vector<Triple> got(m);
+      SPLIT_CHECK_ERR(split_gpuMemcpyAsync(got.data(), d_out, m * sizeof(Triple), split_gpuMemcpyDeviceToHost, s));
+      SPLIT_CHECK_ERR(split_gpuStreamSynchronize(s));
+      size_t j = 0;
+      for (size_t i = 0; i < n; ++i) {
+         if (h[i].b % 2) continue;
+         CHECK(j < m && got[j].a == h[i].a && got[j].b == h[i].b && got[j].c == 7u);
+         ++j;
+      }
+      CHECK(j == m);
+      SPLIT_CHECK_ERR(split_gpuFree(d_in));
+      SPLIT_CHECK_ERR(split_gpuFree(d_out));
+      SPLIT_CHECK_ERR(split_gpuFree(stack));
+   }
+   // a rule that writes its element
+   const size_t n = 50000;
+   using P = hash_pair<uint64_t, uint64_t>;
+   std::vector<P> h(n);
+   for (size_t i = 0; i < n; ++i) h[i] = P(i, (static_cast<uint64_t>(rng()) << 20) | (i % 3 == 0 ? 1u : 0u));
+   split::SplitVector<P, split::split_device_allocator<P>> in(h), out(n);
+   split::tools::copy_if(in, out, BumpOdd(), s);
+   const auto got = out.to_host();
+   const auto after = in.to_host();
+   size_t j = 0;
+   for (size_t i = 0; i < n; ++i) {
+      CHECK(after[i].first == h[i].first && after[i].second == h[i].second);
+      if (!(h[i].second & 1)) continue;
+      CHECK(j < got.size() && got[j].first == h[i].first && got[j].second == h[i].second + 10);
+      ++j;
+   }
+   CHECK(j == got.size());
+   SPLIT_CHECK_ERR(split_gpuStreamDestroy(s));
+   SPLIT_CHECK_ERR(split_gpuDeviceSynchronize());
+   return true;
+}
+
 bool splitvector_host_semantics() {
    split::SplitVector<int> v{1, 2, 3};
    v.push_back(4);
@@ -724,6 +795,7 @@ int main() {
        {"thread_api<u32>", [] { return thread_api<uint32_t>(); }},
        {"thread_api<u64>", [] { return thread_api<uint64_t>(); }},
        {"compaction_and_scan", compaction_and_scan},
+       {"raw_compaction_split_gpu_api", raw_compaction_split_gpu_api},
        {"splitvector_host_semantics", splitvector_host_semantics},
        {"splitvector_device_push_back<unified>", splitvector_device_push_back<split::split_unified_allocator<uint64_t>>},
        {"splitvector_device_push_back<device>", splitvector_device_push_back<split::split_device_allocator<uint64_t>>},
