@@ -17,6 +17,11 @@
  * the caller's translation unit with the caller's rule, so a user rule costs
  * no extra pass over the input.
  *
+ * A selector receives a register copy of each element; through a writable
+ * input pointer, an element the selector changed is stored back (the rule of
+ * the reference's clean_tombstones, hashinator.h:1285-1297, turns tombstones
+ * into empty buckets this way).
+ *
  * Scratch: scratch_bytes(n) bytes of device memory, 256-byte aligned, whose
  * contents need not be initialised (launch() clears them on the stream).
  * Calls that share scratch must not overlap.
@@ -120,6 +125,20 @@ __device__ inline uint64_t tile_prefix(const TileState& st, uint32_t tile, uint6
    return prefix;
 }
 
+// Whether two objects hold the same bytes (4-byte words where the size allows).
+template <typename T>
+__device__ __forceinline__ bool same_bytes(const T& a, const T& b) {
+   constexpr int W = sizeof(T) % 4 == 0 ? 4 : 1;
+   using Word = typename std::conditional<W == 4, uint32_t, unsigned char>::type;
+   Word wa[sizeof(T) / W], wb[sizeof(T) / W];
+   __builtin_memcpy(wa, &a, sizeof(T));
+   __builtin_memcpy(wb, &b, sizeof(T));
+   bool same = true;
+#pragma unroll
+   for (int k = 0; k < static_cast<int>(sizeof(T) / W); ++k) same = same && wa[k] == wb[k];
+   return same;
+}
+
 // Clears the look-back words and the ticket; with no tiles, stores the count 0.
 template <typename Count>
 __global__ __launch_bounds__(256) void k_prepare(TileState st, uint64_t tiles, Count* count) {
@@ -134,7 +153,11 @@ __global__ __launch_bounds__(256) void k_prepare(TileState st, uint64_t tiles, C
 
 // out[rank] = proj(x) for every x = in[i] with sel(i, x), in input order;
 // *count = the number written.  sel receives a register copy of the element
-// (the input is never written) and proj the copy as sel left it.
+// and proj the copy as sel left it.  With a writable input (In not const), a
+// copy that sel changed is stored back to in[i], so a rule that edits its
+// element leaves the edit in the input, as a rule run on the element in
+// place does; for a rule that edits nothing the check folds away at compile
+// time.
 template <typename In, typename Out, class Sel, class Proj>
 __global__ __launch_bounds__(kThreads) void k_compact(In* __restrict__ in, uint64_t n, Out* __restrict__ out,
                                                       uint64_t* count, TileState st, uint64_t tiles, Sel sel,
@@ -158,7 +181,13 @@ __global__ __launch_bounds__(kThreads) void k_compact(In* __restrict__ in, uint6
       bool keep = false;
       if (i < n) {
          v[j] = in[i];
-         keep = sel(i, v[j]);
+         if constexpr (std::is_const<In>::value) {
+            keep = sel(i, v[j]);
+         } else {
+            const T before = v[j];
+            keep = sel(i, v[j]);
+            if (!same_bytes(before, v[j])) in[i] = v[j];
+         }
       }
       bal[j] = __ballot(keep);
       if (lane == 0) s_cnt[j][wave] = static_cast<uint32_t>(__popcll(bal[j]));

@@ -9,9 +9,10 @@
  * compaction kernel (split_compact.h) here, in the caller's translation unit,
  * with the rule as its selector: the input is read once and each element's
  * rule evaluated once.  (The reference evaluates the rule twice, in its
- * count and its scatter pass.)  The rule receives a register copy of the
- * element; the selected copy is written as the rule left it and the input is
- * not written.
+ * count and its scatter pass.)  The rule receives a copy of the element; a
+ * change it makes is stored back to the input, as the reference's rule,
+ * which runs on the element in place, leaves it, and a selected element is
+ * written to the output as the rule left it.
  */
 #pragma once
 #include <hashinator_amd.h>
