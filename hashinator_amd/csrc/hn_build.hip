@@ -56,7 +56,8 @@ namespace hn {
 
 constexpr int kRegionLog = 11;   // 2048 slots per region
 constexpr int kPartBS = 256;     // count workgroup
-constexpr int kPartRounds = 16;  // pairs per thread of a scatter tile
+constexpr int kPartBSS = 256, kPartRoundsS = 16;  // 4,096-pair scatter tiles: threads, pairs per thread
+constexpr int kPartBSL = 1024, kPartRoundsL = 8;  // 8,192-pair scatter tiles
 // Scatter tiles: 8,192 pairs (512 threads, 16-pair runs per 9-bit digit,
 // one block per CU) for inputs of at least kPartLargeMin pairs, 4,096 (256
 // threads, two blocks per CU) below, where the large tile would leave CUs
@@ -271,24 +272,25 @@ __global__ __launch_bounds__(kPartBS) void k_part_count(PartIn<K> in, uint64_t n
 
 // Stable multisplit.  A block walks its `nsub` tiles in order, keeping the
 // running output offset of every digit in LDS.  In a tile, wave w ranks the
-// pairs [1024w, 1024w + 1024) in sixteen rounds of 64: lanes with equal digits are
-// found with one ballot per digit bit, and the wave's own LDS counters carry
-// the earlier rounds (no block barrier per round).  Per-wave counts then give
-// each pair's rank among equal digits in tile order; the tile is staged in
-// LDS in digit order and leaves in runs.
-template <typename K, int SRC, bool FILTER, int BS>
+// pairs [64*ROUNDS*w, 64*ROUNDS*(w+1)) in ROUNDS rounds of 64: lanes with equal
+// digits are found with one ballot per digit bit, and the wave's own LDS
+// counters carry the earlier rounds (no block barrier per round).  Per-wave
+// counts then give each pair's rank among equal digits in tile order; the
+// tile is staged in LDS in digit order and leaves in runs.  The per-wave
+// counters are 16-bit (a count or an offset within a tile is below 2^16),
+// which leaves room in LDS for 16 waves beside an 8,192-pair tile.
+template <typename K, int SRC, bool FILTER, int BS, int ROUNDS>
 __global__ __launch_bounds__(BS) void k_part_scatter(PartIn<K> in, uint64_t n_in, const uint64_t* d_n,
                                                           Slot<K>* out, const uint32_t* offs, uint32_t nblocks,
                                                           uint32_t nsub, int sp, int rlog, int dshift, int dbits,
                                                           const BuildHeader* h) {
    constexpr int NW = BS / kWave;
-   constexpr int ROUNDS = kPartRounds;
-   constexpr int TILE = BS * kPartRounds;
+   constexpr int TILE = BS * ROUNDS;
    constexpr uint32_t DMAX = 1u << kMaxDigitBits;
-   constexpr int DPT = DMAX / BS; // digits per thread in the digit loops
-   static_assert(DPT * BS == DMAX && ROUNDS * BS == TILE, "digit and tile split");
+   constexpr int DPT = DMAX >= BS ? DMAX / BS : 1; // digits per thread in the digit loops
+   static_assert(DPT * BS >= DMAX && TILE < 65536, "digit split; 16-bit counters");
    __shared__ Slot<K> tile_s[TILE];
-   __shared__ uint32_t wcnt_s[NW][DMAX];
+   __shared__ uint16_t wcnt_s[NW][DMAX];
    __shared__ uint32_t lstart_s[DMAX];
    __shared__ uint32_t goff_s[DMAX];
    __shared__ uint32_t wsum_s[NW];
@@ -333,7 +335,7 @@ __global__ __launch_bounds__(BS) void k_part_scatter(PartIn<K> in, uint64_t n_in
             dg[r] = (region_of(el[r].key, sp, rlog) >> dshift) & dmask;
          }
       }
-      uint32_t* wc = wcnt_s[wave];
+      uint16_t* wc = wcnt_s[wave];
 #pragma unroll
       for (int r = 0; r < ROUNDS; ++r) {
          const bool valid = ok[r];
@@ -346,7 +348,7 @@ __global__ __launch_bounds__(BS) void k_part_scatter(PartIn<K> in, uint64_t n_in
          // All lanes read the counter before its one writer (the group's
          // first lane) updates it; a wave's LDS operations run in order.
          if (valid) rk[r] = wc[dg[r]] + static_cast<uint32_t>(__popcll(m & lt));
-         if (valid && first_bit(m) == lane) wc[dg[r]] += static_cast<uint32_t>(__popcll(m));
+         if (valid && first_bit(m) == lane) wc[dg[r]] += static_cast<uint16_t>(__popcll(m));
       }
       __syncthreads();
       // Per digit: waves' counts -> exclusive offsets over waves; tile totals -> exclusive scan.
@@ -360,7 +362,7 @@ __global__ __launch_bounds__(BS) void k_part_scatter(PartIn<K> in, uint64_t n_in
 #pragma unroll
             for (int w = 0; w < NW; ++w) {
                const uint32_t c = wcnt_s[w][d];
-               wcnt_s[w][d] = tot[i];
+               wcnt_s[w][d] = static_cast<uint16_t>(tot[i]);
                tot[i] += c;
             }
          }
@@ -1050,14 +1052,13 @@ int launch_part(const BuildPlan& P, int pass, PartIn<K> in, uint64_t n_in, const
                       P.nblocks, P.tile, P.sub, P.sp, P.rlog, dshift, dbits, h);
    int rc = hn_exclusive_scan_u32(mat, mat, D * P.nblocks, scan_scratch, scan_bytes, reinterpret_cast<hn_stream_t>(s));
    if (rc) return rc;
+   static_assert(kPartBSL * kPartRoundsL == kPartTileL && kPartBSS * kPartRoundsS == kPartTileS, "scatter tiles");
    if (P.tile == kPartTileL)
-      hipLaunchKernelGGL((k_part_scatter<K, SRC, FILTER, kPartTileL / kPartRounds>), dim3(P.nblocks),
-                         dim3(kPartTileL / kPartRounds), 0, s, in, n_in, d_n, out, mat, P.nblocks, P.sub, P.sp, P.rlog,
-                         dshift, dbits, h);
+      hipLaunchKernelGGL((k_part_scatter<K, SRC, FILTER, kPartBSL, kPartRoundsL>), dim3(P.nblocks), dim3(kPartBSL), 0,
+                         s, in, n_in, d_n, out, mat, P.nblocks, P.sub, P.sp, P.rlog, dshift, dbits, h);
    else
-      hipLaunchKernelGGL((k_part_scatter<K, SRC, FILTER, kPartTileS / kPartRounds>), dim3(P.nblocks),
-                         dim3(kPartTileS / kPartRounds), 0, s, in, n_in, d_n, out, mat, P.nblocks, P.sub, P.sp, P.rlog,
-                         dshift, dbits, h);
+      hipLaunchKernelGGL((k_part_scatter<K, SRC, FILTER, kPartBSS, kPartRoundsS>), dim3(P.nblocks), dim3(kPartBSS), 0,
+                         s, in, n_in, d_n, out, mat, P.nblocks, P.sub, P.sp, P.rlog, dshift, dbits, h);
    return static_cast<int>(hipGetLastError());
 }
 
