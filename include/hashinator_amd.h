@@ -83,9 +83,10 @@ typedef struct hn_tune {
    int32_t reserved;
 } hn_tune;
 
-/* Built-in compaction predicates (the C-ABI cannot carry a functor; user
- * rules are evaluated into a flags array by the header shim and passed to
- * hn_copy_if_flags). */
+/* Built-in compaction predicates (the C-ABI cannot carry a functor: a rule
+ * evaluated elsewhere comes in as flags or bits, hn_copy_if_flags /
+ * hn_copy_if_bits, and C++ callers compile the kernel with their rule from
+ * <splitvector/split_compact.h>). */
 typedef struct hn_pred {
    int32_t kind;  /* HN_PRED_* */
    int32_t field; /* 0: key (first), 1: value (second) */
@@ -388,14 +389,16 @@ HN_API int hn_copy_if_pairs_noalloc_u64(const void* in, size_t n, const hn_pred*
                                         hn_stream_t s);
 
 /* copy_if with the rule already evaluated into flags (one byte per element,
- * nonzero = keep): the path the header shim takes for user functors.
- * elem_bytes must be 4, 8 or 16. */
+ * nonzero = keep), for callers that evaluate their rule outside this library
+ * (an FFI caller, or a rule computed by other kernels).  C++ callers with a
+ * rule functor instantiate the same kernel with the rule in their own
+ * translation unit instead (<splitvector/split_compact.h>, used by
+ * split::tools::copy_if).  elem_bytes must be 4, 8 or 16. */
 HN_API int hn_copy_if_flags(const void* in, const uint8_t* flags, size_t n, size_t elem_bytes, void* out,
                      uint64_t* d_count, void* scratch, size_t scratch_bytes, hn_stream_t s);
 
 /* copy_keys_if with the rule already evaluated into flags: .first of each
- * flagged hash_pair, in input order (the shim's path for user functors in
- * copy_keys_if / Hashmap::extractKeysByPattern). */
+ * flagged hash_pair, in input order. */
 HN_API int hn_copy_keys_if_flags_u32(const void* in_pairs, const uint8_t* flags, size_t n, uint32_t* out,
                                      uint64_t* d_count, void* scratch, size_t scratch_bytes, hn_stream_t s);
 HN_API int hn_copy_keys_if_flags_u64(const void* in_pairs, const uint8_t* flags, size_t n, uint64_t* out,
