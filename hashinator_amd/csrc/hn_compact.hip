@@ -315,13 +315,13 @@ int run_scan(const T* in, T* out, uint64_t n, void* scr, size_t bytes, hipStream
 // hn_copy_if_pairs_noalloc_u64 keeps its look-back state (ticket and tile
 // words) in `scratch`, a file-scope device array sized for the largest
 // input it accepts, HN_NOALLOC_MAX_ELEMS elements (a 2^32-slot table):
-// 2^19 + 1 tiles of 16 bytes, 8 MiB in all, held by every device the
-// library runs on until the process ends.  Host code never names the array
+// the ticket and 2^19 one-word tile states, 4 MiB + 256 bytes, held by
+// every device the library runs on until the process ends.  Host code never names the array
 // in a launch or a copy: its host-side name is not a device address.  The
 // device address is looked up once per device (with that device current)
 // and kept in noalloc_ptr[].
 constexpr uint64_t kNoAllocMaxElems = HN_NOALLOC_MAX_ELEMS;
-constexpr size_t kNoAllocBytes = lookback_bytes(compact_tiles(kNoAllocMaxElems) + 1);
+constexpr size_t kNoAllocBytes = sc::scratch_bytes(kNoAllocMaxElems);
 static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
 
 constexpr int kMaxDevices = 64;

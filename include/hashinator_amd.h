@@ -378,7 +378,7 @@ HN_API int hn_copy_keys_if_pairs_u64(const void* in, size_t n, const hn_pred* pr
 /* No counterpart in the reference: for callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc).  hn_copy_if_pairs_u64 without the scratch argument: the
- * look-back state lives in a static device array of the library (8 MiB per
+ * look-back state lives in a static device array of the library (4 MiB per
  * device, held until the process ends; no hipMalloc, hipFree or allocating
  * library call).  Accepts n <= HN_NOALLOC_MAX_ELEMS (else HN_ESCRATCH, before
  * any launch).  Every call on a device shares that device's array: calls that
