@@ -158,3 +158,89 @@ def test_fullsize_u32_probing_insert(gpu):
     assert np.array_equal(rows, exp)
     del hm, dk, dv, out, hit
     torch.cuda.empty_cache()
+
+
+def test_fullsize_merge_equals_probing_insert_then_reorder(big):
+    """The merge at its full size: 120 M keys (5% of them keys already in the
+    map, with new values; no key twice, so the probing insert's result does
+    not depend on which of two copies wins) into the 2^28-slot table holding
+    2^27 keys in Robin Hood order.  Its table equals, slot for slot, the
+    probing insert of the same batch into a copy of the table followed by the
+    reorder -- the path the image tests pin to the oracle at smaller sizes --
+    with the same fill and overflow; every key of the union retrieves its
+    last value; the old table is only read."""
+    n, dk, dv, da, perm = big
+    L = _lib.load()
+    N = 1 << SP
+    n_old, n_new = N // 2, 120_000_000
+    n_ow = n_new // 20
+    ok_, ov_ = dk[:n_old], dv[:n_old]
+    rng = np.random.default_rng(2028)
+    ow = rng.choice(n_old, n_ow, replace=False)  # indices of present keys the batch overwrites
+    hk = np.concatenate([da[: n_new - n_ow].cpu().numpy(), ok_.cpu().numpy()[ow]])
+    order = rng.permutation(n_new)
+    hk = hk[order]
+    hv = (hk ^ np.int64(0x5BD1E9955BD1E995)) + 1  # new values, not the keys' old ones
+    nk = torch.from_numpy(hk).cuda()
+    nv = torch.from_numpy(hv).cuda()
+
+    old = torch.empty((N, 2), dtype=torch.int64, device="cuda")
+    info0 = torch.zeros(5, dtype=torch.int64, device="cuda")
+    _lib.check(L.hn_info_init(info0.data_ptr(), SP, None), "info_init")
+    bsb = L.hn_build_scratch_bytes_u64(SP, n_old)
+    bs = torch.empty(bsb, dtype=torch.uint8, device="cuda")
+    _lib.check(L.hn_build_u64(ok_.data_ptr(), ov_.data_ptr(), old.data_ptr(), info0.data_ptr(), n_old, SP,
+                              bs.data_ptr(), bsb, None, None), "build")
+    torch.cuda.synchronize()
+    del bs
+    assert (int(info0[4].item()) >> 32) == _lib.HN_INFO_ORDERED
+    old_copy = old.clone()
+
+    # the merge, into a second array
+    nb = torch.empty_like(old)
+    inf_m = info0.clone()
+    msb = L.hn_merge_scratch_bytes_u64(SP, n_new)
+    ms = torch.empty(msb, dtype=torch.uint8, device="cuda")
+    _lib.check(L.hn_merge_u64(nk.data_ptr(), nv.data_ptr(), old.data_ptr(), nb.data_ptr(), inf_m.data_ptr(), n_new,
+                              SP, ms.data_ptr(), msb, None), "merge")
+    torch.cuda.synchronize()
+    del ms
+    im = inf_m.cpu().numpy()
+    assert (int(im[1]), int(im[3])) == (n_old + n_new - n_ow, 0)
+    assert (int(im[4]) & 0xFFFFFFFF) == 0 and (int(im[4]) >> 32) == _lib.HN_INFO_ORDERED
+
+    # the probing insert of the same batch into a copy, then the reorder
+    b2 = old.clone()
+    inf_p = info0.clone()
+    _lib.check(L.hn_insert_u64(nk.data_ptr(), nv.data_ptr(), b2.data_ptr(), inf_p.data_ptr(), n_new, None, None),
+               "insert")
+    rsb = L.hn_reorder_scratch_bytes(SP)
+    rs = torch.empty(max(rsb, 1), dtype=torch.uint8, device="cuda")
+    _lib.check(L.hn_reorder_u64(b2.data_ptr(), inf_p.data_ptr(), SP, rs.data_ptr(), rsb, None), "reorder")
+    torch.cuda.synchronize()
+    del rs
+    ip = inf_p.cpu().numpy()
+    assert (int(ip[1]), int(ip[2]), int(ip[3])) == (int(im[1]), int(im[2]), int(im[3]))
+
+    # slot for slot: keys everywhere, values where a key lives
+    for c0 in range(0, N, CHUNK):
+        a, b = nb[c0:c0 + CHUNK], b2[c0:c0 + CHUNK]
+        assert torch.equal(a[:, 0], b[:, 0]), c0
+        live = a[:, 0] != -1
+        assert torch.equal(torch.where(live, a[:, 1], 0), torch.where(live, b[:, 1], 0)), c0
+        assert torch.equal(old[c0:c0 + CHUNK], old_copy[c0:c0 + CHUNK]), c0
+    del b2, old_copy
+
+    # every key of the union retrieves its last value
+    exp_old = ov_.cpu().numpy().copy()
+    exp_old[ow] = hv[np.argsort(order)][n_new - n_ow:]
+    for q, e in ((ok_, torch.from_numpy(exp_old).cuda()), (nk, nv)):
+        out = torch.full_like(q, MISS)
+        hit = torch.empty(q.numel(), dtype=torch.uint8, device="cuda")
+        _lib.check(L.hn_retrieve_u64(q.data_ptr(), out.data_ptr(), hit.data_ptr(), nb.data_ptr(), inf_m.data_ptr(),
+                                     q.numel(), None, None), "retrieve")
+        torch.cuda.synchronize()
+        assert bool(hit.bool().all())
+        assert torch.equal(out, e)
+    del nb, old, nk, nv
+    torch.cuda.empty_cache()
