@@ -244,3 +244,53 @@ def test_fullsize_merge_equals_probing_insert_then_reorder(big):
         assert torch.equal(out, e)
     del nb, old, nk, nv
     torch.cuda.empty_cache()
+
+
+def test_fullsize_growth_by_rehash_build(gpu):
+    """Growth at full size: a 2^27-slot map at load factor 0.85 with a fifth
+    of its keys erased (tombstones in place) grown to 2^28 slots by the bulk
+    rehash (hn_rehash_build_u64, the old table as the partition's source,
+    live slots only).  The new table equals, slot for slot, the bulk build of
+    the old map's live pairs into an empty 2^28 table (the Robin Hood image
+    is a function of the key set), with the same fill and overflow."""
+    L = _lib.load()
+    old_sp, new_sp = SP - 1, SP
+    n = int(0.85 * (1 << old_sp))
+    seed = W.SEED_BASE + 5
+    keys = W.mixed64(n, seed)
+    dk = torch.from_numpy(keys.view(np.int64)).cuda()
+    dv = torch.from_numpy(W.values(keys, seed).view(np.int64)).cuda()
+    er = torch.from_numpy(keys[np.random.default_rng(5).choice(n, n // 5, replace=False)].view(np.int64)).cuda()
+    del keys
+    hm = Hashmap(old_sp, 64)
+    hm.insert(dk, dv, targetLF=1.0)
+    hm.erase(er)
+    i = hm.info()
+    assert (i.fill, i.tombstone_counter) == (n - n // 5, n // 5)
+    live = hm.extractPattern()  # the live pairs, in bucket order
+    assert live.shape[0] == n - n // 5
+    del dk, dv, er
+
+    nb = torch.empty((1 << new_sp, 2), dtype=torch.int64, device="cuda")
+    sb = L.hn_rehash_build_scratch_bytes_u64(old_sp, new_sp)
+    sc = torch.empty(sb, dtype=torch.uint8, device="cuda")
+    rc = L.hn_rehash_build_u64(hm.bucket_ptr(), old_sp, nb.data_ptr(), new_sp, hm.info_ptr(), sc.data_ptr(), sb, None)
+    torch.cuda.synchronize()
+    assert rc == 0
+    inf = hm._info.cpu().numpy()
+    del sc, hm
+
+    ref = Hashmap(new_sp, 64)
+    ref.insert(live[:, 0].contiguous(), live[:, 1].contiguous(), targetLF=1.0)  # the bulk build
+    r = ref.info()
+    assert ref.ordered() and (r.fill, r.tombstone_counter, r.err) == (n - n // 5, 0, 0)
+    assert (int(inf[0]), int(inf[1]), int(inf[2]), int(inf[3])) == (new_sp, r.fill, r.current_max_bucket_overflow, 0)
+    assert (int(inf[4]) & 0xFFFFFFFF) == 0 and (int(inf[4]) >> 32) == _lib.HN_INFO_ORDERED
+    b = ref.buckets
+    for c0 in range(0, 1 << new_sp, CHUNK):
+        x, y = nb[c0:c0 + CHUNK], b[c0:c0 + CHUNK]
+        assert torch.equal(x[:, 0], y[:, 0]), c0
+        lv = x[:, 0] != -1
+        assert torch.equal(torch.where(lv, x[:, 1], 0), torch.where(lv, y[:, 1], 0)), c0
+    del nb, ref, live
+    torch.cuda.empty_cache()
