@@ -106,7 +106,8 @@ struct BuildPlan {
    uint32_t nblocks;    // partition blocks
    uint32_t tile;       // pairs per scatter tile (kPartTileS or kPartTileL)
    uint32_t sub;        // tiles per partition block
-   size_t off_pairs, off_mat, off_scan, off_start, off_word, off_spec, off_carry, off_mark, off_list, off_blk, total;
+   size_t off_pairs, off_mat, off_scan, off_start, off_word, off_spec, off_carry, off_mark, off_list, off_blk, off_dig,
+       total;
 };
 
 inline size_t align256(size_t x) { return (x + 255) / 256 * 256; }
@@ -166,6 +167,11 @@ bool build_plan(int sp, uint64_t n, int dcap, BuildPlan* P) {
    o = align256(o + P->NR * sizeof(uint32_t));
    P->off_blk = o;
    o = align256(o + ((P->NR + 1023) / 1024) * sizeof(RegionSum));
+   // One byte per pair: the next pass's digit, written by a scatter pass
+   // whose successor's digit fits a byte (reserved for every plan of more
+   // than one region, so the size does not depend on the digit width).
+   P->off_dig = o;
+   if (P->NR > 1) o = align256(o + n);
    P->total = o;
    return true;
 }
@@ -239,10 +245,13 @@ __device__ __forceinline__ bool live_key(K k) {
 // Per-block digit counts, written digit-major: mat[d * nblocks + block].  A
 // block covers `sub` consecutive tiles of `tile` pairs, which keeps the
 // count matrix -- and its scan -- small.
-template <typename K, int SRC, bool FILTER>
+// DIG: the digits come from the byte array the previous scatter pass wrote
+// (one byte per pair, 16 per load) instead of from the pairs' keys.
+template <typename K, int SRC, bool FILTER, bool DIG = false>
 __global__ __launch_bounds__(kPartBS) void k_part_count(PartIn<K> in, uint64_t n_in, const uint64_t* d_n, uint32_t* mat,
                                                         uint32_t nblocks, uint32_t tile, uint32_t nsub, int sp,
-                                                        int rlog, int dshift, int dbits, BuildHeader* h) {
+                                                        int rlog, int dshift, int dbits, BuildHeader* h,
+                                                        const uint8_t* dig) {
    __shared__ uint32_t hist[1 << kMaxDigitBits];
    __shared__ uint32_t live_s;
    if (!h->go) return;
@@ -255,11 +264,28 @@ __global__ __launch_bounds__(kPartBS) void k_part_count(PartIn<K> in, uint64_t n
    const uint64_t t0 = static_cast<uint64_t>(blockIdx.x) * super;
    const uint64_t t1 = std::min<uint64_t>(t0 + super, n);
    uint32_t live = 0;
-   for (uint64_t e = t0 + threadIdx.x; e < t1; e += kPartBS) {
-      const K key = part_key<K, SRC>(in, e);
-      if (FILTER && !live_key(key)) continue;
-      atomicAdd(&hist[(region_of(key, sp, rlog) >> dshift) & dmask], 1u);
-      ++live;
+   if constexpr (DIG) {
+      static_assert(!FILTER, "digit bytes follow a pass that kept only live pairs");
+      // t0 is a multiple of the block's span, so the 16-byte loads are aligned.
+      for (uint64_t e = t0 + 16ull * threadIdx.x; e < t1; e += 16ull * kPartBS) {
+         if (e + 16 <= t1) {
+            const uint4 w = *reinterpret_cast<const uint4*>(dig + e);
+            const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+               for (int b = 0; b < 4; ++b) atomicAdd(&hist[(ws[q] >> (8 * b)) & 0xFFu], 1u);
+         } else {
+            for (uint64_t x = e; x < t1; ++x) atomicAdd(&hist[dig[x]], 1u);
+         }
+      }
+   } else {
+      for (uint64_t e = t0 + threadIdx.x; e < t1; e += kPartBS) {
+         const K key = part_key<K, SRC>(in, e);
+         if (FILTER && !live_key(key)) continue;
+         atomicAdd(&hist[(region_of(key, sp, rlog) >> dshift) & dmask], 1u);
+         ++live;
+      }
    }
    if (FILTER) {
       live = wave_sum(live);
@@ -283,7 +309,8 @@ template <typename K, int SRC, bool FILTER, int BS, int ROUNDS>
 __global__ __launch_bounds__(BS) void k_part_scatter(PartIn<K> in, uint64_t n_in, const uint64_t* d_n,
                                                           Slot<K>* out, const uint32_t* offs, uint32_t nblocks,
                                                           uint32_t nsub, int sp, int rlog, int dshift, int dbits,
-                                                          const BuildHeader* h) {
+                                                          const BuildHeader* h, uint8_t* dig, int ndshift,
+                                                          uint32_t nmask) {
    constexpr int NW = BS / kWave;
    constexpr int TILE = BS * ROUNDS;
    constexpr uint32_t DMAX = 1u << kMaxDigitBits;
@@ -399,8 +426,11 @@ __global__ __launch_bounds__(BS) void k_part_scatter(PartIn<K> in, uint64_t n_in
       for (int w = 0; w < NW; ++w) cnt += wsum_s[w];
       for (uint32_t p = threadIdx.x; p < cnt; p += BS) {
          const Slot<K> e = tile_s[p];
-         const uint32_t d = (region_of(e.key, sp, rlog) >> dshift) & dmask; // recomputed: cheaper than LDS for it
-         out[static_cast<uint64_t>(goff_s[d]) + (p - lstart_s[d])] = e;
+         const uint32_t rg = region_of(e.key, sp, rlog);
+         const uint32_t d = (rg >> dshift) & dmask; // recomputed: cheaper than LDS for it
+         const uint64_t pos = static_cast<uint64_t>(goff_s[d]) + (p - lstart_s[d]);
+         out[pos] = e;
+         if (nmask) dig[pos] = static_cast<uint8_t>((rg >> ndshift) & nmask); // the next pass's digit
       }
       __syncthreads();
 #pragma unroll
@@ -1041,24 +1071,42 @@ __global__ void k_rehash_info(hn_info* info, int new_sp, BuildHeader* h) {
    info->reserved1 = 0;
 }
 
+// A pass's digits come from the byte array when the pass before it wrote
+// them: a pass after the first whose digit fits a byte.
+inline bool digits_in_bytes(const BuildPlan& P, int pass) { return pass > 0 && pass < P.passes && P.dbits[pass] <= 8; }
+
 template <typename K, int SRC, bool FILTER>
 int launch_part(const BuildPlan& P, int pass, PartIn<K> in, uint64_t n_in, const uint64_t* d_n, Slot<K>* out,
-                uint32_t* mat, void* scan_scratch, size_t scan_bytes, BuildHeader* h, hipStream_t s) {
+                uint32_t* mat, void* scan_scratch, size_t scan_bytes, BuildHeader* h, uint8_t* dig, hipStream_t s) {
    int dshift = 0;
    for (int i = 0; i < pass; ++i) dshift += P.dbits[i];
    const int dbits = P.dbits[pass];
    const uint64_t D = 1ull << dbits;
-   hipLaunchKernelGGL((k_part_count<K, SRC, FILTER>), dim3(P.nblocks), dim3(kPartBS), 0, s, in, n_in, d_n, mat,
-                      P.nblocks, P.tile, P.sub, P.sp, P.rlog, dshift, dbits, h);
+   const int ndshift = dshift + dbits;
+   const uint32_t nmask = digits_in_bytes(P, pass + 1) ? (1u << P.dbits[pass + 1]) - 1u : 0u;
+   if constexpr (SRC == SRC_TABLE && !FILTER) {
+      if (digits_in_bytes(P, pass)) {
+         hipLaunchKernelGGL((k_part_count<K, SRC, FILTER, true>), dim3(P.nblocks), dim3(kPartBS), 0, s, in, n_in,
+                            d_n, mat, P.nblocks, P.tile, P.sub, P.sp, P.rlog, dshift, dbits, h, dig);
+      } else {
+         hipLaunchKernelGGL((k_part_count<K, SRC, FILTER>), dim3(P.nblocks), dim3(kPartBS), 0, s, in, n_in, d_n,
+                            mat, P.nblocks, P.tile, P.sub, P.sp, P.rlog, dshift, dbits, h, dig);
+      }
+   } else {
+      hipLaunchKernelGGL((k_part_count<K, SRC, FILTER>), dim3(P.nblocks), dim3(kPartBS), 0, s, in, n_in, d_n, mat,
+                         P.nblocks, P.tile, P.sub, P.sp, P.rlog, dshift, dbits, h, dig);
+   }
    int rc = hn_exclusive_scan_u32(mat, mat, D * P.nblocks, scan_scratch, scan_bytes, reinterpret_cast<hn_stream_t>(s));
    if (rc) return rc;
    static_assert(kPartBSL * kPartRoundsL == kPartTileL && kPartBSS * kPartRoundsS == kPartTileS, "scatter tiles");
    if (P.tile == kPartTileL)
       hipLaunchKernelGGL((k_part_scatter<K, SRC, FILTER, kPartBSL, kPartRoundsL>), dim3(P.nblocks), dim3(kPartBSL), 0,
-                         s, in, n_in, d_n, out, mat, P.nblocks, P.sub, P.sp, P.rlog, dshift, dbits, h);
+                         s, in, n_in, d_n, out, mat, P.nblocks, P.sub, P.sp, P.rlog, dshift, dbits, h, dig, ndshift,
+                         nmask);
    else
       hipLaunchKernelGGL((k_part_scatter<K, SRC, FILTER, kPartBSS, kPartRoundsS>), dim3(P.nblocks), dim3(kPartBSS), 0,
-                         s, in, n_in, d_n, out, mat, P.nblocks, P.sub, P.sp, P.rlog, dshift, dbits, h);
+                         s, in, n_in, d_n, out, mat, P.nblocks, P.sub, P.sp, P.rlog, dshift, dbits, h, dig, ndshift,
+                         nmask);
    return static_cast<int>(hipGetLastError());
 }
 
@@ -1085,6 +1133,7 @@ int partition_regions(const BuildPlan& P, int src0, PartIn<K> in0, uint64_t n_in
    void* scan_scratch = base + P.off_scan;
    const size_t scan_bytes = P.off_start - P.off_scan;
    uint64_t* start = reinterpret_cast<uint64_t*>(base + P.off_start);
+   uint8_t* dig = reinterpret_cast<uint8_t*>(base + P.off_dig);
    RegionLB lb;
    lb.word = reinterpret_cast<uint64_t*>(base + P.off_word);
    lb.spec = reinterpret_cast<int64_t*>(base + P.off_spec);
@@ -1098,21 +1147,22 @@ int partition_regions(const BuildPlan& P, int src0, PartIn<K> in0, uint64_t n_in
       if (pass == 0) {
          switch (src0) {
          case SRC_PAIRS:
-            rc = launch_part<K, SRC_PAIRS, false>(P, 0, in0, n_in, nullptr, dst, mat, scan_scratch, scan_bytes, h, s);
+            rc = launch_part<K, SRC_PAIRS, false>(P, 0, in0, n_in, nullptr, dst, mat, scan_scratch, scan_bytes, h, dig, s);
             break;
          case SRC_SOA:
-            rc = launch_part<K, SRC_SOA, false>(P, 0, in0, n_in, nullptr, dst, mat, scan_scratch, scan_bytes, h, s);
+            rc = launch_part<K, SRC_SOA, false>(P, 0, in0, n_in, nullptr, dst, mat, scan_scratch, scan_bytes, h, dig, s);
             break;
          case SRC_INDEX:
-            rc = launch_part<K, SRC_INDEX, false>(P, 0, in0, n_in, nullptr, dst, mat, scan_scratch, scan_bytes, h, s);
+            rc = launch_part<K, SRC_INDEX, false>(P, 0, in0, n_in, nullptr, dst, mat, scan_scratch, scan_bytes, h, dig, s);
             break;
          default:
-            rc = launch_part<K, SRC_TABLE, true>(P, 0, in0, n_in, nullptr, dst, mat, scan_scratch, scan_bytes, h, s);
+            rc = launch_part<K, SRC_TABLE, true>(P, 0, in0, n_in, nullptr, dst, mat, scan_scratch, scan_bytes, h, dig,
+                                                 s);
             break;
          }
       } else {
          PartIn<K> in{nullptr, nullptr, src};
-         rc = launch_part<K, SRC_TABLE, false>(P, pass, in, 0, &h->n, dst, mat, scan_scratch, scan_bytes, h, s);
+         rc = launch_part<K, SRC_TABLE, false>(P, pass, in, 0, &h->n, dst, mat, scan_scratch, scan_bytes, h, dig, s);
       }
       if (rc) return rc;
       src = dst;

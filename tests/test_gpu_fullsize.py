@@ -294,3 +294,41 @@ def test_fullsize_growth_by_rehash_build(gpu):
         assert torch.equal(torch.where(lv, x[:, 1], 0), torch.where(lv, y[:, 1], 0)), c0
     del nb, ref, live
     torch.cuda.empty_cache()
+
+
+def test_build_with_nine_bit_second_digit_u32(gpu):
+    """2^29 u32 slots: 18 region bits, two partition passes of 9-bit digits,
+    so the second pass counts its digits from the pairs (a digit that fits a
+    byte comes from the byte array the first pass writes; 2^28 slots and less
+    take that path).  The bulk build of 0.6 of the slots equals the probing
+    insert followed by the reorder slot for slot, with the same Info, and
+    every key retrieves its value."""
+    sp = 29
+    n = int(0.6 * (1 << sp))
+    k = (np.arange(n, dtype=np.uint64) * np.uint64(2654435761)) & np.uint64(0xFFFFFFFF)  # distinct: odd multiplier
+    k = k[k < np.uint64(0xFFFFFFFE)].astype(np.uint32)  # no sentinel keys
+    n = k.size
+    dk = torch.from_numpy(k.view(np.int32)).cuda()
+    dv = torch.from_numpy((k ^ np.uint32(0x5BD1E995)).view(np.int32)).cuda()
+    del k
+    hb = Hashmap(sp, 32)
+    hb.insert(dk, dv, targetLF=1.0)  # the bulk build
+    hp = Hashmap(sp, 32, auto_build=0)
+    hp.insert(dk, dv, targetLF=1.0)  # the probing insert, then the reorder
+    ib, ip = hb.info(), hp.info()
+    assert hb.ordered() and hp.ordered()
+    assert (ib.fill, ib.tombstone_counter, ib.err) == (n, 0, 0)
+    assert (ip.fill, ip.current_max_bucket_overflow, ip.tombstone_counter) == (
+        ib.fill, ib.current_max_bucket_overflow, ib.tombstone_counter)
+    for c0 in range(0, 1 << sp, CHUNK):
+        x, y = hb.buckets[c0:c0 + CHUNK], hp.buckets[c0:c0 + CHUNK]
+        assert torch.equal(x[:, 0], y[:, 0]), c0
+        lv = x[:, 0] != -1
+        assert torch.equal(torch.where(lv, x[:, 1], 0), torch.where(lv, y[:, 1], 0)), c0
+    del hp
+    out = torch.zeros_like(dk)
+    hit = torch.empty(n, dtype=torch.uint8, device="cuda")
+    hb.retrieve(dk, out, hit)
+    assert bool(hit.bool().all()) and torch.equal(out, dv)
+    del hb, dk, dv, out, hit
+    torch.cuda.empty_cache()
