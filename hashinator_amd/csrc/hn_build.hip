@@ -280,11 +280,22 @@ __global__ __launch_bounds__(kPartBS) void k_part_count(PartIn<K> in, uint64_t n
          }
       }
    } else {
-      for (uint64_t e = t0 + threadIdx.x; e < t1; e += kPartBS) {
-         const K key = part_key<K, SRC>(in, e);
-         if (FILTER && !live_key(key)) continue;
-         atomicAdd(&hist[(region_of(key, sp, rlog) >> dshift) & dmask], 1u);
-         ++live;
+      // Four keys loaded before their four counts: more loads in flight per thread.
+      constexpr int U = 4;
+      for (uint64_t e0 = t0 + threadIdx.x; e0 < t1; e0 += U * kPartBS) {
+         K key[U];
+#pragma unroll
+         for (int u = 0; u < U; ++u) {
+            const uint64_t e = e0 + static_cast<uint64_t>(u) * kPartBS;
+            if (e < t1) key[u] = part_key<K, SRC>(in, e);
+         }
+#pragma unroll
+         for (int u = 0; u < U; ++u) {
+            const uint64_t e = e0 + static_cast<uint64_t>(u) * kPartBS;
+            if (e >= t1 || (FILTER && !live_key(key[u]))) continue;
+            atomicAdd(&hist[(region_of(key[u], sp, rlog) >> dshift) & dmask], 1u);
+            ++live;
+         }
       }
    }
    if (FILTER) {
