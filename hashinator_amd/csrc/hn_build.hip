@@ -112,6 +112,11 @@ struct BuildPlan {
 
 inline size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 
+// The scratch layouts place their arrays at 256-byte offsets from the start
+// and read them with 16-byte loads, so the start must be 256-byte aligned
+// (hipMalloc's and hipMallocAsync's are).
+inline bool scratch_aligned(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 255u) == 0; }
+
 // dcap: widest partition digit (1..9 bits); narrower digits mean more passes.
 template <typename K>
 bool build_plan(int sp, uint64_t n, int dcap, BuildPlan* P) {
@@ -1232,6 +1237,7 @@ int build(const void* keys, const void* vals, const void* pairs, void* buckets, 
    BuildPlan P;
    if (!build_plan<K>(sp, len, dcap, &P)) return HN_EINVAL;
    if (!scratch || bytes < P.total) return HN_ESCRATCH;
+   if (!scratch_aligned(scratch)) return HN_EINVAL;
    PartIn<K> in{static_cast<const K*>(keys), static_cast<const K*>(vals), static_cast<const Slot<K>*>(pairs)};
    const int src0 = pairs ? SRC_PAIRS : (vals ? SRC_SOA : SRC_INDEX);
    return build_core<K>(P, BUILD_EMPTY, src0, in, len, buckets, info, scratch, s);
@@ -1270,6 +1276,7 @@ int rehash_build(const void* old_b, int old_sp, void* new_b, int new_sp, hn_info
    const char* n0 = static_cast<const char*>(new_b);
    if (o0 < n0 + P.N * sizeof(Slot<K>) && n0 < o0 + (1ull << old_sp) * sizeof(Slot<K>)) return HN_EINVAL;
    if (!scratch || bytes < P.total) return HN_ESCRATCH;
+   if (!scratch_aligned(scratch)) return HN_EINVAL;
    hipLaunchKernelGGL(k_rehash_info, dim3(1), dim3(64), 0, s, info, new_sp, static_cast<BuildHeader*>(scratch));
    PartIn<K> in{nullptr, nullptr, static_cast<const Slot<K>*>(old_b)};
    return build_core<K>(P, BUILD_REHASH, SRC_TABLE, in, 1ull << old_sp, new_b, info, scratch, s);
@@ -1855,6 +1862,7 @@ int merge(const void* keys, const void* vals, const void* pairs, const void* old
    MergePlan<K> M;
    if (!merge_plan<K>(sp, len, &M)) return HN_EINVAL;
    if (!scratch || bytes < M.total) return HN_ESCRATCH;
+   if (!scratch_aligned(scratch)) return HN_EINVAL;
    const BuildPlan& P = M.P;
    char* base = static_cast<char*>(scratch);
    BuildHeader* h = reinterpret_cast<BuildHeader*>(base);

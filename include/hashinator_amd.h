@@ -277,7 +277,9 @@ HN_API int hn_reorder_u64(void* buckets, hn_info* info, int size_power, void* sc
  * and no atomics on the table: a stable radix partition by 2048-slot region,
  * an LDS sort per region and a carry scan (about 10 streaming passes' worth
  * of the pair array).  Needs len < 2^size_power and `scratch` of
- * hn_build_scratch_bytes_*(size_power, len) bytes; the table's previous slot
+ * hn_build_scratch_bytes_*(size_power, len) bytes, 256-byte aligned as
+ * hipMalloc gives (else HN_EINVAL; the same holds for the scratch of
+ * hn_rehash_build_* and hn_merge_*); the table's previous slot
  * contents are overwritten, every slot: the bucket memory of a fresh table
  * need not be initialised (hn_table_init_*) first, only its Info
  * (hn_info_init).  When the table is not empty the call sets

@@ -404,3 +404,47 @@ def test_build_invalid_args(gpu):
                           None) == _lib.HN_ESCRATCH
     assert L.hn_build_u64(k.data_ptr(), None, b.data_ptr(), inf.data_ptr(), 0, 10, sc.data_ptr(), sb, None,
                           None) == 0
+
+
+def test_build_rejects_misaligned_scratch(gpu):
+    """Scratch that does not start on a 256-byte boundary is refused before
+    any launch (HN_EINVAL), for the build, the rehash build and the merge;
+    the tables and Info stay as they were."""
+    L = _lib.load()
+    sp = 12
+    N = 1 << sp
+    keys = keyset(N // 2, 64, seed=5)
+    b = torch.empty((N, 2), dtype=torch.int64, device="cuda")
+    _lib.check(L.hn_table_init_u64(b.data_ptr(), N, None), "init")
+    inf = torch.zeros(5, dtype=torch.int64, device="cuda")
+    _lib.check(L.hn_info_init(inf.data_ptr(), sp, None), "info")
+    dk = dev(keys, 64)
+    sb = L.hn_build_scratch_bytes_u64(sp, keys.size)
+    sc = torch.empty(sb + 256, dtype=torch.uint8, device="cuda")
+    assert sc.data_ptr() % 256 == 0
+    img0, inf0 = b.clone(), inf.clone()
+    rc = L.hn_build_u64(dk.data_ptr(), dk.data_ptr(), b.data_ptr(), inf.data_ptr(), keys.size, sp, sc.data_ptr() + 16,
+                        sb, None, None)
+    torch.cuda.synchronize()
+    assert rc == _lib.HN_EINVAL and torch.equal(b, img0) and torch.equal(inf, inf0)
+    # the same table built with aligned scratch, then grown and merged into with misaligned scratch
+    rc = L.hn_build_u64(dk.data_ptr(), dk.data_ptr(), b.data_ptr(), inf.data_ptr(), keys.size, sp, sc.data_ptr(), sb,
+                        None, None)
+    torch.cuda.synchronize()
+    assert rc == 0
+    img1, inf1 = b.clone(), inf.clone()
+    nb = torch.empty((2 * N, 2), dtype=torch.int64, device="cuda")
+    rsb = L.hn_rehash_build_scratch_bytes_u64(sp, sp + 1)
+    rs = torch.empty(rsb + 256, dtype=torch.uint8, device="cuda")
+    rc = L.hn_rehash_build_u64(b.data_ptr(), sp, nb.data_ptr(), sp + 1, inf.data_ptr(), rs.data_ptr() + 128, rsb,
+                               None)
+    torch.cuda.synchronize()
+    assert rc == _lib.HN_EINVAL and torch.equal(inf, inf1)
+    nk = dev(keyset(N // 8, 64, seed=6) | np.uint64(1 << 62), 64)
+    fresh = torch.empty_like(b)
+    msb = L.hn_merge_scratch_bytes_u64(sp, nk.numel())
+    ms = torch.empty(msb + 256, dtype=torch.uint8, device="cuda")
+    rc = L.hn_merge_u64(nk.data_ptr(), nk.data_ptr(), b.data_ptr(), fresh.data_ptr(), inf.data_ptr(), nk.numel(), sp,
+                        ms.data_ptr() + 64, msb, None)
+    torch.cuda.synchronize()
+    assert rc == _lib.HN_EINVAL and torch.equal(b, img1) and torch.equal(inf, inf1)
