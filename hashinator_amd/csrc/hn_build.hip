@@ -31,10 +31,12 @@
 //      carry was wrong, or whose range wraps past the table's end, are
 //      placed again.
 //
-// Traffic per key: each partition pass reads the key for its count, then
-// reads and writes the 16 (u64) or 8 (u32) byte pair; the region step reads
-// the pair once more and the table is written once.  No atomics on the
-// table, no probing.
+// Traffic per key: the first partition pass reads the key for its count,
+// each pass reads and writes the 16 (u64) or 8 (u32) byte pair, and a pass
+// whose successor's digit fits a byte also writes that digit at the pair's
+// output position, so the successor counts from one byte per pair instead of
+// the pair; the region step reads the pair once more and the table is
+// written once.  No atomics on the table, no probing.
 //
 // The table's memory is the output of the first partition pass when there
 // are two, so the scratch holds one pair array, not two.  A region holding
